@@ -1,0 +1,165 @@
+/*
+ * pyexner_hip.h - C-ABI of the MI355X (gfx950) backend for PyExner's per-step
+ * finite-volume hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never
+ * allocates device memory, never synchronises the host, and returns an int
+ * status (0 = success; >0 = hipError_t of the failing launch; <0 = PXH_E_*).
+ * All work is ordered on the stream passed in (a hipStream_t passed as void*;
+ * NULL = the default stream), so a sequence of calls can be captured into a
+ * hipGraph.  Arrays are (ny, nx) row-major and contiguous, x along the
+ * fastest axis, +y towards decreasing row index.  Field data is float
+ * (PXH_F32) or double (PXH_F64) as grid->dtype says; masks are uint8.
+ *
+ * Reference interfaces replaced (paths relative to src/PyExner/):
+ *   step_fn   "Roe Exner"  solvers/roe_exner_solver.py:80-116  -> pxh_roe_exner_step
+ *   compute_dt_fn "Roe Exner" solvers/roe_exner_solver.py:118-122 -> pxh_roe_exner_dt
+ *   init_fn   "Roe Exner"  solvers/roe_exner_solver.py:58-78   -> pxh_roe_exner_init (+ pxh_bc_apply)
+ *   step_fn   "Roe"        solvers/roe_solver.py:40-54         -> pxh_roe_step
+ *   compute_dt_fn "Roe"    solvers/roe_solver.py:56-60         -> pxh_roe_dt
+ *   BoundaryManager.apply  domain/boundary_registry.py:183-187 -> pxh_bc_apply
+ *   SSPRK2 combine          integrators/SSPRK2.py:40            -> pxh_ssprk2_combine
+ */
+#ifndef PYEXNER_HIP_H
+#define PYEXNER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXH_F32 0
+#define PXH_F64 1
+
+#define PXH_OK 0
+#define PXH_E_ARG (-1)      /* invalid argument (null pointer, bad size, bad dtype) */
+#define PXH_E_RANGE (-2)    /* size beyond a compiled-in limit */
+#define PXH_E_UNSUPPORTED (-3)
+
+#define PXH_MAX_SED 8
+#define PXH_MAX_HANDLERS 16
+#define PXH_MAX_CHAIN 6
+
+/* Grid geometry. */
+typedef struct {
+    int32_t ny, nx;   /* array shape, ghost ring included */
+    int32_t dtype;    /* PXH_F32 or PXH_F64 */
+    int32_t reserved;
+    double dx;        /* cell size (dx == dy) */
+} pxh_grid;
+
+/* Device pointers to the structure-of-arrays state (state/roe_exner_state.py:8-48;
+ * the plain Roe state uses h, hu, hv, z, n only).  Each is (ny, nx). */
+typedef struct {
+    void *h, *hu, *hv, *z, *z_b, *n, *n_b, *G;
+} pxh_fields;
+
+/* Sediment table, precomputed on the host in the working dtype from
+ * seds = [frac, d50, density, k_d, k_e, theta_c, porosity] (io/pnetcdf_reader.py:146-158):
+ *   A = np^2, B = (s-1)*d50, C = np^3*sqrt(g), D = s-1, with s = density/1000,
+ *   np = d50^(1/6)/21.1 (solvers/kernels/roe_exner.py:95-116);
+ *   one_minus_por = 1 - mean(porosity); nb_sed = mean(d50*frac)^(1/6)/21.1 (roe_exner.py:64-73). */
+typedef struct {
+    int32_t n_sed;
+    int32_t mpm;              /* 1: Meyer-Peter-Mueller G; 0: constant G (grass_factor number) */
+    double one_minus_por;
+    double nb_sed;
+    double frac[PXH_MAX_SED], A[PXH_MAX_SED], B[PXH_MAX_SED], C[PXH_MAX_SED],
+           D[PXH_MAX_SED], theta_c[PXH_MAX_SED];
+} pxh_sediments;
+
+/* Composed ghost-cell program (domain/boundary_registry.py:183-187 applied in
+ * YAML order, resolved on the host into one gather/scatter per target cell). */
+#define PXH_BC_COPY 0          /* Transmissive, TransmissiveBed, Periodic */
+#define PXH_BC_REFLECT_RX 1    /* domain/boundaries/roeexner_reflective.py:31-59 */
+#define PXH_BC_REFLECT_ROE 2   /* domain/boundaries/roe_reflective.py:34-39 */
+
+#define PXH_G_H 0
+#define PXH_G_HUV 1
+#define PXH_G_Z 2
+#define PXH_G_ZB 3
+#define PXH_G_N 4
+#define PXH_G_NB 5
+#define PXH_G_G 6
+
+typedef struct {
+    int32_t target, src;            /* linear cell indices (row*nx + col) */
+    uint8_t group;                  /* PXH_G_* */
+    uint8_t nchain;                 /* handlers applied to src, in order (momentum only) */
+    uint8_t chain[PXH_MAX_CHAIN];
+} pxh_bc_entry;                     /* 16 bytes */
+
+typedef struct {
+    int32_t kind;                   /* PXH_BC_* */
+    int32_t reserved;
+    double nx, ny;                  /* outward normal as given in the YAML */
+} pxh_bc_handler;
+
+typedef struct {
+    const pxh_bc_entry *entries;    /* device array, n_entries long */
+    int32_t n_entries;
+    int32_t n_handlers;
+    uint32_t group_mask;            /* which PXH_G_* groups this call writes (bit i = group i) */
+    int32_t reserved;
+    pxh_bc_handler handlers[PXH_MAX_HANDLERS];
+} pxh_bc;
+
+/* ---- library ------------------------------------------------------------ */
+int pxh_version(void);
+const char *pxh_error_string(int status);
+
+/* ---- Roe-Exner ----------------------------------------------------------- */
+/* init_fn minus the boundary fill: n_b = compute_n (roe_exner.py:64-73) unless
+ * constant mode, then G = compute_G (roe_exner.py:85-144). In place on s. */
+int pxh_roe_exner_init(const pxh_grid *grid, const pxh_fields *s, const uint8_t *mask0,
+                       const pxh_sediments *sed, void *stream);
+
+/* One Forward-Euler step_fn: out = S(in, dt).  out->h, hu, hv, z_b, G must not
+ * alias in's; z, n, n_b are static and may be shared.  dt is read from the
+ * device scalar dt_dev (dtype of the grid).  bc must hold an idempotent program
+ * (sources never targets) covering groups h, hu/hv, z_b and G. */
+int pxh_roe_exner_step(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
+                       const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc,
+                       const void *dt_dev, void *stream);
+
+/* The four kernels step_fn sequences, one call each, for callers that
+ * sequence them themselves (e.g. with halo exchanges in between):
+ *   roe_solve_2D        solvers/kernels/roe_exner.py:535-612  out.{h,hu,hv} from in
+ *   momentum_corrections roe_exner.py:14-62                   in place on out.{hu,hv}; bed from in
+ *   compute_G           roe_exner.py:85-144                   out.G from out.{h,hu,hv} (constant mode: in.G)
+ *   exner_solve_2D      roe_exner.py:665-717                  out.z_b from in.z_b and out.{h,hu,hv,G} */
+int pxh_roe_solve_2D(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out, const uint8_t *mask0,
+                     const void *dt_dev, void *stream);
+int pxh_momentum_corrections(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
+                             const uint8_t *mask0, void *stream);
+int pxh_compute_G(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out, const uint8_t *mask0,
+                  const pxh_sediments *sed, void *stream);
+int pxh_exner_solve_2D(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out, const uint8_t *mask0,
+                       const void *dt_dev, void *stream);
+
+/* compute_dt_fn: *dt_out = max(cfl * min over active edges, 1e-6) written to
+ * device memory.  workspace must hold pxh_dt_workspace_size(grid) bytes. */
+int pxh_roe_exner_dt(const pxh_grid *grid, const pxh_fields *s, const uint8_t *mask0,
+                     double cfl, void *dt_out, void *workspace, void *stream);
+int64_t pxh_dt_workspace_size(const pxh_grid *grid);
+
+/* SSPRK2 combination out = 0.5*a + 0.5*b over h, hu, hv, z_b, G (the static
+ * fields are equal in a and b).  out may alias a or b. */
+int pxh_ssprk2_combine(const pxh_grid *grid, const pxh_fields *a, const pxh_fields *b,
+                       const pxh_fields *out, void *stream);
+
+/* ---- plain Roe ----------------------------------------------------------- */
+int pxh_roe_step(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
+                 const pxh_bc *bc, const void *dt_dev, void *stream);
+int pxh_roe_dt(const pxh_grid *grid, const pxh_fields *s, double cfl, void *dt_out,
+               void *workspace, void *stream);
+
+/* ---- boundaries / halos -------------------------------------------------- */
+/* Apply the composed boundary program in place (groups bc->group_mask). */
+int pxh_bc_apply(const pxh_grid *grid, const pxh_fields *s, const pxh_bc *bc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

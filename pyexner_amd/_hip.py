@@ -1,0 +1,116 @@
+"""ctypes binding of libpyexner_hip.so (include/pyexner_hip.h).
+
+The library is built in-tree by ``pyexner_amd.build.build()`` into
+``pyexner_amd/lib/``.  There is no fallback: if the library is missing or a
+call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpyexner_hip.so")
+
+PXH_F32 = 0
+PXH_F64 = 1
+MAX_SED = 8
+MAX_HANDLERS = 16
+MAX_CHAIN = 6
+
+BC_COPY, BC_REFLECT_RX, BC_REFLECT_ROE = 0, 1, 2
+G_H, G_HUV, G_Z, G_ZB, G_N, G_NB, G_G = range(7)
+GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G_N: ("n",),
+                G_NB: ("n_b",), G_G: ("G",)}
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("dx", ctypes.c_double)]
+
+
+class Fields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")]
+
+
+class Sediments(ctypes.Structure):
+    _fields_ = [("n_sed", ctypes.c_int32), ("mpm", ctypes.c_int32), ("one_minus_por", ctypes.c_double),
+                ("nb_sed", ctypes.c_double)] + [
+        (n, ctypes.c_double * MAX_SED) for n in ("frac", "A", "B", "C", "D", "theta_c")]
+
+
+class BCEntry(ctypes.Structure):
+    _fields_ = [("target", ctypes.c_int32), ("src", ctypes.c_int32), ("group", ctypes.c_uint8),
+                ("nchain", ctypes.c_uint8), ("chain", ctypes.c_uint8 * MAX_CHAIN)]
+
+
+class BCHandler(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("nx", ctypes.c_double),
+                ("ny", ctypes.c_double)]
+
+
+class BC(ctypes.Structure):
+    _fields_ = [("entries", ctypes.c_void_p), ("n_entries", ctypes.c_int32), ("n_handlers", ctypes.c_int32),
+                ("group_mask", ctypes.c_uint32), ("reserved", ctypes.c_int32),
+                ("handlers", BCHandler * MAX_HANDLERS)]
+
+
+assert ctypes.sizeof(BCEntry) == 16
+
+_P = ctypes.POINTER
+_SIGS = {
+    "pxh_version": (ctypes.c_int, []),
+    "pxh_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "pxh_roe_exner_init": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_void_p, _P(Sediments), ctypes.c_void_p]),
+    "pxh_roe_exner_step": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, _P(Sediments), _P(BC),
+                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_roe_exner_dt": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_dt_workspace_size": (ctypes.c_int64, [_P(Grid)]),
+    "pxh_ssprk2_combine": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p]),
+    "pxh_roe_step": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(BC), ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_roe_dt": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p]),
+    "pxh_bc_apply": (ctypes.c_int, [_P(Grid), _P(Fields), _P(BC), ctypes.c_void_p]),
+    "pxh_roe_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "pxh_momentum_corrections": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_compute_G": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, _P(Sediments),
+                                     ctypes.c_void_p]),
+    "pxh_exner_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+def lib_path():
+    return _LIB_PATH
+
+
+def load():
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(f"{_LIB_PATH} is missing: build it with pyexner_amd.build.build() "
+                          "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = ctypes.CDLL(_LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def check(status, what):
+    if status != 0:
+        msg = load().pxh_error_string(status).decode()
+        raise HipError(f"{what} failed with status {status}: {msg}")
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args), name)

@@ -1,0 +1,270 @@
+"""Ghost-cell boundaries: the handler registry and BoundaryManager.
+
+Mirrors domain/boundary_registry.py:11-187.  Handlers are registered under
+``f"{flux_scheme} {type}"`` and constructed from a YAML polygon and normal.  The
+polygons are rasterised on the host (point_in_polygon, boundary_registry.py:19-44)
+and the handler list is resolved, in YAML order, into one composed program: for
+every target cell and field group, the source cell in the state before the
+fill and the chain of momentum transforms applied on the way
+(boundary_registry.py:183-187 applies the handlers one after another, each
+gathering from the state its predecessors left).  The GPU applies the program
+in one launch (pxh_bc_apply).
+
+Periodic is not in the reference (README.md:163 advertises it; nothing is
+registered in domain/boundaries/__init__.py).  Here a Periodic boundary cell takes
+every field from the cell N-2 cells against its outward normal: the last owned
+column or row on the opposite side of the domain.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _hip
+
+BOUNDARY_REGISTRY = {}
+
+
+def register_boundary(name):
+    def decorator(cls):
+        BOUNDARY_REGISTRY[name] = cls
+        return cls
+    return decorator
+
+
+# ---------------------------------------------------------------------------
+# geometry (host, set-up only)
+# ---------------------------------------------------------------------------
+def point_in_polygon(points, polygon):
+    """Even-odd ray test with the reference's half-open crossing rule and its
+    1e-8 tolerance (boundary_registry.py:19-44), vectorised over points (N, 2)."""
+    polygon = np.asarray(polygon, dtype=np.float64)
+    x = points[:, 0][:, None]
+    y = points[:, 1][:, None]
+    xi, yi = polygon[None, :, 0], polygon[None, :, 1]
+    xj, yj = np.roll(polygon[:, 0], 1)[None, :], np.roll(polygon[:, 1], 1)[None, :]
+    inside = np.zeros(points.shape[0], dtype=np.int64)
+    crosses = (yi > y) != (yj > y)
+    denom = np.where(crosses, yj - yi, 1.0)
+    test_x = xi + (xj - xi) * (y - yi) / denom
+    inside = np.sum(crosses & (x <= test_x + 1e-8), axis=1)
+    return inside % 2 == 1
+
+
+def compute_label_mask(X, Y, polygon):
+    """Cells whose centre is inside the polygon.  Only points inside the
+    polygon's bounding box (widened by the 1e-8 tolerance) are tested: a point
+    outside it crosses no edge (above, below or right of the box) or every
+    edge its row crosses (left of the box), an even count either way."""
+    X = np.asarray(X, np.float64)
+    Y = np.asarray(Y, np.float64)
+    poly = np.asarray(polygon, dtype=np.float64)
+    lo, hi = poly.min(axis=0), poly.max(axis=0)
+    cand = np.nonzero(((X >= lo[0] - 1e-8) & (X <= hi[0] + 1e-8) & (Y >= lo[1]) & (Y <= hi[1])).ravel())[0]
+    out = np.zeros(X.size, dtype=bool)
+    pts = np.stack([X.ravel()[cand], Y.ravel()[cand]], axis=-1)
+    step = 1 << 20
+    for s in range(0, len(cand), step):
+        out[cand[s:s + step]] = point_in_polygon(pts[s:s + step], poly)
+    return out.reshape(X.shape)
+
+
+def _clip_pairs(b, shift, shape):
+    ny, nx = shape
+    i = b + np.asarray(shift, dtype=np.int64)
+    i[:, 0] = np.clip(i[:, 0], 0, ny - 1)
+    i[:, 1] = np.clip(i[:, 1], 0, nx - 1)
+    return i
+
+
+def compute_reflective_indices(mask, normal):
+    """boundary_registry.py:61-82: interior = boundary + (round(ny), -round(nx)), clipped."""
+    b = np.argwhere(mask)
+    return b, _clip_pairs(b, (int(np.round(normal[1])), -int(np.round(normal[0]))), mask.shape)
+
+
+def compute_periodic_indices(mask, normal):
+    ny, nx = mask.shape
+    b = np.argwhere(mask)
+    shift = (int(np.round(normal[1])) * (ny - 2), -int(np.round(normal[0])) * (nx - 2))
+    return b, _clip_pairs(b, shift, mask.shape)
+
+
+# ---------------------------------------------------------------------------
+# handlers: what each writes, and the momentum transform it applies
+# ---------------------------------------------------------------------------
+G = _hip
+
+
+@dataclass
+class Handler:
+    mask: np.ndarray
+    normal: np.ndarray
+    boundary_indices: np.ndarray = None
+    interior_indices: np.ndarray = None
+    values: np.ndarray = None
+
+    groups = ()
+    kind = _hip.BC_COPY
+    periodic = False
+
+    def __post_init__(self):
+        self.normal = np.asarray(self.normal, dtype=np.float64)
+        if self.boundary_indices is None:
+            fn = compute_periodic_indices if self.periodic else compute_reflective_indices
+            self.boundary_indices, self.interior_indices = fn(self.mask, self.normal)
+
+
+@register_boundary("Roe Exner Reflective")
+class RoeExner_ReflectiveBoundary(Handler):
+    """roeexner_reflective.py:17-63: h, z_b, n, G copied; normal momentum negated."""
+    groups = (G.G_H, G.G_HUV, G.G_ZB, G.G_N, G.G_G)
+    kind = _hip.BC_REFLECT_RX
+
+
+@register_boundary("Roe Exner Transmissive")
+class RoeExner_TransmissiveBoundary(Handler):
+    """roeexner_transmissive.py:17-31: h, hu, hv copied."""
+    groups = (G.G_H, G.G_HUV)
+
+
+@register_boundary("Roe Exner TransmissiveBed")
+class RoeExner_TransmissiveBedBoundary(Handler):
+    """roeexner_transmissivebed.py:17-27: z_b copied."""
+    groups = (G.G_ZB,)
+
+
+@register_boundary("Roe Exner Periodic")
+class RoeExner_PeriodicBoundary(Handler):
+    groups = (G.G_H, G.G_HUV, G.G_Z, G.G_ZB, G.G_N, G.G_NB, G.G_G)
+    periodic = True
+
+
+@register_boundary("Roe Reflective")
+class Roe_ReflectiveBoundary(Handler):
+    """roe_reflective.py:18-44: h, z, n copied; hu*(1-2nx^2), hv*(1-2ny^2)."""
+    groups = (G.G_H, G.G_Z, G.G_N, G.G_HUV)
+    kind = _hip.BC_REFLECT_ROE
+
+
+@register_boundary("Roe Transmissive")
+class Roe_TransmissiveBoundary(Handler):
+    """roe_transmissive.py:17-43: h, z, n, hu, hv copied."""
+    groups = (G.G_H, G.G_Z, G.G_N, G.G_HUV)
+
+
+@register_boundary("Roe Periodic")
+class Roe_PeriodicBoundary(Handler):
+    groups = (G.G_H, G.G_HUV, G.G_Z, G.G_N)
+    periodic = True
+
+
+STATIC_GROUPS = (G.G_Z, G.G_N, G.G_NB)
+
+
+# ---------------------------------------------------------------------------
+# composed program
+# ---------------------------------------------------------------------------
+def compose(handlers, shape):
+    """Resolve the sequential handlers into per-target (src, chain) entries.
+
+    Returns (entries: structured numpy array of pxh_bc_entry, idempotent: bool).
+    A program is idempotent when no entry's source is a target of the same
+    group: then one launch reads only values the launch never writes, and
+    applying it twice equals applying it once."""
+    ny, nx = shape
+    rows = []
+    idempotent = True
+    for grp in range(7):
+        src_map, chain_map = {}, {}
+        for k, hd in enumerate(handlers):
+            if grp not in hd.groups:
+                continue
+            b = hd.boundary_indices[:, 0] * nx + hd.boundary_indices[:, 1]
+            i = hd.interior_indices[:, 0] * nx + hd.interior_indices[:, 1]
+            transform = grp == G.G_HUV and hd.kind != _hip.BC_COPY
+            new = []
+            for bb, ii in zip(b.tolist(), i.tolist()):
+                ch = chain_map.get(ii, ())
+                new.append((bb, src_map.get(ii, ii), ch + (k,) if transform else ch))
+            for bb, s, ch in new:
+                src_map[bb] = s
+                chain_map[bb] = ch
+        targets = set(src_map)
+        for t, s in src_map.items():
+            ch = chain_map[t]
+            if s == t and not ch:
+                continue
+            if s in targets:
+                idempotent = False
+            if len(ch) > _hip.MAX_CHAIN:
+                raise NotImplementedError(f"boundary chain of {len(ch)} reflections exceeds {_hip.MAX_CHAIN}")
+            rows.append((t, s, grp, ch))
+    ent = np.zeros(len(rows), dtype=np.dtype([("target", "<i4"), ("src", "<i4"), ("group", "u1"),
+                                              ("nchain", "u1"), ("chain", "u1", (_hip.MAX_CHAIN,))]))
+    for q, (t, s, grp, ch) in enumerate(rows):
+        ent[q]["target"], ent[q]["src"], ent[q]["group"], ent[q]["nchain"] = t, s, grp, len(ch)
+        ent[q]["chain"][:len(ch)] = ch
+    assert ent.itemsize == 16
+    return ent, idempotent
+
+
+class BoundaryManager:
+    """BoundaryManager(params, X, Y) (boundary_registry.py:84-187)."""
+
+    def __init__(self, params, X, Y, device="cuda"):
+        self.X, self.Y = np.asarray(X), np.asarray(Y)
+        self.flux_scheme = params["flux_scheme"]
+        self.boundary_specs = params.get("boundaries") or {}
+        self.boundary_handlers = []
+        self.names = []
+        shape = self.X.shape
+        masks = []
+        for key, spec in self.boundary_specs.items():
+            self.names.append(key)
+            btype = f"{self.flux_scheme} {spec['type']}"
+            if btype not in BOUNDARY_REGISTRY:
+                raise KeyError(f"Unknown boundary '{btype}'. Available options: {list(BOUNDARY_REGISTRY.keys())}")
+            m = compute_label_mask(self.X, self.Y, spec["polygon"])
+            masks.append(m)
+            vals = np.array([np.nan if str(v).lower() == "nan" else float(v) for v in spec.get("values", [])])
+            self.boundary_handlers.append(BOUNDARY_REGISTRY[btype](mask=m, normal=spec["normal"], values=vals))
+        self.boundary_mask = np.any(np.stack(masks), axis=0) if masks else np.zeros(shape, dtype=bool)
+        if len(self.boundary_handlers) > _hip.MAX_HANDLERS:
+            raise NotImplementedError(f"more than {_hip.MAX_HANDLERS} boundary handlers")
+        entries, self.idempotent = compose(self.boundary_handlers, shape)
+        self.n_entries = len(entries)
+        self.device = device
+        self._entries_dev = torch.from_numpy(entries.view(np.uint8).copy()).to(device) if len(entries) else None
+        self.groups_present = sorted({int(g) for g in entries["group"]}) if len(entries) else []
+
+    def program(self, groups=None):
+        """pxh_bc struct restricted to the given groups (default: all)."""
+        bc = _hip.BC()
+        bc.entries = self._entries_dev.data_ptr() if self._entries_dev is not None else None
+        bc.n_entries = self.n_entries
+        bc.n_handlers = len(self.boundary_handlers)
+        mask = 0
+        for g_ in (range(7) if groups is None else groups):
+            mask |= 1 << g_
+        bc.group_mask = mask
+        for k, hd in enumerate(self.boundary_handlers):
+            bc.handlers[k].kind = hd.kind
+            bc.handlers[k].nx = float(hd.normal[0])
+            bc.handlers[k].ny = float(hd.normal[1])
+        return bc
+
+    def apply_inplace(self, state, groups=None):
+        from .state import fields_struct, dtype_code
+        if self.n_entries == 0:
+            return state
+        grid = _hip.Grid(state.h.shape[0], state.h.shape[1], dtype_code(state.h.dtype), 0, 0.0)
+        fs = fields_struct(state)
+        bc = self.program(groups)
+        stream = torch.cuda.current_stream().cuda_stream
+        _hip.call("pxh_bc_apply", grid, fs, bc, stream)
+        return state
+
+    def apply(self, state, time=0.0):
+        """Functional ring fill: returns a new state (boundary_registry.py:183-187)."""
+        return self.apply_inplace(state.clone())

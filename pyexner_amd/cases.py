@@ -1,0 +1,146 @@
+"""Synthetic workloads (initial states, boundaries, parameters).
+
+All draws come from numpy.random.default_rng(seed) in the order listed in each
+builder.  Bed bumps are z_b = sum_k A_k exp(-((x-xc_k)^2 + (y-yc_k)^2) / (2 sigma_k^2)),
+each bump evaluated on the cells within 8 sigma_k of its centre (exactly zero
+beyond; exp(-32) ~ 1e-14).
+
+* dambreak_1d  - plain Roe dam break on a wet bed, 512 x 32, dx = 0.1 m,
+  h = 1.0 for x < 25.6 m else 0.5, Reflective ring (the Stoker case).
+* channel      - Roe-Exner channel, 2048 x 512, dx = 0.02 m, 32 bumps, h = 0.1,
+  u = 0.5, MPM sediment, Transmissive in/outflow, Reflective side walls.
+* radial_dambreak - Roe-Exner radial dam break, 8192 x 8192, dx = 0.05 m,
+  h = 2.0 inside r < 50 m else 0.5, 256 bumps, MPM sediment, Reflective ring.
+Every builder takes (nx, ny) so tests can shrink the same physics.
+"""
+import numpy as np
+
+from .mesh import uniform_mesh
+
+MPM_SEDIMENT = {"sed1": {"fraction": 1.0, "diameter": 1.61e-3, "density": 2630.0, "erosion_flux": 0.04,
+                         "deposition_flux": 0.01}}
+
+
+def ring_boundaries(mesh, kinds):
+    """Axis-aligned boundary polygons around the outer ring of cells.
+
+    kinds: list of (name, type, side) with side in W, E, S, N; polygons span
+    half a cell either side of the ring's cell centres and the full other
+    extent (corners included), as tests/runtime/2D_dambreak/build_scaling_cases.py:7-55 lays them out."""
+    X, Y, dh = mesh.local_X, mesh.local_Y, mesh.dh
+    x0, x1 = X[0, 0], X[0, -1]
+    yb, yt = Y[-1, 0], Y[0, 0]
+    hh = dh / 2
+    full_x = (x0 - hh, x1 + hh)
+    full_y = (yb - hh, yt + hh)
+    rect = lambda xa, xb, ya, yb_: [[xa, ya], [xb, ya], [xb, yb_], [xa, yb_]]  # noqa: E731
+    side = {
+        "W": (rect(x0 - hh, x0 + hh, *full_y), [-1.0, 0.0]),
+        "E": (rect(x1 - hh, x1 + hh, *full_y), [1.0, 0.0]),
+        "S": (rect(*full_x, yb - hh, yb + hh), [0.0, -1.0]),
+        "N": (rect(*full_x, yt - hh, yt + hh), [0.0, 1.0]),
+    }
+    out = {}
+    for name, typ, sd in kinds:
+        poly, normal = side[sd]
+        out[name] = {"type": typ, "polygon": [[float(a), float(b)] for a, b in poly],
+                     "values": [0.0, 0.0, 0.0, 0.0], "normal": normal}
+    return out
+
+
+def gaussian_bumps(X, Y, rng, nb, amp=(0.0, 0.02), sigma=(0.05, 0.3)):
+    dh = abs(X[0, 1] - X[0, 0])
+    x0, x1 = X[0, 0], X[0, -1]
+    ymin, ymax = Y[-1, 0], Y[0, 0]
+    xc = rng.uniform(x0, x1, nb)
+    yc = rng.uniform(ymin, ymax, nb)
+    A = rng.uniform(amp[0], amp[1], nb)
+    sg = rng.uniform(sigma[0], sigma[1], nb)
+    zb = np.zeros(X.shape)
+    ny, nx = X.shape
+    for k in range(nb):
+        r = 8.0 * sg[k]
+        c0 = max(int(np.floor((xc[k] - r - x0) / dh)), 0)
+        c1 = min(int(np.ceil((xc[k] + r - x0) / dh)) + 1, nx)
+        r0 = max(int(np.floor((ymax - (yc[k] + r)) / dh)), 0)
+        r1 = min(int(np.ceil((ymax - (yc[k] - r)) / dh)) + 1, ny)
+        if c0 >= c1 or r0 >= r1:
+            continue
+        xs, ys = X[r0:r1, c0:c1], Y[r0:r1, c0:c1]
+        d2 = (xs - xc[k]) ** 2 + (ys - yc[k]) ** 2
+        zb[r0:r1, c0:c1] += np.where(d2 <= r * r, A[k] * np.exp(-d2 / (2 * sg[k] ** 2)), 0.0)
+    return zb
+
+
+def dambreak_1d(nx=512, ny=32, dx=0.1, x_split=25.6, hl=1.0, hr=0.5):
+    mesh = uniform_mesh(ny, nx, dx)
+    X = mesh.local_X
+    h = np.where(X < x_split, hl, hr)
+    zeros = np.zeros_like(h)
+    arrays = {"h": h, "hu": zeros.copy(), "hv": zeros.copy(), "z": zeros.copy(), "n": zeros.copy()}
+    params = {"flux_scheme": "Roe", "integrator": "Forward Euler", "cfl": 0.5, "end_time": 2.0,
+              "out_freq": 2.0, "parNx": 1, "parNy": 1,
+              "boundaries": ring_boundaries(mesh, [("west", "Reflective", "W"), ("east", "Reflective", "E"),
+                                                   ("south", "Reflective", "S"), ("north", "Reflective", "N")])}
+    return mesh, arrays, params
+
+
+def _exner_params(mesh, bounds, integrator="SSPRK2"):
+    return {"flux_scheme": "Roe Exner", "integrator": integrator, "cfl": 0.5, "end_time": 1.0, "out_freq": 1.0,
+            "parNx": 1, "parNy": 1,
+            "erosion": {"grass_factor": "MPM", "bulk_porosity": 0.42, "sediments": MPM_SEDIMENT},
+            "boundaries": bounds}
+
+
+def channel(nx=2048, ny=512, dx=0.02, n_bumps=32, seed=2, h0=0.1, u0=0.5):
+    mesh = uniform_mesh(ny, nx, dx)
+    rng = np.random.default_rng(seed)
+    zb = gaussian_bumps(mesh.local_X, mesh.local_Y, rng, n_bumps)
+    h = np.full(zb.shape, h0)
+    arrays = {"h": h, "hu": h * u0, "hv": np.zeros_like(h), "z": np.zeros_like(h), "z_b": zb,
+              "n": np.zeros_like(h), "G": np.zeros_like(h)}
+    bounds = ring_boundaries(mesh, [("inlet", "Transmissive", "W"), ("outlet", "Transmissive", "E"),
+                                    ("inlet_bed", "TransmissiveBed", "W"), ("outlet_bed", "TransmissiveBed", "E"),
+                                    ("south", "Reflective", "S"), ("north", "Reflective", "N")])
+    return mesh, arrays, _exner_params(mesh, bounds)
+
+
+def radial_dambreak(nx=8192, ny=8192, dx=0.05, n_bumps=256, seed=3, radius=50.0, h_in=2.0, h_out=0.5):
+    mesh = uniform_mesh(ny, nx, dx)
+    rng = np.random.default_rng(seed)
+    X, Y = mesh.local_X, mesh.local_Y
+    zb = gaussian_bumps(X, Y, rng, n_bumps)
+    xm = 0.5 * (X[0, 0] + X[0, -1])
+    ym = 0.5 * (Y[0, 0] + Y[-1, 0])
+    h = np.where((X - xm) ** 2 + (Y - ym) ** 2 < radius * radius, h_in, h_out)
+    zeros = np.zeros_like(h)
+    arrays = {"h": h, "hu": zeros.copy(), "hv": zeros.copy(), "z": zeros.copy(), "z_b": zb, "n": zeros.copy(),
+              "G": zeros.copy()}
+    bounds = ring_boundaries(mesh, [("west", "Reflective", "W"), ("east", "Reflective", "E"),
+                                    ("south", "Reflective", "S"), ("north", "Reflective", "N")])
+    return mesh, arrays, _exner_params(mesh, bounds)
+
+
+def periodic_box(nx=64, ny=48, dx=0.05, seed=11, exner=True):
+    """Smooth periodic-compatible state for translation/conservation checks."""
+    mesh = uniform_mesh(ny, nx, dx)
+    rng = np.random.default_rng(seed)
+    X, Y = mesh.local_X, mesh.local_Y
+    Lx, Ly = (nx - 2) * dx, (ny - 2) * dx
+    kx, ky = 2 * np.pi / Lx, 2 * np.pi / Ly
+    ph = rng.uniform(0, 2 * np.pi, 4)
+    h = 0.3 + 0.05 * np.sin(kx * X + ph[0]) * np.cos(ky * Y + ph[1])
+    u = 0.2 + 0.1 * np.cos(kx * X + ph[2])
+    v = 0.1 * np.sin(ky * Y + ph[3])
+    zb = 0.01 + 0.005 * np.cos(kx * X) * np.sin(ky * Y)
+    zeros = np.zeros_like(h)
+    arrays = {"h": h, "hu": h * u, "hv": h * v, "z": zeros.copy(), "z_b": zb, "n": zeros.copy(),
+              "G": zeros.copy()}
+    kinds = [("west", "Periodic", "W"), ("east", "Periodic", "E"), ("south", "Periodic", "S"),
+             ("north", "Periodic", "N")]
+    bounds = ring_boundaries(mesh, kinds)
+    if exner:
+        return mesh, arrays, _exner_params(mesh, bounds)
+    params = {"flux_scheme": "Roe", "integrator": "Forward Euler", "cfl": 0.5, "end_time": 1.0,
+              "out_freq": 1.0, "parNx": 1, "parNy": 1, "boundaries": bounds}
+    return mesh, {k: arrays[k] for k in ("h", "hu", "hv", "z", "n")}, params

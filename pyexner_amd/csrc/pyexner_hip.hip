@@ -1,0 +1,618 @@
+// PyExner per-step hot path for gfx950: kernels and the extern "C" entry points
+// declared in include/pyexner_hip.h.
+//
+// Layout: structure-of-arrays (ny, nx) fields in HBM, one uint8 nodata mask.
+// A Forward-Euler step (solvers/roe_exner_solver.py:80-116) runs as
+//   hydro flux + update -> momentum corrections -> ring fill (h, hu, hv)
+//   -> G -> Exner flux + bed update -> ring fill (z_b, G)
+// and the CFL dt is a two-level min reduction into a device scalar.
+#include "common.hpp"
+#include "flux_roe.hpp"
+#include "flux_roe_exner.hpp"
+
+namespace pxh {
+
+template <typename T>
+struct F {
+    T *h, *hu, *hv, *z, *z_b, *n, *n_b, *G;
+};
+
+template <typename T>
+static F<T> cast_fields(const pxh_fields *f) {
+    F<T> r;
+    r.h = static_cast<T *>(f->h);
+    r.hu = static_cast<T *>(f->hu);
+    r.hv = static_cast<T *>(f->hv);
+    r.z = static_cast<T *>(f->z);
+    r.z_b = static_cast<T *>(f->z_b);
+    r.n = static_cast<T *>(f->n);
+    r.n_b = static_cast<T *>(f->n_b);
+    r.G = static_cast<T *>(f->G);
+    return r;
+}
+
+constexpr int BX = 64, BY = 4;
+
+static inline dim3 grid2d(int ny, int nx) {
+    return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY);
+}
+
+// ---------------------------------------------------------------------------
+// Roe-Exner hydro: roe_solve_2D (roe_exner.py:535-612), one thread per cell,
+// summing F = M_x[r,c] + P_x[r,c-1] + M_y[r,c] + P_y[r-1,c] in that order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ HydroCell<T> load_hc(const F<T> &s, int i) {
+    HydroCell<T> c;
+    c.h = s.h[i]; c.hu = s.hu[i]; c.hv = s.hv[i];
+    c.zb = s.z_b[i]; c.z = s.z[i]; c.n = s.n_b[i];
+    return c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void k_rx_hydro(F<T> in, F<T> out, const uint8_t *__restrict__ m0,
+                                                      const T *__restrict__ dtp, int ny, int nx, T dx) {
+    const int c = blockIdx.x * BX + threadIdx.x;
+    const int r = blockIdx.y * BY + threadIdx.y;
+    if (r >= ny || c >= nx) return;
+    const int i = r * nx + c;
+    const T T_ = C<T>(DRY_TOL);
+    const HydroCell<T> me = load_hc(in, i);
+    const bool mme = m0[i] != 0;
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    T P[3], M[3];
+    if (c + 1 < nx) {  // east edge: this cell is side i -> upwM
+        const int j = i + 1;
+        const HydroCell<T> o = load_hc(in, j);
+        if (((me.h > T_) || (o.h > T_)) && !(mme || m0[j])) {
+            roe_exner_edge<0>(me, o, dx, P, M);
+            acc0 = acc0 + M[0]; acc1 = acc1 + M[1]; acc2 = acc2 + M[2];
+        }
+    }
+    if (c > 0) {  // west edge: this cell is side j -> upwP
+        const int j = i - 1;
+        const HydroCell<T> o = load_hc(in, j);
+        if (((o.h > T_) || (me.h > T_)) && !(m0[j] || mme)) {
+            roe_exner_edge<0>(o, me, dx, P, M);
+            acc0 = acc0 + P[0]; acc1 = acc1 + P[1]; acc2 = acc2 + P[2];
+        }
+    }
+    if (r + 1 < ny) {  // south edge (row r+1): this cell is side i -> upwM
+        const int j = i + nx;
+        const HydroCell<T> o = load_hc(in, j);
+        if (((me.h > T_) || (o.h > T_)) && !(mme || m0[j])) {
+            roe_exner_edge<1>(me, o, dx, P, M);
+            acc0 = acc0 + M[0]; acc1 = acc1 + M[1]; acc2 = acc2 + M[2];
+        }
+    }
+    if (r > 0) {  // north edge: this cell is side j -> upwP
+        const int j = i - nx;
+        const HydroCell<T> o = load_hc(in, j);
+        if (((o.h > T_) || (me.h > T_)) && !(m0[j] || mme)) {
+            roe_exner_edge<1>(o, me, dx, P, M);
+            acc0 = acc0 + P[0]; acc1 = acc1 + P[1]; acc2 = acc2 + P[2];
+        }
+    }
+    const T dt = *dtp;
+    out.h[i] = me.h - dt * acc0 / dx;
+    out.hu[i] = me.hu - dt * acc1 / dx;
+    out.hv[i] = me.hv - dt * acc2 / dx;
+}
+
+// momentum_corrections (roe_exner.py:14-62) on interior cells, in place on the
+// updated hu/hv; z + z_b uses the bed before this step's Exner update.
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void k_rx_momcorr(F<T> in, F<T> out, const uint8_t *__restrict__ m0,
+                                                        int ny, int nx) {
+    const int c = blockIdx.x * BX + threadIdx.x;
+    const int r = blockIdx.y * BY + threadIdx.y;
+    if (r < 1 || c < 1 || r >= ny - 1 || c >= nx - 1) return;
+    const int i = r * nx + c;
+    const T T_ = C<T>(DRY_TOL);
+    const int iW = i - 1, iE = i + 1, iN = i - nx, iS = i + nx;
+    const T hc = out.h[i];
+    const T eta = hc + (in.z[i] + in.z_b[i]);
+    const T zW = in.z[iW] + in.z_b[iW], zE = in.z[iE] + in.z_b[iE];
+    const T zN = in.z[iN] + in.z_b[iN], zS = in.z[iS] + in.z_b[iS];
+    const bool dryW = out.h[iW] < T_, dryE = out.h[iE] < T_;
+    const bool dryN = out.h[iN] < T_, dryS = out.h[iS] < T_;
+    const bool stop_u = ((eta < zE) && dryE) || ((eta < zW) && dryW) || m0[iE] || m0[iW];
+    const bool stop_v = ((eta < zN) && dryN) || ((eta < zS) && dryS) || m0[iN] || m0[iS];
+    const bool active = (hc > T_) && !m0[i];
+    if (!(active && !stop_u)) out.hu[i] = T(0);
+    if (!(active && !stop_v)) out.hv[i] = T(0);
+}
+
+// G for every cell from the post-fill hydro state (roe_exner_solver.py:99).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rx_G(F<T> in, F<T> out, const uint8_t *__restrict__ m0, int ncell,
+                                               pxh_sediments sd) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ncell) return;
+    const bool nod = m0[i] != 0;
+    out.G[i] = sd.mpm ? mpm_G<T>(out.h[i], out.hu[i], out.hv[i], nod, sd) : const_G<T>(out.h[i], in.G[i], nod);
+}
+
+// exner_solve_2D (roe_exner.py:665-717): reads the updated h, hu, hv, G and
+// the bed of the step start; div = ((0 - Fx[c-1]) + Fx[c]) - Fy[r-1] + Fy[r].
+template <typename T>
+__device__ __forceinline__ ExnerCell<T> load_ec(const F<T> &s, const T *zb, int i) {
+    ExnerCell<T> c;
+    c.h = s.h[i]; c.hu = s.hu[i]; c.hv = s.hv[i]; c.zb = zb[i]; c.G = s.G[i];
+    return c;
+}
+
+template <typename T>
+__device__ __forceinline__ bool exner_active(const ExnerCell<T> &a, const ExnerCell<T> &b, bool ma, bool mb) {
+    const T T_ = C<T>(DRY_TOL), S_ = C<T>(SED_TOL);
+    const bool stop = (ma || mb) || ((a.zb < S_) && (b.zb < S_));
+    const bool depth = (a.h > T_) || (b.h > T_);
+    return depth && !stop;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void k_rx_exner(F<T> in, F<T> out, const uint8_t *__restrict__ m0,
+                                                      const T *__restrict__ dtp, int ny, int nx, T dx) {
+    const int c = blockIdx.x * BX + threadIdx.x;
+    const int r = blockIdx.y * BY + threadIdx.y;
+    if (r >= ny || c >= nx) return;
+    const int i = r * nx + c;
+    const ExnerCell<T> me = load_ec(out, in.z_b, i);
+    const bool mme = m0[i] != 0;
+    T div = T(0);
+    if (c > 0) {
+        const int j = i - 1;
+        const ExnerCell<T> o = load_ec(out, in.z_b, j);
+        const T f = exner_active(o, me, m0[j] != 0, mme) ? exner_edge<0>(o, me) : T(0);
+        div = div + (-f);
+    }
+    if (c + 1 < nx) {
+        const int j = i + 1;
+        const ExnerCell<T> o = load_ec(out, in.z_b, j);
+        const T f = exner_active(me, o, mme, m0[j] != 0) ? exner_edge<0>(me, o) : T(0);
+        div = div + f;
+    }
+    if (r > 0) {
+        const int j = i - nx;
+        const ExnerCell<T> o = load_ec(out, in.z_b, j);
+        const T f = exner_active(o, me, m0[j] != 0, mme) ? exner_edge<1>(o, me) : T(0);
+        div = div + (-f);
+    }
+    if (r + 1 < ny) {
+        const int j = i + nx;
+        const ExnerCell<T> o = load_ec(out, in.z_b, j);
+        const T f = exner_active(me, o, mme, m0[j] != 0) ? exner_edge<1>(me, o) : T(0);
+        div = div + f;
+    }
+    out.z_b[i] = me.zb - *dtp * div / dx;
+}
+
+// ---------------------------------------------------------------------------
+// Plain Roe hydro (roe.py:251-315): no active-edge masking.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ RoeCell<T> load_rc(const F<T> &s, int i) {
+    RoeCell<T> c;
+    c.h = s.h[i]; c.hu = s.hu[i]; c.hv = s.hv[i]; c.z = s.z[i]; c.n = s.n[i];
+    return c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void k_roe_hydro(F<T> in, F<T> out, const T *__restrict__ dtp, int ny,
+                                                       int nx, T dx) {
+    const int c = blockIdx.x * BX + threadIdx.x;
+    const int r = blockIdx.y * BY + threadIdx.y;
+    if (r >= ny || c >= nx) return;
+    const int i = r * nx + c;
+    const RoeCell<T> me = load_rc(in, i);
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    T P[3], M[3];
+    if (c + 1 < nx) {
+        roe_edge<0>(me, load_rc(in, i + 1), dx, P, M);
+        acc0 = acc0 + M[0]; acc1 = acc1 + M[1]; acc2 = acc2 + M[2];
+    }
+    if (c > 0) {
+        roe_edge<0>(load_rc(in, i - 1), me, dx, P, M);
+        acc0 = acc0 + P[0]; acc1 = acc1 + P[1]; acc2 = acc2 + P[2];
+    }
+    if (r + 1 < ny) {
+        roe_edge<1>(me, load_rc(in, i + nx), dx, P, M);
+        acc0 = acc0 + M[0]; acc1 = acc1 + M[1]; acc2 = acc2 + M[2];
+    }
+    if (r > 0) {
+        roe_edge<1>(load_rc(in, i - nx), me, dx, P, M);
+        acc0 = acc0 + P[0]; acc1 = acc1 + P[1]; acc2 = acc2 + P[2];
+    }
+    const T dt = *dtp;
+    out.h[i] = me.h - dt * acc0 / dx;
+    out.hu[i] = me.hu - dt * acc1 / dx;
+    out.hv[i] = me.hv - dt * acc2 / dx;
+}
+
+// ---------------------------------------------------------------------------
+// CFL dt: per-block NaN-propagating min into partials, then a one-block final
+// reduction that applies cfl and the 1e-6 clamp (roe_exner_solver.py:118-122).
+// ---------------------------------------------------------------------------
+constexpr int DT_THREADS = 256;
+constexpr int DT_MAX_BLOCKS = 4096;
+
+template <typename T>
+__device__ __forceinline__ T block_min(T v) {
+    __shared__ T red[DT_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < DT_THREADS / 64; ++k) v = pmin(v, red[k]);
+    }
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(DT_THREADS) void k_rx_dt_partial(F<T> s, const uint8_t *__restrict__ m0, int ny,
+                                                              int nx, T dx, T *__restrict__ partial) {
+    const T T_ = C<T>(DRY_TOL);
+    T v = inf_v<T>();
+    const long ncell = long(ny) * nx;
+    for (long i = long(blockIdx.x) * DT_THREADS + threadIdx.x; i < ncell; i += long(gridDim.x) * DT_THREADS) {
+        const int r = int(i / nx), c = int(i - long(r) * nx);
+        const T hi = s.h[i];
+        const bool mi = m0[i] != 0;
+        if (c + 1 < nx) {
+            const long j = i + 1;
+            const T hj = s.h[j];
+            if (((hi > T_) || (hj > T_)) && !(mi || m0[j]))
+                v = pmin(v, dt_edge<0>(hi, s.hu[i], s.hv[i], s.G[i], hj, s.hu[j], s.hv[j], s.G[j], dx));
+        }
+        if (r + 1 < ny) {
+            const long j = i + nx;
+            const T hj = s.h[j];
+            if (((hi > T_) || (hj > T_)) && !(mi || m0[j]))
+                v = pmin(v, dt_edge<1>(hi, s.hu[i], s.hv[i], s.G[i], hj, s.hu[j], s.hv[j], s.G[j], dx));
+        }
+    }
+    v = block_min(v);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// roe.py:13-30: cell-based speeds over every cell (the mask has no effect).
+template <typename T>
+__global__ __launch_bounds__(DT_THREADS) void k_roe_dt_partial(F<T> s, long ncell, T dx, T *__restrict__ partial) {
+    const T T_ = C<T>(DRY_TOL), g = C<T>(GRAV), V = C<T>(VEL_TOL), zero = T(0);
+    T v = inf_v<T>();
+    for (long i = long(blockIdx.x) * DT_THREADS + threadIdx.x; i < ncell; i += long(gridDim.x) * DT_THREADS) {
+        const T hr = s.h[i];
+        const bool wet = hr > T_;
+        const T h = wet ? hr : zero;
+        const T hv = wet ? s.hv[i] : zero;
+        const T hu = wet ? s.hu[i] : zero;
+        const T u = hu / (h + T_), vv = hv / (h + T_);
+        const T cc = d_sqrt(g * h);
+        const T dtx = dx / (d_abs(u) + cc + V);
+        const T dty = dx / (d_abs(vv) + cc + V);
+        v = pmin(v, pmin(dtx, dty));
+    }
+    v = block_min(v);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(DT_THREADS) void k_dt_final(const T *__restrict__ partial, int nblk, T cfl, int clamp,
+                                                         T *__restrict__ out) {
+    T v = inf_v<T>();
+    for (int k = threadIdx.x; k < nblk; k += DT_THREADS) v = pmin(v, partial[k]);
+    v = block_min(v);
+    if (threadIdx.x == 0) {
+        T d = cfl * v;
+        if (clamp) d = pmax(d, C<T>(1e-6));
+        *out = d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SSPRK2 combine and init
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_combine(F<T> a, F<T> b, F<T> o, long ncell) {
+    const long i = long(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= ncell) return;
+    const T hf = T(0.5);
+    o.h[i] = hf * a.h[i] + hf * b.h[i];
+    o.hu[i] = hf * a.hu[i] + hf * b.hu[i];
+    o.hv[i] = hf * a.hv[i] + hf * b.hv[i];
+    o.z_b[i] = hf * a.z_b[i] + hf * b.z_b[i];
+    o.G[i] = hf * a.G[i] + hf * b.G[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rx_init(F<T> s, const uint8_t *__restrict__ m0, long ncell,
+                                                  pxh_sediments sd) {
+    const long i = long(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= ncell) return;
+    const bool nod = m0[i] != 0;
+    if (sd.mpm) {
+        // compute_n (roe_exner.py:64-73)
+        s.n_b[i] = (s.z_b[i] > C<T>(SED_TOL)) ? C<T>(sd.nb_sed) : s.n[i];
+        s.G[i] = mpm_G<T>(s.h[i], s.hu[i], s.hv[i], nod, sd);
+    } else {
+        s.n_b[i] = s.n[i];
+        s.G[i] = const_G<T>(s.h[i], s.G[i], nod);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Composed ring fill.  Each entry copies one field group from src to target;
+// momentum entries apply the chain of handler transforms in order
+// (roeexner_reflective.py:31-59, roe_reflective.py:34-39).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void bc_transform(const pxh_bc_handler &hd, T &hu, T &hv) {
+    const T nx = static_cast<T>(hd.nx), ny = static_cast<T>(hd.ny);
+    if (hd.kind == PXH_BC_REFLECT_RX) {
+        const T mn = hu * nx + hv * ny;
+        const T mt = -hu * ny + hv * nx;
+        const T mr = -mn;
+        const T u2 = mr * nx - mt * ny;
+        const T v2 = mr * ny + mt * nx;
+        hu = u2;
+        hv = v2;
+    } else if (hd.kind == PXH_BC_REFLECT_ROE) {
+        hu = hu * (T(1) - T(2) * nx * nx);
+        hv = hv * (T(1) - T(2) * ny * ny);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restrict__ ent, int n, pxh_bc bc) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const pxh_bc_entry e = ent[k];
+    if (!((bc.group_mask >> e.group) & 1u)) return;
+    const int t = e.target, src = e.src;
+    switch (e.group) {
+        case PXH_G_H: s.h[t] = s.h[src]; break;
+        case PXH_G_Z: s.z[t] = s.z[src]; break;
+        case PXH_G_ZB: s.z_b[t] = s.z_b[src]; break;
+        case PXH_G_N: s.n[t] = s.n[src]; break;
+        case PXH_G_NB: s.n_b[t] = s.n_b[src]; break;
+        case PXH_G_G: s.G[t] = s.G[src]; break;
+        case PXH_G_HUV: {
+            T hu = s.hu[src], hv = s.hv[src];
+            for (int q = 0; q < e.nchain; ++q) bc_transform<T>(bc.handlers[e.chain[q]], hu, hv);
+            s.hu[t] = hu;
+            s.hv[t] = hv;
+        } break;
+        default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch helpers
+// ---------------------------------------------------------------------------
+static inline bool grid_ok(const pxh_grid *g) {
+    return g && g->ny >= 2 && g->nx >= 2 && (g->dtype == PXH_F32 || g->dtype == PXH_F64) &&
+           (long(g->ny) * g->nx) < (1L << 31);
+}
+
+static inline int dt_blocks(const pxh_grid *g) {
+    long n = (long(g->ny) * g->nx + DT_THREADS - 1) / DT_THREADS;
+    return int(n < DT_MAX_BLOCKS ? n : DT_MAX_BLOCKS);
+}
+
+template <typename T>
+static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_t groups, hipStream_t st) {
+    if (!bc || bc->n_entries <= 0) return PXH_OK;
+    pxh_bc b = *bc;
+    b.group_mask &= groups;
+    if (!b.group_mask) return PXH_OK;
+    const int n = bc->n_entries;
+    hipLaunchKernelGGL(k_bc<T>, dim3((n + 255) / 256), dim3(256), 0, st, s, bc->entries, n, b);
+    return launch_status();
+}
+
+template <typename T>
+static int rx_step(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
+                   const pxh_sediments *sd, const pxh_bc *bc, const void *dt, hipStream_t st) {
+    F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
+    const int ny = g->ny, nx = g->nx;
+    const T dx = static_cast<T>(g->dx);
+    const T *dtp = static_cast<const T *>(dt);
+    hipLaunchKernelGGL(k_rx_hydro<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, dtp, ny, nx, dx);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_rx_momcorr<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, ny, nx);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
+    const int ncell = ny * nx;
+    hipLaunchKernelGGL(k_rx_G<T>, dim3((ncell + 255) / 256), dim3(256), 0, st, in, out, m0, ncell, *sd);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_rx_exner<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, dtp, ny, nx, dx);
+    if (int e = launch_status()) return e;
+    return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
+}
+
+template <typename T>
+static int rx_dt(const pxh_grid *g, const pxh_fields *s_, const uint8_t *m0, double cfl, void *dt_out, void *ws,
+                 hipStream_t st) {
+    F<T> s = cast_fields<T>(s_);
+    const int nb = dt_blocks(g);
+    T *part = static_cast<T *>(ws);
+    hipLaunchKernelGGL(k_rx_dt_partial<T>, dim3(nb), dim3(DT_THREADS), 0, st, s, m0, g->ny, g->nx,
+                       static_cast<T>(g->dx), part);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1,
+                       static_cast<T *>(dt_out));
+    return launch_status();
+}
+
+template <typename T>
+static int roe_dt(const pxh_grid *g, const pxh_fields *s_, double cfl, void *dt_out, void *ws, hipStream_t st) {
+    F<T> s = cast_fields<T>(s_);
+    const int nb = dt_blocks(g);
+    T *part = static_cast<T *>(ws);
+    hipLaunchKernelGGL(k_roe_dt_partial<T>, dim3(nb), dim3(DT_THREADS), 0, st, s, long(g->ny) * g->nx,
+                       static_cast<T>(g->dx), part);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 0,
+                       static_cast<T *>(dt_out));
+    return launch_status();
+}
+
+template <typename T>
+static int roe_step(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const pxh_bc *bc,
+                    const void *dt, hipStream_t st) {
+    F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
+    hipLaunchKernelGGL(k_roe_hydro<T>, grid2d(g->ny, g->nx), dim3(BX, BY), 0, st, in, out,
+                       static_cast<const T *>(dt), g->ny, g->nx, static_cast<T>(g->dx));
+    if (int e = launch_status()) return e;
+    return bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st);
+}
+
+template <typename T>
+static int bc_apply_t(const pxh_grid *g, const pxh_fields *s_, const pxh_bc *bc, hipStream_t st) {
+    return bc_launch<T>(g, cast_fields<T>(s_), bc, 0xffffffffu, st);
+}
+
+static bool fields_ok(const pxh_fields *f, bool exner) {
+    if (!f || !f->h || !f->hu || !f->hv || !f->z || !f->n) return false;
+    if (exner && (!f->z_b || !f->n_b || !f->G)) return false;
+    return true;
+}
+
+template <typename T>
+static int rx_part(int which, const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
+                   const pxh_sediments *sd, const void *dt, hipStream_t st) {
+    F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
+    const int ny = g->ny, nx = g->nx, ncell = ny * nx;
+    const T dx = static_cast<T>(g->dx);
+    const T *dtp = static_cast<const T *>(dt);
+    switch (which) {
+        case 0: hipLaunchKernelGGL(k_rx_hydro<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, dtp, ny, nx, dx); break;
+        case 1: hipLaunchKernelGGL(k_rx_momcorr<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, ny, nx); break;
+        case 2: hipLaunchKernelGGL(k_rx_G<T>, dim3((ncell + 255) / 256), dim3(256), 0, st, in, out, m0, ncell, *sd); break;
+        case 3: hipLaunchKernelGGL(k_rx_exner<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, dtp, ny, nx, dx); break;
+        default: return PXH_E_ARG;
+    }
+    return launch_status();
+}
+
+static int part_args_ok(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0) {
+    return grid_ok(g) && fields_ok(in, true) && fields_ok(out, true) && m0;
+}
+
+}  // namespace pxh
+
+using namespace pxh;
+
+#define DISPATCH(g, fn, ...) ((g)->dtype == PXH_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+int pxh_version(void) { return 1; }
+
+const char *pxh_error_string(int status) {
+    if (status == PXH_OK) return "ok";
+    if (status == PXH_E_ARG) return "invalid argument";
+    if (status == PXH_E_RANGE) return "size beyond a compiled-in limit";
+    if (status == PXH_E_UNSUPPORTED) return "unsupported configuration";
+    if (status > 0) return hipGetErrorString(static_cast<hipError_t>(status));
+    return "unknown error";
+}
+
+int pxh_roe_exner_init(const pxh_grid *g, const pxh_fields *s, const uint8_t *m0, const pxh_sediments *sd,
+                       void *stream) {
+    if (!grid_ok(g) || !fields_ok(s, true) || !m0 || !sd) return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    const long n = long(g->ny) * g->nx;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (g->dtype == PXH_F64)
+        hipLaunchKernelGGL(k_rx_init<double>, dim3((n + 255) / 256), dim3(256), 0, st, cast_fields<double>(s), m0, n,
+                           *sd);
+    else
+        hipLaunchKernelGGL(k_rx_init<float>, dim3((n + 255) / 256), dim3(256), 0, st, cast_fields<float>(s), m0, n,
+                           *sd);
+    return launch_status();
+}
+
+int pxh_roe_exner_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                       const pxh_sediments *sd, const pxh_bc *bc, const void *dt_dev, void *stream) {
+    if (!grid_ok(g) || !fields_ok(in, true) || !fields_ok(out, true) || !m0 || !sd || !dt_dev) return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    if (in->h == out->h || in->hu == out->hu || in->hv == out->hv || in->z_b == out->z_b || in->G == out->G)
+        return PXH_E_ARG;
+    if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
+    return DISPATCH(g, rx_step, g, in, out, m0, sd, bc, dt_dev, static_cast<hipStream_t>(stream));
+}
+
+int pxh_roe_solve_2D(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                     const void *dt_dev, void *stream) {
+    if (!part_args_ok(g, in, out, m0) || !dt_dev || in->h == out->h || in->hu == out->hu || in->hv == out->hv)
+        return PXH_E_ARG;
+    return DISPATCH(g, rx_part, 0, g, in, out, m0, nullptr, dt_dev, static_cast<hipStream_t>(stream));
+}
+
+int pxh_momentum_corrections(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                             void *stream) {
+    if (!part_args_ok(g, in, out, m0)) return PXH_E_ARG;
+    return DISPATCH(g, rx_part, 1, g, in, out, m0, nullptr, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int pxh_compute_G(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                  const pxh_sediments *sd, void *stream) {
+    if (!part_args_ok(g, in, out, m0) || !sd) return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    return DISPATCH(g, rx_part, 2, g, in, out, m0, sd, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int pxh_exner_solve_2D(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                       const void *dt_dev, void *stream) {
+    if (!part_args_ok(g, in, out, m0) || !dt_dev || in->z_b == out->z_b) return PXH_E_ARG;
+    return DISPATCH(g, rx_part, 3, g, in, out, m0, nullptr, dt_dev, static_cast<hipStream_t>(stream));
+}
+
+int64_t pxh_dt_workspace_size(const pxh_grid *g) {
+    if (!grid_ok(g)) return PXH_E_ARG;
+    return int64_t(DT_MAX_BLOCKS) * 8;
+}
+
+int pxh_roe_exner_dt(const pxh_grid *g, const pxh_fields *s, const uint8_t *m0, double cfl, void *dt_out,
+                     void *ws, void *stream) {
+    if (!grid_ok(g) || !fields_ok(s, true) || !m0 || !dt_out || !ws) return PXH_E_ARG;
+    return DISPATCH(g, rx_dt, g, s, m0, cfl, dt_out, ws, static_cast<hipStream_t>(stream));
+}
+
+int pxh_ssprk2_combine(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *o,
+                       void *stream) {
+    if (!grid_ok(g) || !fields_ok(a, true) || !fields_ok(b, true) || !fields_ok(o, true)) return PXH_E_ARG;
+    const long n = long(g->ny) * g->nx;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (g->dtype == PXH_F64)
+        hipLaunchKernelGGL(k_combine<double>, dim3((n + 255) / 256), dim3(256), 0, st, cast_fields<double>(a),
+                           cast_fields<double>(b), cast_fields<double>(o), n);
+    else
+        hipLaunchKernelGGL(k_combine<float>, dim3((n + 255) / 256), dim3(256), 0, st, cast_fields<float>(a),
+                           cast_fields<float>(b), cast_fields<float>(o), n);
+    return launch_status();
+}
+
+int pxh_roe_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const pxh_bc *bc,
+                 const void *dt_dev, void *stream) {
+    if (!grid_ok(g) || !fields_ok(in, false) || !fields_ok(out, false) || !dt_dev) return PXH_E_ARG;
+    if (in->h == out->h || in->hu == out->hu || in->hv == out->hv) return PXH_E_ARG;
+    if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
+    return DISPATCH(g, roe_step, g, in, out, bc, dt_dev, static_cast<hipStream_t>(stream));
+}
+
+int pxh_roe_dt(const pxh_grid *g, const pxh_fields *s, double cfl, void *dt_out, void *ws, void *stream) {
+    if (!grid_ok(g) || !fields_ok(s, false) || !dt_out || !ws) return PXH_E_ARG;
+    return DISPATCH(g, roe_dt, g, s, cfl, dt_out, ws, static_cast<hipStream_t>(stream));
+}
+
+int pxh_bc_apply(const pxh_grid *g, const pxh_fields *s, const pxh_bc *bc, void *stream) {
+    if (!grid_ok(g) || !s || !bc) return PXH_E_ARG;
+    if (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries)) return PXH_E_ARG;
+    return DISPATCH(g, bc_apply_t, g, s, bc, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+"""run_driver (runtime/driver.py:17-62): YAML -> mesh/state -> boundaries ->
+solver bundle -> integrator bundle -> run.
+
+``run_driver`` takes the path of a YAML file (or the parsed dict) whose
+``input_file`` is a NetCDF-3 initial condition.  ``setup_case`` / ``run_case``
+take in-memory arrays instead (the form the synthetic workloads use, like
+tests/runtime/2D_dambreak/run2D_dambreak.py:139-162).
+"""
+import os
+
+import numpy as np
+import torch
+import yaml
+
+from ..boundaries import BoundaryManager
+from ..inputs import mesh_and_arrays_from_netcdf, seds_from_params, initial_G
+from ..integrators.registry import create_integrator_bundle
+from ..parallel import Parallel
+from ..solvers.registry import create_solver_bundle
+from ..state import RoeExnerState, RoeState
+
+
+def build_state(scheme, arrays, params, dtype=torch.float32, device="cuda"):
+    if scheme == "Roe Exner":
+        arr = dict(arrays)
+        if "G" not in arr:
+            arr["G"] = initial_G(params, np.shape(arr["h"]))
+        return RoeExnerState.from_arrays(arr, seds_from_params(params), dtype=dtype, device=device)
+    if scheme == "Roe":
+        return RoeState.from_arrays(arrays, dtype=dtype, device=device)
+    raise KeyError(f"Unknown state '{scheme}'. Available options: ['Roe', 'Roe Exner']")
+
+
+def setup_case(mesh, arrays, params, dtype=torch.float32, device="cuda"):
+    scheme = params.get("flux_scheme", "Roe")
+    mpi = Parallel(params)
+    state = build_state(scheme, arrays, params, dtype, device)
+    boundaries = BoundaryManager(params, mesh.local_X, mesh.local_Y, device=device)
+    solver = create_solver_bundle(scheme)
+    solver_config = solver.config(state, mpi, boundaries, mesh.dh, params)
+    integ = create_integrator_bundle(params.get("integrator", "Forward Euler"))
+    integ_config = integ.config(params.get("cfl", 0.5), params.get("end_time", 1), params.get("out_freq", 1),
+                                solver, solver_config)
+    return state, boundaries, solver, solver_config, integ, integ_config
+
+
+def run_case(mesh, arrays, params, dtype=torch.float32, device="cuda", io=None):
+    state, boundaries, solver, sc, integ, ic = setup_case(mesh, arrays, params, dtype, device)
+    state = integ.run_fn(state, ic, io, mesh, boundaries.boundary_mask)
+    return state, (mesh.local_X, mesh.local_Y)
+
+
+def run_driver(config_path, dtype=torch.float32, device="cuda", io=None):
+    if isinstance(config_path, dict):
+        params, base = config_path, os.getcwd()
+    else:
+        with open(config_path, "r") as f:
+            params = yaml.safe_load(f)
+        base = os.path.dirname(os.path.abspath(config_path))
+    path = params["input_file"]
+    if not os.path.isabs(path):
+        path = os.path.join(base, path)
+    mesh, arrays = mesh_and_arrays_from_netcdf(path, params)
+    return run_case(mesh, arrays, params, dtype, device, io)

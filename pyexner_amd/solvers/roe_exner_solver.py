@@ -1,0 +1,167 @@
+"""The "Roe Exner" solver bundle (solvers/roe_exner_solver.py:21-133) on the GPU.
+
+Same names, signatures and functional semantics as the reference bundle:
+``mask_fn(state, mpi_handler, b_mask)``, ``init_fn(state, mask, config)``,
+``step_fn(state, time, dt, mask, config)``, ``compute_dt_fn(state, cfl, mask,
+config)``.  Each returns new tensors; ``dt`` stays on the device (a 0-d tensor)
+so a loop of steps never waits on the host.
+"""
+import numpy as np
+import torch
+
+from .. import _hip
+from ..constants import g
+from ..parallel import identity_halo
+from ..state import dtype_code, fields_struct
+from .registry import SolverBundle, SolverConfig, register_solver_bundle
+
+
+def sediment_table(seds, constant, np_dtype):
+    """pxh_sediments from seds (7, nsed) = [frac, d50, density, k_d, k_e, theta_c,
+    porosity] (io/pnetcdf_reader.py:146-158), evaluated in the working dtype the
+    way compute_G / compute_n do (roe_exner.py:64-130)."""
+    sd = _hip.Sediments()
+    if constant:
+        sd.n_sed, sd.mpm, sd.one_minus_por, sd.nb_sed = 0, 0, 1.0, 0.0
+        return sd
+    s7 = np.asarray(seds, dtype=np_dtype)
+    if s7.ndim != 2 or s7.shape[0] != 7:
+        raise ValueError(f"seds must be (7, nsed), got {s7.shape}")
+    nsed = s7.shape[1]
+    if not 1 <= nsed <= _hip.MAX_SED:
+        raise ValueError(f"1..{_hip.MAX_SED} sediment classes supported, got {nsed}")
+    frac, d50, density, k_d, k_e, theta_c, porosity = s7
+    s = density / np_dtype(1000.0)
+    np_ = d50 ** np_dtype(1 / 6) / np_dtype(21.1)
+    sqrt_g = np.sqrt(np_dtype(g))
+    A = np_ * np_
+    B = (s - np_dtype(1.0)) * d50
+    Cc = np_ * (np_ * np_) * sqrt_g
+    D = s - np_dtype(1.0)
+    sd.n_sed, sd.mpm = nsed, 1
+    sd.one_minus_por = float(np_dtype(1.0) - porosity.mean(dtype=np_dtype))
+    d50m = np.mean(d50 * frac, dtype=np_dtype)
+    sd.nb_sed = float(d50m ** np_dtype(1 / 6) / np_dtype(21.1))
+    for k in range(nsed):
+        sd.frac[k], sd.A[k], sd.B[k] = float(frac[k]), float(A[k]), float(B[k])
+        sd.C[k], sd.D[k], sd.theta_c[k] = float(Cc[k]), float(D[k]), float(theta_c[k])
+    return sd
+
+
+class ExnerBackend:
+    """Host-side description of one domain: grid, sediment table, ring-fill
+    programs and the dt workspace.  Built once by config_fn."""
+
+    STEP_GROUPS = (_hip.G_H, _hip.G_HUV, _hip.G_ZB, _hip.G_G)
+
+    def __init__(self, shape, dtype, dx, seds, constant, boundaries, device="cuda"):
+        self.shape = tuple(shape)
+        self.dtype = dtype
+        self.np_dtype = np.float32 if dtype == torch.float32 else np.float64
+        self.dx = float(dx)
+        self.grid = _hip.Grid(self.shape[0], self.shape[1], dtype_code(dtype), 0, self.dx)
+        self.constant = constant
+        self.sed = sediment_table(seds, constant, self.np_dtype)
+        self.boundaries = boundaries
+        if boundaries is not None and boundaries.n_entries and not boundaries.idempotent:
+            raise NotImplementedError("boundary polygons whose ring fill reads cells another handler writes "
+                                      "are not supported by the fused ring fill")
+        self.bc_all = boundaries.program() if boundaries is not None else None
+        self.bc_step = boundaries.program(self.STEP_GROUPS) if boundaries is not None else None
+        ws = _hip.load().pxh_dt_workspace_size(self.grid)
+        if ws < 0:
+            raise ValueError("invalid grid")
+        self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
+        self.device = device
+
+    # raw entry points on preallocated buffers (no allocation) -----------------
+    def init_into(self, fs, mask0_ptr, stream):
+        _hip.call("pxh_roe_exner_init", self.grid, fs, mask0_ptr, self.sed, stream)
+        if self.bc_all is not None and self.bc_all.n_entries:
+            _hip.call("pxh_bc_apply", self.grid, fs, self.bc_all, stream)
+
+    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream):
+        _hip.call("pxh_roe_exner_step", self.grid, fin, fout, mask0_ptr, self.sed, self.bc_step, dt_ptr, stream)
+
+    def dt_into(self, fs, mask0_ptr, cfl, dt_ptr, stream):
+        _hip.call("pxh_roe_exner_dt", self.grid, fs, mask0_ptr, float(cfl), dt_ptr, self.workspace.data_ptr(),
+                  stream)
+
+    def combine_into(self, fa, fb, fo, stream):
+        _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dt_tensor(dt, dtype, device):
+    if isinstance(dt, torch.Tensor):
+        t = dt.to(device=device, dtype=dtype)
+        return t.reshape(()).contiguous()
+    return torch.full((), float(dt), dtype=dtype, device=device)
+
+
+def get_mask(state, mpi_handler=None, b_mask=None):
+    """roe_exner_solver.py:21-39 on one rank: nodata = isnan(z) plus the outer
+    ring, minus boundary cells; returns bool (2, Ny, Nx) = [nodata, b_mask]."""
+    z = state.z
+    mask = torch.isnan(z)
+    mask[0, :] = True
+    mask[-1, :] = True
+    mask[:, 0] = True
+    mask[:, -1] = True
+    if b_mask is None:
+        b_mask = torch.zeros_like(mask)
+    bm = torch.as_tensor(np.asarray(b_mask), device=z.device).to(torch.bool) if not isinstance(
+        b_mask, torch.Tensor) else b_mask.to(device=z.device, dtype=torch.bool)
+    mask = torch.where(bm, torch.zeros_like(mask), mask)
+    return torch.stack([mask, bm], dim=0).contiguous()
+
+
+def config_fn_roeexner(state, mpi_handler, boundaries, dx, params):
+    grass = params["erosion"]["grass_factor"]
+    constant = not isinstance(grass, str)
+    backend = ExnerBackend(state.h.shape, state.h.dtype, dx, state.seds, constant, boundaries,
+                           device=state.h.device)
+    return SolverConfig(mpi_handler=mpi_handler, boundaries=boundaries, dx=dx, halo_exchange=identity_halo,
+                        compute_G="constant" if constant else "MPM", compute_n="constant" if constant else "MPM",
+                        backend=backend)
+
+
+def _mask0_ptr(mask):
+    m0 = mask[0]
+    if m0.dtype not in (torch.bool, torch.uint8) or not m0.is_contiguous():
+        raise TypeError("mask must be a contiguous bool/uint8 (2, Ny, Nx) tensor")
+    return m0.data_ptr()
+
+
+def init_fn_roeexner(state, mask, config):
+    """roe_exner_solver.py:58-78: n_b, G, then the ring fill at t = 0."""
+    s = state.clone()
+    config.backend.init_into(fields_struct(s), _mask0_ptr(mask), _stream())
+    return s
+
+
+def step_fn_roeexner(state, time, dt, mask, config):
+    """roe_exner_solver.py:80-116: one Forward-Euler step, out of place."""
+    be = config.backend
+    out = state.replace(h=torch.empty_like(state.h), hu=torch.empty_like(state.hu),
+                        hv=torch.empty_like(state.hv), z_b=torch.empty_like(state.z_b),
+                        G=torch.empty_like(state.G))
+    dtt = _dt_tensor(dt, state.h.dtype, state.h.device)
+    be.step_into(fields_struct(state), fields_struct(out), _mask0_ptr(mask), dtt.data_ptr(), _stream())
+    return out
+
+
+def compute_dt_roeexner(state, cfl, mask, config):
+    """roe_exner_solver.py:118-122: max(cfl * min over active edges, 1e-6), on device."""
+    dt = torch.empty((), dtype=state.h.dtype, device=state.h.device)
+    config.backend.dt_into(fields_struct(state), _mask0_ptr(mask), cfl, dt.data_ptr(), _stream())
+    return dt
+
+
+@register_solver_bundle("Roe Exner")
+def solver_roeexner():
+    return SolverBundle(name="Roe Exner", config=config_fn_roeexner, mask_fn=get_mask, init_fn=init_fn_roeexner,
+                        step_fn=step_fn_roeexner, compute_dt_fn=compute_dt_roeexner)

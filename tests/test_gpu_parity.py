@@ -1,0 +1,171 @@
+"""GPU parity: the HIP path through the C-ABI against the float64 oracle on the
+same seeded inputs.
+
+Tolerances (written here, checked below):
+* float64 build: relative L-inf per field <= 1e-12 (against max |oracle field|),
+  after 1000 SSPRK2 steps on a small channel and after full runs of the other
+  cases.
+* float32 build (cubic in float64): h, hu <= 1e-5 relative; hv <= 1e-5 of
+  max|hu| (hv is a small cross-flow); z_b <= 1e-4 of max |z_b - z_b(0)|.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import steps as osteps
+from oracle.closed_forms import stoker_wet_dambreak
+from pyexner_amd import cases
+from tests.helpers import (FIELDS_EXNER, FIELDS_ROE, compare, gpu_case, gpu_fe, gpu_ssprk2, oracle_exner,
+                           oracle_roe, rel_linf)
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+
+
+def _print(tag, errs):
+    print(tag, " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
+
+
+def test_init_and_dt_f64(cuda):
+    mesh, arr, params = cases.channel(nx=96, ny=40, n_bumps=12, seed=5)
+    so, prob = oracle_exner(mesh, arr, params)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    errs = compare(sg, so, ("h", "hu", "hv", "z_b", "G", "n_b"))
+    _print("init", errs)
+    assert max(errs.values()) <= 1e-14
+    dto = prob.dt(so)
+    dtg = float(solver.compute_dt_fn(sg, prob.cfl, mask, sc))
+    assert abs(dtg - float(dto)) <= 1e-14 * float(dto)
+
+
+def test_bc_apply_matches_oracle(cuda):
+    mesh, arr, params = cases.channel(nx=40, ny=24, n_bumps=4, seed=6)
+    rng = np.random.default_rng(0)
+    so, prob = oracle_exner(mesh, arr, params)
+    for f in ("h", "hu", "hv", "z_b", "G", "n"):
+        setattr(so, f, rng.uniform(-1, 1, so.h.shape))
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    for f in ("h", "hu", "hv", "z_b", "G", "n"):
+        getattr(sg, f).copy_(torch.from_numpy(getattr(so, f)))
+    out_g = sc.boundaries.apply(sg, 0.0).to_host()
+    out_o = prob.bcs.apply(so)
+    for f in ("h", "hu", "hv", "z_b", "G", "n"):
+        assert np.array_equal(getattr(out_g, f), getattr(out_o, f)), f
+
+
+@pytest.mark.parametrize("integrator", ["Forward Euler", "SSPRK2"])
+def test_exner_one_step_f64(cuda, integrator):
+    mesh, arr, params = cases.channel(nx=96, ny=40, n_bumps=12, seed=5)
+    so, prob = oracle_exner(mesh, arr, params)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    if integrator == "Forward Euler":
+        so, _ = osteps.fe_step(prob, so)
+        sg, _ = gpu_fe(sg, mask, solver, sc, prob.cfl)
+    else:
+        so, _ = osteps.ssprk2_step(prob, so)
+        sg, _ = gpu_ssprk2(sg, mask, solver, sc, prob.cfl)
+    errs = compare(sg, so, FIELDS_EXNER)
+    _print(integrator, errs)
+    assert max(errs.values()) <= 1e-14
+
+
+def test_exner_ssprk2_1000_steps_f64(cuda):
+    mesh, arr, params = cases.channel(nx=64, ny=32, n_bumps=10, seed=7)
+    so, prob = oracle_exner(mesh, arr, params)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    for _ in range(1000):
+        so, _ = osteps.ssprk2_step(prob, so)
+        sg, _ = gpu_ssprk2(sg, mask, solver, sc, prob.cfl)
+    errs = compare(sg, so, FIELDS_EXNER)
+    _print("ssprk2 x1000", errs)
+    assert max(errs.values()) <= F64_TOL
+
+
+def test_exner_radial_dambreak_fe_f64(cuda):
+    mesh, arr, params = cases.radial_dambreak(nx=80, ny=72, dx=0.05, n_bumps=16, radius=1.0)
+    so, prob = oracle_exner(mesh, arr, params)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    for _ in range(300):
+        so, _ = osteps.fe_step(prob, so)
+        sg, _ = gpu_fe(sg, mask, solver, sc, prob.cfl)
+    errs = compare(sg, so, FIELDS_EXNER)
+    _print("radial fe x300", errs)
+    assert max(errs.values()) <= F64_TOL
+
+
+def test_exner_f32_within_tolerance(cuda):
+    mesh, arr, params = cases.channel(nx=96, ny=40, n_bumps=12, seed=8)
+    so, prob = oracle_exner(mesh, arr, params)
+    zb0 = so.z_b.copy()
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+    for _ in range(300):
+        so, _ = osteps.ssprk2_step(prob, so)
+        sg, _ = gpu_ssprk2(sg, mask, solver, sc, prob.cfl)
+    hg = sg.to_host()
+    e_h = rel_linf(hg.h, so.h)
+    e_hu = rel_linf(hg.hu, so.hu)
+    e_hv = np.max(np.abs(hg.hv - so.hv)) / np.max(np.abs(so.hu))
+    dzb = np.max(np.abs(so.z_b - zb0))
+    e_zb = np.max(np.abs(hg.z_b - so.z_b)) / dzb
+    print(f"f32 x300: h={e_h:.3e} hu={e_hu:.3e} hv/|hu|={e_hv:.3e} zb/|dzb|={e_zb:.3e} (dzb={dzb:.3e})", flush=True)
+    assert e_h <= 1e-5 and e_hu <= 1e-5 and e_hv <= 1e-5
+    assert e_zb <= 1e-4
+
+
+def test_roe_dambreak_c1_f64(cuda):
+    """Plain Roe, Forward Euler, fp64, 512 x 32, to t = 2 s: oracle parity and Stoker."""
+    from pyexner_amd.runtime.driver import run_case
+    from pyexner_amd.integrators import forwardeuler
+    mesh, arr, params = cases.dambreak_1d()
+    so, prob = oracle_roe(mesh, arr, params)
+    so, t_o, it_o = osteps.run_forward_euler(prob, so, params["end_time"], params["out_freq"])
+    sg, _ = run_case(mesh, arr, params, dtype=torch.float64)
+    assert forwardeuler.last_run["iters"] == it_o
+    errs = compare(sg, so, FIELDS_ROE)
+    _print("roe c1", errs)
+    assert max(errs.values()) <= F64_TOL
+    # Stoker: the interior row against the closed form at the final time
+    x = mesh.local_X[16, 1:-1]
+    h_ex, u_ex = stoker_wet_dambreak(x, t_o, 1.0, 0.5, 25.6)
+    hg = sg.to_host().h[16, 1:-1]
+    l1 = np.mean(np.abs(hg - h_ex))
+    print(f"stoker L1(h) = {l1:.4e} at t = {t_o:.4f}", flush=True)
+    assert l1 < 5e-3
+    # column-uniform data stays column-uniform
+    hh = sg.to_host().h[1:-1, 1:-1]
+    assert np.max(np.abs(hh - hh[0:1, :])) <= 1e-12
+
+
+def test_periodic_translation_invariance_f64(cuda):
+    mesh, arr, params = cases.periodic_box(nx=40, ny=30)
+    shift = 7
+
+    def roll(a):
+        inner = a[1:-1, 1:-1]
+        out = a.copy()
+        out[1:-1, 1:-1] = np.roll(inner, shift, axis=1)
+        return out
+
+    arr2 = {k: roll(v) for k, v in arr.items()}
+    sa, ma, sol, sca = gpu_case(mesh, arr, params, torch.float64)
+    sb, mb, _, scb = gpu_case(mesh, arr2, params, torch.float64)
+    mass0 = sa.h[1:-1, 1:-1].sum().item()
+    for _ in range(50):
+        sa, _ = gpu_ssprk2(sa, ma, sol, sca, 0.5)
+        sb, _ = gpu_ssprk2(sb, mb, sol, scb, 0.5)
+    ha, hb = sa.to_host(), sb.to_host()
+    for f in FIELDS_EXNER:
+        assert np.array_equal(roll(getattr(ha, f))[1:-1, 1:-1], getattr(hb, f)[1:-1, 1:-1]), f
+    mass1 = ha.h[1:-1, 1:-1].sum()
+    assert abs(mass1 - mass0) <= 1e-12 * mass0
+
+
+def test_reflective_box_mass_conservation_f64(cuda):
+    mesh, arr, params = cases.radial_dambreak(nx=48, ny=48, dx=0.05, n_bumps=0, radius=0.6)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    m0 = sg.h[1:-1, 1:-1].sum().item()
+    for _ in range(100):
+        sg, _ = gpu_fe(sg, mask, solver, sc, 0.5)
+    m1 = sg.h[1:-1, 1:-1].sum().item()
+    assert abs(m1 - m0) <= 1e-12 * m0

@@ -1,0 +1,197 @@
+"""Host-side logic (no GPU): the C-ABI library's exports, the composed ring
+fill against the oracle's sequential handlers, the sediment table, polygon
+rasterisation and the input front end."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import boundaries as ob
+from oracle import roe_exner as rx
+from oracle.state import ExnerState, RoeState
+from pyexner_amd import _hip, cases
+from pyexner_amd.boundaries import BoundaryManager, compose, compute_label_mask
+from pyexner_amd.inputs import seds_from_params
+from pyexner_amd.solvers.roe_exner_solver import sediment_table
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "pyexner_hip.h")).read()
+    names = sorted(set(re.findall(r"\b(pxh_[A-Za-z0-9_]+)\s*\(", hdr)))
+    assert len(names) >= 10
+    lib = _hip.load()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_hip._SIGS), set(names) ^ set(_hip._SIGS)
+    assert lib.pxh_version() >= 1
+    assert lib.pxh_error_string(-1) == b"invalid argument"
+
+
+def _emulate_program(entries, handlers, state, groups=None):
+    """Apply the composed program the way k_bc does (every read before any write)."""
+    out = state.copy()
+    ny, nx = state.h.shape
+    names = {_hip.G_H: "h", _hip.G_Z: "z", _hip.G_ZB: "z_b", _hip.G_N: "n", _hip.G_NB: "n_b", _hip.G_G: "G"}
+    for e in entries:
+        g_ = int(e["group"])
+        if groups is not None and g_ not in groups:
+            continue
+        t, s = divmod(int(e["target"]), nx), divmod(int(e["src"]), nx)
+        if g_ == _hip.G_HUV:
+            hu, hv = state.hu[s], state.hv[s]
+            for q in range(int(e["nchain"])):
+                hd = handlers[int(e["chain"][q])]
+                nxv, nyv = hd.normal
+                if hd.kind == _hip.BC_REFLECT_RX:
+                    mn = hu * nxv + hv * nyv
+                    mt = -hu * nyv + hv * nxv
+                    mr = -mn
+                    hu, hv = mr * nxv - mt * nyv, mr * nyv + mt * nxv
+                elif hd.kind == _hip.BC_REFLECT_ROE:
+                    hu, hv = hu * (1 - 2 * nxv * nxv), hv * (1 - 2 * nyv * nyv)
+            out.hu[t], out.hv[t] = hu, hv
+        else:
+            getattr(out, names[g_])[t] = getattr(state, names[g_])[s]
+    return out
+
+
+def _random_exner(shape, rng):
+    return ExnerState(**{k: rng.uniform(-1, 1, shape) for k in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")},
+                      seds=np.zeros((7, 1)))
+
+
+@pytest.mark.parametrize("builder,order", [
+    ("channel", None), ("radial", None), ("periodic", None),
+    ("radial", [3, 0, 2, 1]), ("channel", [5, 4, 0, 3, 1, 2]),
+])
+def test_composed_program_equals_sequential_handlers(builder, order):
+    rng = np.random.default_rng(7)
+    if builder == "channel":
+        mesh, arr, params = cases.channel(nx=14, ny=9, n_bumps=1)
+    elif builder == "radial":
+        mesh, arr, params = cases.radial_dambreak(nx=11, ny=13, n_bumps=1, radius=0.2)
+    else:
+        mesh, arr, params = cases.periodic_box(nx=12, ny=10)
+    specs = params["boundaries"]
+    if order is not None:
+        keys = list(specs)
+        specs = {keys[i]: specs[keys[i]] for i in order}
+    bm = BoundaryManager({"flux_scheme": "Roe Exner", "boundaries": specs}, mesh.local_X, mesh.local_Y,
+                         device="cpu")
+    entries, idem = compose(bm.boundary_handlers, mesh.local_shape)
+    assert idem
+    oc = ob.Boundaries("Roe Exner", specs, mesh.local_X, mesh.local_Y)
+    np.testing.assert_array_equal(bm.boundary_mask, oc.boundary_mask)
+    st = _random_exner(mesh.local_shape, rng)
+    got = _emulate_program(entries, bm.boundary_handlers, st)
+    exp = oc.apply(st)
+    for f in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G"):
+        np.testing.assert_array_equal(getattr(got, f), getattr(exp, f), err_msg=f)
+    # idempotent: applying twice equals once
+    again = _emulate_program(entries, bm.boundary_handlers, got)
+    for f in ("h", "hu", "hv", "z_b", "G"):
+        np.testing.assert_array_equal(getattr(again, f), getattr(got, f))
+
+
+def test_composed_program_roe_reflective_and_oblique_normal():
+    rng = np.random.default_rng(8)
+    mesh, arr, params = cases.dambreak_1d(nx=10, ny=6)
+    specs = dict(params["boundaries"])
+    specs["north"] = dict(specs["north"], normal=[0.6, 0.8])  # oblique normal: rounding pinned too
+    bm = BoundaryManager({"flux_scheme": "Roe", "boundaries": specs}, mesh.local_X, mesh.local_Y, device="cpu")
+    entries, idem = compose(bm.boundary_handlers, mesh.local_shape)
+    oc = ob.Boundaries("Roe", specs, mesh.local_X, mesh.local_Y)
+    st = RoeState(**{k: rng.uniform(-1, 1, mesh.local_shape) for k in ("h", "hu", "hv", "z", "n")})
+    ex = ExnerState(h=st.h, hu=st.hu, hv=st.hv, z=st.z, z_b=st.z * 0, n=st.n, n_b=st.n * 0, G=st.n * 0,
+                    seds=None)
+    got = _emulate_program(entries, bm.boundary_handlers, ex)
+    exp = oc.apply(st)
+    for f in ("h", "hu", "hv", "z", "n"):
+        np.testing.assert_array_equal(getattr(got, f), getattr(exp, f), err_msg=f)
+
+
+def test_non_idempotent_layout_is_detected():
+    # a two-cell-thick west polygon: the outer cell reads a cell the same handler writes
+    mesh, arr, params = cases.channel(nx=12, ny=8, n_bumps=1)
+    X, dh = mesh.local_X, mesh.dh
+    poly = [[X[0, 0] - dh / 2, -10], [X[0, 1] + dh / 2, -10], [X[0, 1] + dh / 2, 10], [X[0, 0] - dh / 2, 10]]
+    specs = {"w": {"type": "Transmissive", "polygon": poly, "values": [0], "normal": [-1.0, 0.0]}}
+    bm = BoundaryManager({"flux_scheme": "Roe Exner", "boundaries": specs}, mesh.local_X, mesh.local_Y,
+                         device="cpu")
+    assert not bm.idempotent
+
+
+def test_sediment_table_matches_oracle_terms_float32():
+    """The per-class constants precomputed on the host give the oracle's
+    Shields number and gamma_1 bit for bit in float32."""
+    params = cases.channel(nx=8, ny=8, n_bumps=1)[2]
+    seds = seds_from_params(params).astype(np.float32)
+    sd = sediment_table(seds, False, np.float32)
+    rng = np.random.default_rng(3)
+    h = rng.uniform(0.05, 1, 1000).astype(np.float32)
+    vel2 = rng.uniform(0, 4, 1000).astype(np.float32)
+    frac, d50, density, k_d, k_e, theta_c, porosity = seds
+    s = density / np.float32(1000.0)
+    np_ = d50 ** np.float32(1 / 6) / np.float32(21.1)
+    th_or = (np_[0] * np_[0] * vel2) / ((s[0] - np.float32(1.0)) * d50[0] * np.cbrt(h))
+    th_k = (np.float32(sd.A[0]) * vel2) / (np.float32(sd.B[0]) * np.cbrt(h))
+    np.testing.assert_array_equal(th_or, th_k)
+    g1_or = np_[0] * (np_[0] * np_[0]) * np.sqrt(np.float32(9.81)) / ((s[0] - np.float32(1.0)) * np.sqrt(h))
+    g1_k = np.float32(sd.C[0]) / (np.float32(sd.D[0]) * np.sqrt(h))
+    np.testing.assert_array_equal(g1_or, g1_k)
+    nb = rx.compute_n(np.zeros((1, 1), np.float32), np.ones((1, 1), np.float32), seds)
+    assert np.float32(sd.nb_sed) == nb[0, 0]
+
+
+def test_polygon_rasterisation_against_matplotlib():
+    mpl = pytest.importorskip("matplotlib.path")
+    rng = np.random.default_rng(5)
+    X, Y = np.meshgrid(np.linspace(0, 1, 101), np.linspace(1, 0, 77))
+    poly = rng.uniform(0.1, 0.9, (7, 2))
+    poly = poly[np.argsort(np.arctan2(poly[:, 1] - 0.5, poly[:, 0] - 0.5))]
+    ours = compute_label_mask(X, Y, poly)
+    ref = mpl.Path(poly).contains_points(np.stack([X.ravel(), Y.ravel()], -1)).reshape(X.shape)
+    # compare away from the polygon outline
+    from scipy.spatial.distance import cdist  # noqa: F401
+    edge = np.zeros_like(ours)
+    for a, b in zip(poly, np.roll(poly, 1, axis=0)):
+        t = np.clip(((X - a[0]) * (b[0] - a[0]) + (Y - a[1]) * (b[1] - a[1])) / np.sum((b - a) ** 2), 0, 1)
+        d = np.hypot(X - (a[0] + t * (b[0] - a[0])), Y - (a[1] + t * (b[1] - a[1])))
+        edge |= d < 1e-6
+    np.testing.assert_array_equal(ours[~edge], ref[~edge])
+
+
+def test_netcdf_front_end(tmp_path):
+    from scipy.io import netcdf_file
+    from pyexner_amd.inputs import mesh_and_arrays_from_netcdf
+    nx, ny, dh = 9, 5, 0.1
+    x = np.arange(nx) * dh
+    y = np.arange(ny - 1, -1, -1) * dh
+    p = tmp_path / "in.nc"
+    with netcdf_file(str(p), "w", version=2) as f:
+        f.createDimension("x", nx)
+        f.createDimension("y", ny)
+        vx = f.createVariable("x", "f4", ("x",))
+        vx[:] = x
+        vy = f.createVariable("y", "f4", ("y",))
+        vy[:] = y
+        for k, val in (("h", 0.5), ("hu", 0.1), ("hv", 0.0), ("z", 0.0), ("z_b", 0.02), ("n", 0.0)):
+            v = f.createVariable(k, "f8", ("y", "x"))
+            v[:] = np.full((ny, nx), val)
+    params = {"flux_scheme": "Roe Exner", "erosion": {"grass_factor": 0.005}}
+    mesh, arrays = mesh_and_arrays_from_netcdf(str(p), params)
+    assert mesh.local_shape == (ny, nx) and mesh.dh == 0.1
+    np.testing.assert_allclose(mesh.local_X[0], x + 0.05, atol=1e-6)
+    np.testing.assert_allclose(mesh.local_Y[:, 0], y - 0.05, atol=1e-6)
+    assert np.all(arrays["G"] == 0.005) and np.all(arrays["h"] == 0.5)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
