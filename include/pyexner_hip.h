@@ -42,17 +42,24 @@ extern "C" {
 #define PXH_MAX_CHAIN 6
 
 /* Grid geometry. */
+#define PXH_EXACT_F32 1  /* flags bit: float32 with IEEE division/sqrt/libm (default: hardware rcp/rsq/log2/exp2) */
+
 typedef struct {
     int32_t ny, nx;   /* array shape, ghost ring included */
     int32_t dtype;    /* PXH_F32 or PXH_F64 */
-    int32_t reserved;
+    int32_t flags;    /* PXH_EXACT_F32 */
     double dx;        /* cell size (dx == dy) */
 } pxh_grid;
 
 /* Device pointers to the structure-of-arrays state (state/roe_exner_state.py:8-48;
- * the plain Roe state uses h, hu, hv, z, n only).  Each is (ny, nx). */
+ * the plain Roe state uses h, hu, hv, z, n only).  Each is (ny, nx).
+ * z_b_lo (float32 only, may be NULL): the low part of a compensated bed,
+ * z_b + z_b_lo, |z_b_lo| <= ulp(z_b)/2; the Exner update and the SSPRK2
+ * combine then accumulate the bed without float32 round-off drift while every
+ * other kernel reads z_b. */
 typedef struct {
     void *h, *hu, *hv, *z, *z_b, *n, *n_b, *G;
+    void *z_b_lo;
 } pxh_fields;
 
 /* Sediment table, precomputed on the host in the working dtype from
@@ -67,6 +74,7 @@ typedef struct {
     double nb_sed;
     double frac[PXH_MAX_SED], A[PXH_MAX_SED], B[PXH_MAX_SED], C[PXH_MAX_SED],
            D[PXH_MAX_SED], theta_c[PXH_MAX_SED];
+    double AB[PXH_MAX_SED], CD[PXH_MAX_SED];  /* A/B and C/D for the hardware-math float32 path */
 } pxh_sediments;
 
 /* Composed ghost-cell program (domain/boundary_registry.py:183-187 applied in
@@ -137,6 +145,15 @@ int pxh_compute_G(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *
                   const pxh_sediments *sed, void *stream);
 int pxh_exner_solve_2D(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out, const uint8_t *mask0,
                        const void *dt_dev, void *stream);
+
+/* One whole SSPRK2 step on three caller-owned buffer sets (integrators/SSPRK2.py:32-40):
+ * dt = compute_dt_fn(a) into dt_dev; b = step_fn(a, dt); c = step_fn(b, dt);
+ * a = 0.5 a + 0.5 c.  a is updated in place; b and c are scratch whose z, n,
+ * n_b may share a's (static fields); the evolving fields of a, b, c must not
+ * alias.  workspace as for pxh_roe_exner_dt. */
+int pxh_roe_exner_ssprk2(const pxh_grid *grid, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
+                         const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc, double cfl,
+                         void *dt_dev, void *workspace, void *stream);
 
 /* compute_dt_fn: *dt_out = max(cfl * min over active edges, 1e-6) written to
  * device memory.  workspace must hold pxh_dt_workspace_size(grid) bytes. */

@@ -21,19 +21,25 @@ GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G
                 G_NB: ("n_b",), G_G: ("G",)}
 
 
+EXACT_F32 = 1
+
+
 class Grid(ctypes.Structure):
     _fields_ = [("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("dtype", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("dx", ctypes.c_double)]
+                ("flags", ctypes.c_int32), ("dx", ctypes.c_double)]
+
+
+FIELD_NAMES = ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G", "z_b_lo")
 
 
 class Fields(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")]
+    _fields_ = [(n, ctypes.c_void_p) for n in FIELD_NAMES]
 
 
 class Sediments(ctypes.Structure):
     _fields_ = [("n_sed", ctypes.c_int32), ("mpm", ctypes.c_int32), ("one_minus_por", ctypes.c_double),
                 ("nb_sed", ctypes.c_double)] + [
-        (n, ctypes.c_double * MAX_SED) for n in ("frac", "A", "B", "C", "D", "theta_c")]
+        (n, ctypes.c_double * MAX_SED) for n in ("frac", "A", "B", "C", "D", "theta_c", "AB", "CD")]
 
 
 class BCEntry(ctypes.Structure):
@@ -69,6 +75,9 @@ _SIGS = {
     "pxh_roe_dt": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p]),
     "pxh_bc_apply": (ctypes.c_int, [_P(Grid), _P(Fields), _P(BC), ctypes.c_void_p]),
+    "pxh_roe_exner_ssprk2": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
+                                            _P(Sediments), _P(BC), ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_roe_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
     "pxh_momentum_corrections": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),

@@ -121,6 +121,28 @@ def radial_dambreak(nx=8192, ny=8192, dx=0.05, n_bumps=256, seed=3, radius=50.0,
     return mesh, arrays, _exner_params(mesh, bounds)
 
 
+def wetdry_obstacle(nx=150, ny=70, dx=0.05, seed=9):
+    """Dam break onto a dry bed with an emergent block and a sloping erodible
+    layer: wet/dry fronts, cells whose water surface is below a neighbour's
+    bed, and a width/height that are not multiples of the sweep strips."""
+    mesh = uniform_mesh(ny, nx, dx)
+    rng = np.random.default_rng(seed)
+    X, Y = mesh.local_X, mesh.local_Y
+    Lx = X[0, -1] - X[0, 0]
+    h = np.where(X < X[0, 0] + Lx / 3, 0.4, 0.0)
+    z = np.zeros_like(h)
+    blk = (np.abs(X - (X[0, 0] + 0.55 * Lx)) < 0.3) & (np.abs(Y - Y[:, 0].mean()) < 0.5)
+    z[blk] = 0.8
+    zb = gaussian_bumps(X, Y, rng, 6, amp=(0.0, 0.05), sigma=(0.1, 0.4)) + 0.02 * (X - X[0, 0]) / Lx
+    h = np.where(blk, 0.0, h)
+    arrays = {"h": h, "hu": np.zeros_like(h), "hv": np.zeros_like(h), "z": z, "z_b": zb,
+              "n": np.full_like(h, 0.02), "G": np.zeros_like(h)}
+    bounds = ring_boundaries(mesh, [("west", "Reflective", "W"), ("east", "Transmissive", "E"),
+                                    ("east_bed", "TransmissiveBed", "E"), ("south", "Reflective", "S"),
+                                    ("north", "Reflective", "N")])
+    return mesh, arrays, _exner_params(mesh, bounds)
+
+
 def periodic_box(nx=64, ny=48, dx=0.05, seed=11, exner=True):
     """Smooth periodic-compatible state for translation/conservation checks."""
     mesh = uniform_mesh(ny, nx, dx)

@@ -50,8 +50,11 @@ __device__ __forceinline__ float inf_v<float>() { return __builtin_huge_valf(); 
 template <typename T>
 __device__ __forceinline__ T C(double v) { return static_cast<T>(v); }
 
-struct Geom {
-    int ny, nx;
+// Typed view of pxh_fields.
+template <typename T>
+struct F {
+    T *h, *hu, *hv, *z, *z_b, *n, *n_b, *G;
+    T *z_b_lo;  // compensated-bed low part (float32 only) or null
 };
 
 inline int launch_status() {

@@ -6,16 +6,14 @@
 //   hydro flux + update -> momentum corrections -> ring fill (h, hu, hv)
 //   -> G -> Exner flux + bed update -> ring fill (z_b, G)
 // and the CFL dt is a two-level min reduction into a device scalar.
+#include <initializer_list>
+
 #include "common.hpp"
 #include "flux_roe.hpp"
 #include "flux_roe_exner.hpp"
+#include "strip.hpp"
 
 namespace pxh {
-
-template <typename T>
-struct F {
-    T *h, *hu, *hv, *z, *z_b, *n, *n_b, *G;
-};
 
 template <typename T>
 static F<T> cast_fields(const pxh_fields *f) {
@@ -28,6 +26,7 @@ static F<T> cast_fields(const pxh_fields *f) {
     r.n = static_cast<T *>(f->n);
     r.n_b = static_cast<T *>(f->n_b);
     r.G = static_cast<T *>(f->G);
+    r.z_b_lo = sizeof(T) == 4 ? static_cast<T *>(f->z_b_lo) : nullptr;
     return r;
 }
 
@@ -322,7 +321,15 @@ __global__ __launch_bounds__(256) void k_combine(F<T> a, F<T> b, F<T> o, long nc
     o.h[i] = hf * a.h[i] + hf * b.h[i];
     o.hu[i] = hf * a.hu[i] + hf * b.hu[i];
     o.hv[i] = hf * a.hv[i] + hf * b.hv[i];
-    o.z_b[i] = hf * a.z_b[i] + hf * b.z_b[i];
+    T zc = hf * a.z_b[i] + hf * b.z_b[i];
+    if constexpr (sizeof(T) == 4) {
+        if (o.z_b_lo) {
+            T zl;
+            cb_half_sum(a.z_b[i], a.z_b_lo[i], b.z_b[i], b.z_b_lo[i], zc, zl);
+            o.z_b_lo[i] = zl;
+        }
+    }
+    o.z_b[i] = zc;
     o.G[i] = hf * a.G[i] + hf * b.G[i];
 }
 
@@ -374,7 +381,10 @@ __global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restri
     switch (e.group) {
         case PXH_G_H: s.h[t] = s.h[src]; break;
         case PXH_G_Z: s.z[t] = s.z[src]; break;
-        case PXH_G_ZB: s.z_b[t] = s.z_b[src]; break;
+        case PXH_G_ZB:
+            s.z_b[t] = s.z_b[src];
+            if (s.z_b_lo) s.z_b_lo[t] = s.z_b_lo[src];
+            break;
         case PXH_G_N: s.n[t] = s.n[src]; break;
         case PXH_G_NB: s.n_b[t] = s.n_b[src]; break;
         case PXH_G_G: s.G[t] = s.G[src]; break;
@@ -412,34 +422,91 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
     return launch_status();
 }
 
-template <typename T>
-static int rx_step(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
-                   const pxh_sediments *sd, const pxh_bc *bc, const void *dt, hipStream_t st) {
+// One Forward-Euler step (roe_exner_solver.py:80-116) as two strip sweeps and
+// two ring fills: hydro + momentum corrections -> fill (h, hu, hv) ->
+// G + Exner -> fill (z_b, G).
+template <typename T, bool FAST>
+static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
+                     const pxh_sediments *sd, const pxh_bc *bc, const void *dt, hipStream_t st) {
     F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     const T *dtp = static_cast<const T *>(dt);
-    hipLaunchKernelGGL(k_rx_hydro<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, dtp, ny, nx, dx);
-    if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_rx_momcorr<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, ny, nx);
+    const dim3 sg = strip_grid(ny, nx), sb(SW * SWPB);
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, in, out, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
-    const int ncell = ny * nx;
-    hipLaunchKernelGGL(k_rx_G<T>, dim3((ncell + 255) / 256), dim3(256), 0, st, in, out, m0, ncell, *sd);
-    if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_rx_exner<T>, grid2d(ny, nx), dim3(BX, BY), 0, st, in, out, m0, dtp, ny, nx, dx);
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0>), sg, sb, 0, st, out, in.z_b, in.z_b_lo, in.G, out, out, m0, dtp,
+                       ny, nx, dx, *sd, 1);
     if (int e = launch_status()) return e;
     return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
+}
+
+// One SSPRK2 step (integrators/SSPRK2.py:32-40) on three buffer sets:
+// dt from a; b = S(a); c = S(b) with the combine a = 0.5 a + 0.5 c fused into
+// the second bed sweep.  In MPM mode the stage-1 G is never read, so it is
+// not stored.
+template <typename T, bool FAST>
+static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields *b_, const pxh_fields *c_,
+                       const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl, void *dt, void *ws,
+                       hipStream_t st) {
+    F<T> a = cast_fields<T>(a_), b = cast_fields<T>(b_), c = cast_fields<T>(c_);
+    const int ny = g->ny, nx = g->nx;
+    const T dx = static_cast<T>(g->dx);
+    T *dtp = static_cast<T *>(dt);
+    const dim3 sg = strip_grid(ny, nx), sb(SW * SWPB);
+    const int nb = int(sg.x * sg.y);
+    T *part = static_cast<T *>(ws);
+    hipLaunchKernelGGL(k_rx_dt_strip<T>, sg, sb, 0, st, a, m0, ny, nx, dx, part);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
+    if (int e = launch_status()) return e;
+    const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
+    const uint32_t BED = (1u << PXH_G_ZB) | (sd->mpm ? 0u : (1u << PXH_G_G));
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, a, b, m0, dtp, ny, nx, dx, 1);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(g, b, bc, HY, st)) return e;
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0>), sg, sb, 0, st, b, a.z_b, a.z_b_lo, a.G, b, b, m0, dtp, ny, nx,
+                       dx, *sd, sd->mpm ? 0 : 1);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, b, c, m0, dtp, ny, nx, dx, 1);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1>), sg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny, nx,
+                       dx, *sd, 1);
+    if (int e = launch_status()) return e;
+    return bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
+}
+
+template <typename T>
+static int rx_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                   const pxh_sediments *sd, const pxh_bc *bc, const void *dt, hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (!(g->flags & PXH_EXACT_F32)) return rx_step_t<T, true>(g, in, out, m0, sd, bc, dt, st);
+    }
+    return rx_step_t<T, false>(g, in, out, m0, sd, bc, dt, st);
+}
+
+template <typename T>
+static int rx_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
+                     const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl, void *dt, void *ws,
+                     hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (!(g->flags & PXH_EXACT_F32)) return rx_ssprk2_t<T, true>(g, a, b, c, m0, sd, bc, cfl, dt, ws, st);
+    }
+    return rx_ssprk2_t<T, false>(g, a, b, c, m0, sd, bc, cfl, dt, ws, st);
 }
 
 template <typename T>
 static int rx_dt(const pxh_grid *g, const pxh_fields *s_, const uint8_t *m0, double cfl, void *dt_out, void *ws,
                  hipStream_t st) {
     F<T> s = cast_fields<T>(s_);
-    const int nb = dt_blocks(g);
+    const dim3 sg = strip_grid(g->ny, g->nx);
+    const int nb = int(sg.x * sg.y);
     T *part = static_cast<T *>(ws);
-    hipLaunchKernelGGL(k_rx_dt_partial<T>, dim3(nb), dim3(DT_THREADS), 0, st, s, m0, g->ny, g->nx,
-                       static_cast<T>(g->dx), part);
+    hipLaunchKernelGGL(k_rx_dt_strip<T>, sg, dim3(SW * SWPB), 0, st, s, m0, g->ny, g->nx, static_cast<T>(g->dx),
+                       part);
     if (int e = launch_status()) return e;
     hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1,
                        static_cast<T *>(dt_out));
@@ -472,6 +539,16 @@ static int roe_step(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *
 template <typename T>
 static int bc_apply_t(const pxh_grid *g, const pxh_fields *s_, const pxh_bc *bc, hipStream_t st) {
     return bc_launch<T>(g, cast_fields<T>(s_), bc, 0xffffffffu, st);
+}
+
+// A compensated bed is float32-only and all-or-nothing across the buffers of a call.
+static bool lo_ok(const pxh_grid *g, std::initializer_list<const pxh_fields *> fs) {
+    int n = 0, k = 0;
+    for (const pxh_fields *f : fs) {
+        ++k;
+        if (f->z_b_lo) ++n;
+    }
+    return n == 0 || (n == k && g->dtype == PXH_F32);
 }
 
 static bool fields_ok(const pxh_fields *f, bool exner) {
@@ -541,6 +618,7 @@ int pxh_roe_exner_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields
     if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
     if (in->h == out->h || in->hu == out->hu || in->hv == out->hv || in->z_b == out->z_b || in->G == out->G)
         return PXH_E_ARG;
+    if (!lo_ok(g, {in, out}) || (in->z_b_lo && in->z_b_lo == out->z_b_lo)) return PXH_E_ARG;
     if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
     return DISPATCH(g, rx_step, g, in, out, m0, sd, bc, dt_dev, static_cast<hipStream_t>(stream));
 }
@@ -573,7 +651,27 @@ int pxh_exner_solve_2D(const pxh_grid *g, const pxh_fields *in, const pxh_fields
 
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
-    return int64_t(DT_MAX_BLOCKS) * 8;
+    const dim3 sg = strip_grid(g->ny, g->nx);
+    const int64_t nb = int64_t(sg.x) * sg.y;
+    return (nb > DT_MAX_BLOCKS ? nb : DT_MAX_BLOCKS) * 8;
+}
+
+int pxh_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
+                         const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl, void *dt_dev,
+                         void *ws, void *stream) {
+    if (!grid_ok(g) || !fields_ok(a, true) || !fields_ok(b, true) || !fields_ok(c, true) || !m0 || !sd || !dt_dev ||
+        !ws)
+        return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    const void *pa[] = {a->h, a->hu, a->hv, a->z_b, a->G}, *pb[] = {b->h, b->hu, b->hv, b->z_b, b->G},
+               *pc[] = {c->h, c->hu, c->hv, c->z_b, c->G};
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j)
+            if (pa[i] == pb[j] || pa[i] == pc[j] || pb[i] == pc[j]) return PXH_E_ARG;
+    if (!lo_ok(g, {a, b, c}) || (a->z_b_lo && (a->z_b_lo == b->z_b_lo || a->z_b_lo == c->z_b_lo || b->z_b_lo == c->z_b_lo)))
+        return PXH_E_ARG;
+    if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
+    return DISPATCH(g, rx_ssprk2, g, a, b, c, m0, sd, bc, cfl, dt_dev, ws, static_cast<hipStream_t>(stream));
 }
 
 int pxh_roe_exner_dt(const pxh_grid *g, const pxh_fields *s, const uint8_t *m0, double cfl, void *dt_out,
@@ -585,6 +683,7 @@ int pxh_roe_exner_dt(const pxh_grid *g, const pxh_fields *s, const uint8_t *m0, 
 int pxh_ssprk2_combine(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *o,
                        void *stream) {
     if (!grid_ok(g) || !fields_ok(a, true) || !fields_ok(b, true) || !fields_ok(o, true)) return PXH_E_ARG;
+    if (!lo_ok(g, {a, b, o})) return PXH_E_ARG;
     const long n = long(g->ny) * g->nx;
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (g->dtype == PXH_F64)

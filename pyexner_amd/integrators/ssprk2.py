@@ -40,9 +40,8 @@ def ssprk2_step(sb, sc, state, time, dt, mask):
     be = getattr(sc, "backend", None)
     if be is not None and hasattr(be, "combine_into"):
         from ..state import fields_struct
-        out = state.replace(h=torch.empty_like(state.h), hu=torch.empty_like(state.hu),
-                            hv=torch.empty_like(state.hv), z_b=torch.empty_like(state.z_b),
-                            G=torch.empty_like(state.G))
+        from ..solvers.roe_exner_solver import _new_evolving
+        out = _new_evolving(state)
         be.combine_into(fields_struct(state), fields_struct(s2), fields_struct(out),
                         torch.cuda.current_stream().cuda_stream)
         return out

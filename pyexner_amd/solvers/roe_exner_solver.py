@@ -45,6 +45,7 @@ def sediment_table(seds, constant, np_dtype):
     for k in range(nsed):
         sd.frac[k], sd.A[k], sd.B[k] = float(frac[k]), float(A[k]), float(B[k])
         sd.C[k], sd.D[k], sd.theta_c[k] = float(Cc[k]), float(D[k]), float(theta_c[k])
+        sd.AB[k], sd.CD[k] = float(A[k]) / float(B[k]), float(Cc[k]) / float(D[k])
     return sd
 
 
@@ -54,12 +55,18 @@ class ExnerBackend:
 
     STEP_GROUPS = (_hip.G_H, _hip.G_HUV, _hip.G_ZB, _hip.G_G)
 
-    def __init__(self, shape, dtype, dx, seds, constant, boundaries, device="cuda"):
+    def __init__(self, shape, dtype, dx, seds, constant, boundaries, device="cuda", compensated_bed=None,
+                 exact_f32=False):
         self.shape = tuple(shape)
         self.dtype = dtype
         self.np_dtype = np.float32 if dtype == torch.float32 else np.float64
         self.dx = float(dx)
-        self.grid = _hip.Grid(self.shape[0], self.shape[1], dtype_code(dtype), 0, self.dx)
+        f32 = dtype == torch.float32
+        # float32 keeps the bed as z_b + z_b_lo unless told otherwise
+        self.compensated_bed = f32 if compensated_bed is None else (bool(compensated_bed) and f32)
+        self.exact_f32 = bool(exact_f32) and f32
+        flags = _hip.EXACT_F32 if self.exact_f32 else 0
+        self.grid = _hip.Grid(self.shape[0], self.shape[1], dtype_code(dtype), flags, self.dx)
         self.constant = constant
         self.sed = sediment_table(seds, constant, self.np_dtype)
         self.boundaries = boundaries
@@ -89,6 +96,11 @@ class ExnerBackend:
 
     def combine_into(self, fa, fb, fo, stream):
         _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
+
+    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream):
+        """dt, two stages and the combine; a is advanced in place."""
+        _hip.call("pxh_roe_exner_ssprk2", self.grid, fa, fb, fc, mask0_ptr, self.sed, self.bc_step, float(cfl),
+                  dt_ptr, self.workspace.data_ptr(), stream)
 
 
 def _stream():
@@ -120,10 +132,14 @@ def get_mask(state, mpi_handler=None, b_mask=None):
 
 
 def config_fn_roeexner(state, mpi_handler, boundaries, dx, params):
+    """roe_exner_solver.py:41-56.  Backend options (not in the reference YAML):
+    ``backend: {compensated_bed: bool, exact_f32: bool}``."""
     grass = params["erosion"]["grass_factor"]
     constant = not isinstance(grass, str)
+    opts = params.get("backend", {}) or {}
     backend = ExnerBackend(state.h.shape, state.h.dtype, dx, state.seds, constant, boundaries,
-                           device=state.h.device)
+                           device=state.h.device, compensated_bed=opts.get("compensated_bed"),
+                           exact_f32=opts.get("exact_f32", False))
     return SolverConfig(mpi_handler=mpi_handler, boundaries=boundaries, dx=dx, halo_exchange=identity_halo,
                         compute_G="constant" if constant else "MPM", compute_n="constant" if constant else "MPM",
                         backend=backend)
@@ -139,16 +155,24 @@ def _mask0_ptr(mask):
 def init_fn_roeexner(state, mask, config):
     """roe_exner_solver.py:58-78: n_b, G, then the ring fill at t = 0."""
     s = state.clone()
+    if config.backend.compensated_bed and s.z_b_lo is None:
+        s.z_b_lo = torch.zeros_like(s.z_b)
     config.backend.init_into(fields_struct(s), _mask0_ptr(mask), _stream())
     return s
+
+
+def _new_evolving(state):
+    kw = dict(h=torch.empty_like(state.h), hu=torch.empty_like(state.hu), hv=torch.empty_like(state.hv),
+              z_b=torch.empty_like(state.z_b), G=torch.empty_like(state.G))
+    if state.z_b_lo is not None:
+        kw["z_b_lo"] = torch.empty_like(state.z_b_lo)
+    return state.replace(**kw)
 
 
 def step_fn_roeexner(state, time, dt, mask, config):
     """roe_exner_solver.py:80-116: one Forward-Euler step, out of place."""
     be = config.backend
-    out = state.replace(h=torch.empty_like(state.h), hu=torch.empty_like(state.hu),
-                        hv=torch.empty_like(state.hv), z_b=torch.empty_like(state.z_b),
-                        G=torch.empty_like(state.G))
+    out = _new_evolving(state)
     dtt = _dt_tensor(dt, state.h.dtype, state.h.device)
     be.step_into(fields_struct(state), fields_struct(out), _mask0_ptr(mask), dtt.data_ptr(), _stream())
     return out

@@ -8,7 +8,7 @@ solver is configured).
 """
 import traceback
 from dataclasses import dataclass, fields, replace as dc_replace
-from typing import Any
+from typing import Any, Optional
 
 import numpy as np
 import torch
@@ -57,7 +57,7 @@ class BaseState:
         return type(self)(**kw)
 
     def field_names(self):
-        return [f.name for f in fields(self) if f.name != "seds"]
+        return [f.name for f in fields(self) if f.name != "seds" and getattr(self, f.name) is not None]
 
     @property
     def dtype(self):
@@ -89,6 +89,8 @@ class RoeExnerState(BaseState):
     n_b: torch.Tensor
     G: torch.Tensor
     seds: Any
+    # low part of a compensated float32 bed (z_b + z_b_lo); None when not used
+    z_b_lo: Optional[torch.Tensor] = None
 
     @classmethod
     def empty(cls, mesh, dtype=torch.float32, device="cuda"):
@@ -133,7 +135,7 @@ class RoeState(BaseState):
         return cls(**{k: _dev_field(arrays[k], dtype, device) for k in ("h", "hu", "hv", "z", "n")})
 
 
-_FIELD_NAMES = ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")
+_FIELD_NAMES = _hip.FIELD_NAMES
 
 
 def check_field(t, dtype, shape):

@@ -29,8 +29,9 @@ class ExnerStepper:
         self.mask = mask
         self.m0 = mask[0].data_ptr()
         self.a = state.clone()
-        self.b = _like(self.a, EVOLVING)
-        self.c = _like(self.a, EVOLVING)
+        names = EVOLVING + (("z_b_lo",) if state.z_b_lo is not None else ())
+        self.b = _like(self.a, names)
+        self.c = _like(self.a, names)
         self.fa, self.fb, self.fc = (fields_struct(s) for s in (self.a, self.b, self.c))
         self.dt = torch.zeros((), dtype=state.h.dtype, device=state.h.device)
         self.graph = None
@@ -46,12 +47,7 @@ class ExnerStepper:
         self.be.dt_into(self.fa, self.m0, self.cfl, self.dt.data_ptr(), self._stream())
 
     def ssprk2_step(self):
-        st = self._stream()
-        dtp = self.dt.data_ptr()
-        self.be.dt_into(self.fa, self.m0, self.cfl, dtp, st)
-        self.be.step_into(self.fa, self.fb, self.m0, dtp, st)
-        self.be.step_into(self.fb, self.fc, self.m0, dtp, st)
-        self.be.combine_into(self.fa, self.fc, self.fa, st)
+        self.be.ssprk2_into(self.fa, self.fb, self.fc, self.m0, self.cfl, self.dt.data_ptr(), self._stream())
 
     def fe_step(self):
         st = self._stream()
