@@ -9,6 +9,9 @@ domain (dt reduction, two Forward-Euler stages, combine) with the state
 resident in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu]
+
+The timed region is K SSPRK2 steps from the initial state (K = 500 by default,
+the whole workload), after W warm-up steps on the same buffers.
 """
 import argparse
 import json
@@ -22,29 +25,34 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic HBM bytes per cell for each float32 kernel of the step
-# (state fields 4 B each, nodata mask 1 B; neighbour re-reads not counted).
+# Algorithmic HBM bytes per cell of each float32 kernel of the step (fields
+# 4 B, nodata mask 1 B, compensated bed z_b + z_b_lo; neighbour re-reads and
+# the ring fills not counted).
 ALG_BYTES = {
-    "roe_solve_2D": 6 * 4 + 1 + 3 * 4,      # read h hu hv z_b z n_b + mask, write h hu hv
-    "momentum_corrections": 3 * 4 + 1,      # read h, z, z_b + mask (writes only where zeroed)
-    "compute_G": 3 * 4 + 1 + 4,             # read h hu hv + mask, write G
-    "exner_solve_2D": 5 * 4 + 1 + 4,        # read h hu hv z_b G + mask, write z_b
-    "compute_dt": 4 * 4 + 1,                # read h hu hv G + mask
-    "ssprk2_combine": 10 * 4 + 5 * 4,       # read 2 x (h hu hv z_b G), write 5
+    # roe_solve_2D + momentum_corrections: read h hu hv z_b z n_b + mask, write h hu hv
+    "hydro_sweep": 6 * 4 + 1 + 3 * 4,
+    # stage-1 compute_G + exner_solve_2D: read h hu hv z_b z_b_lo + mask, write z_b z_b_lo
+    "bed_sweep": 5 * 4 + 1 + 2 * 4,
+    # stage-2 bed sweep + SSPRK2 combine + next dt: read h hu hv z_b z_b_lo + mask and
+    # h hu hv z_b z_b_lo G of u^n, write h hu hv z_b z_b_lo G
+    "bed_sweep_combine_dt": 5 * 4 + 1 + 6 * 4 + 6 * 4,
+    # standalone compute_dt (first step only): read h hu hv G + mask
+    "compute_dt": 4 * 4 + 1,
 }
+PER_STEP = {"hydro_sweep": 2, "bed_sweep": 1, "bed_sweep_combine_dt": 1, "compute_dt": 0}
 HBM_PEAK_GBS = 8000.0
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=500)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--size", type=int, default=8192)
     p.add_argument("--kernel-reps", type=int, default=10)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-size", type=int, default=512)
-    p.add_argument("--cpu-steps", type=int, default=8)
+    p.add_argument("--cpu-size", type=int, default=1024)
+    p.add_argument("--cpu-steps", type=int, default=30)
     p.add_argument("--graph", type=int, default=1, help="capture each step into a hipGraph (1) or not (0)")
     return p.parse_args()
 
@@ -99,21 +107,25 @@ def main():
     mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
     state = solver.init_fn(state, mask, sc)
     st = ExnerStepper(sc.backend, state, mask, params["cfl"])
+    init = {f: getattr(state, f).clone() for f in ("h", "hu", "hv", "z_b", "z_b_lo", "G")}
     del state
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t0:.1f} s")
 
+    # warm up (and capture one step into a hipGraph), then restart from the initial state
     for _ in range(a.warmup):
         st.ssprk2_step()
-    torch.cuda.synchronize()
     use_graph = bool(a.graph)
     if use_graph:
         st.capture(1, "SSPRK2")
-        st.replay()
-        torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    for f, t in init.items():
+        getattr(st.a, f).copy_(t)
+    del init
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
+    st.compute_dt()  # the first step's dt; every step then leaves the next one
     for _ in range(a.steps):
         if use_graph:
             st.replay()
@@ -128,20 +140,23 @@ def main():
     h = st.state.h
     if not torch.isfinite(h).all():
         raise RuntimeError("non-finite depth after the timed steps")
+    t_end = float(st.dt)  # next dt (sanity)
+    log(f"next dt {t_end:.4e} s")
 
-    # per-kernel durations (HIP events on the stream the kernels run on)
+    # per-kernel durations: each kernel alone, HIP events on the stream it runs on
     be, m0 = sc.backend, st.m0
     stream = torch.cuda.current_stream().cuda_stream
     g = be.grid
     fa, fb, fc = st.fa, st.fb, st.fc
     dtp = st.dt.data_ptr()
+    part = be.workspace.data_ptr()
     calls = {
-        "roe_solve_2D": lambda: _hip.call("pxh_roe_solve_2D", g, fa, fb, m0, dtp, stream),
-        "momentum_corrections": lambda: _hip.call("pxh_momentum_corrections", g, fa, fb, m0, stream),
-        "compute_G": lambda: _hip.call("pxh_compute_G", g, fa, fb, m0, be.sed, stream),
-        "exner_solve_2D": lambda: _hip.call("pxh_exner_solve_2D", g, fa, fb, m0, dtp, stream),
+        "hydro_sweep": lambda: _hip.call("pxh_roe_exner_hydro_sweep", g, fa, fb, m0, dtp, stream),
+        "bed_sweep": lambda: _hip.call("pxh_roe_exner_bed_sweep", g, fb, fa, fb, None, m0, be.sed, dtp, None,
+                                       stream),
+        "bed_sweep_combine_dt": lambda: _hip.call("pxh_roe_exner_bed_sweep", g, fc, fb, fa, fa, m0, be.sed, dtp,
+                                                  part, stream),
         "compute_dt": lambda: be.dt_into(fa, m0, st.cfl, dtp, stream),
-        "ssprk2_combine": lambda: be.combine_into(fa, fc, fb, stream),
     }
     kern = {}
     for name, fn in calls.items():
@@ -155,16 +170,15 @@ def main():
         torch.cuda.synchronize()
         us = k0.elapsed_time(k1) * 1e3 / a.kernel_reps
         gbs = ALG_BYTES[name] * n * n / (us * 1e-6) / 1e9
-        kern[name] = {"us": round(us, 2), "alg_GBs": round(gbs, 1)}
+        kern[name] = {"us": round(us, 2), "alg_bytes_per_cell": ALG_BYTES[name], "alg_GBs": round(gbs, 1),
+                      "per_step": PER_STEP[name]}
         log(f"  {name:22s} {us:9.1f} us  {gbs:8.1f} GB/s algorithmic")
-    per_step_counts = {"roe_solve_2D": 2, "momentum_corrections": 2, "compute_G": 2, "exner_solve_2D": 2,
-                       "compute_dt": 1, "ssprk2_combine": 1}
-    dom = max(kern, key=lambda k: kern[k]["us"] * per_step_counts[k])
+    dom = max(kern, key=lambda k: kern[k]["us"] * PER_STEP[k])
     achieved = kern[dom]["alg_GBs"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "alg_bytes_per_cell": ALG_BYTES[dom], "cells_per_launch": n * n}
-    step_bytes = sum(ALG_BYTES[k] * c for k, c in per_step_counts.items())
+    step_bytes = sum(ALG_BYTES[k] * c for k, c in PER_STEP.items())
 
     cpu = None
     if not a.no_cpu:
@@ -186,8 +200,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "roe_exner_ssprk2_radial_dambreak", "nx": n, "ny": n, "dx": 0.05,
                    "integrator": "SSPRK2", "cfl": 0.5, "bumps": 256 if n == 8192 else None,
-                   "hipgraph": use_graph, "step_alg_bytes_per_cell": step_bytes,
-                   "step_alg_GBs": round(step_bytes * value / 1e9, 1)},
+                   "steps_from_initial_state": a.steps, "hipgraph": use_graph,
+                   "bed": "float32 z_b + z_b_lo (compensated)", "math": "float32 hardware rcp/rsq/log2/exp2",
+                   "step_alg_bytes_per_cell": step_bytes, "step_alg_GBs": round(step_bytes * value / 1e9, 1)},
         "roofline": roofline,
         "kernels": kern,
         "cpu_baseline": cpu,

@@ -147,13 +147,33 @@ int pxh_exner_solve_2D(const pxh_grid *grid, const pxh_fields *in, const pxh_fie
                        const void *dt_dev, void *stream);
 
 /* One whole SSPRK2 step on three caller-owned buffer sets (integrators/SSPRK2.py:32-40):
- * dt = compute_dt_fn(a) into dt_dev; b = step_fn(a, dt); c = step_fn(b, dt);
- * a = 0.5 a + 0.5 c.  a is updated in place; b and c are scratch whose z, n,
- * n_b may share a's (static fields); the evolving fields of a, b, c must not
- * alias.  workspace as for pxh_roe_exner_dt. */
+ * dt = compute_dt_fn(a) into dt_dev (skipped when dt_ready != 0: dt_dev already
+ * holds it); b = step_fn(a, dt); c = step_fn(b, dt); a = 0.5 a + 0.5 c.  On
+ * return dt_dev holds compute_dt_fn(a) of the new a (reduced inside the last
+ * sweep), so consecutive calls pass dt_ready = 1.  a is updated in place; b
+ * and c are scratch whose z, n, n_b may share a's (static fields); the evolving
+ * fields of a, b, c must not alias.  workspace as for pxh_roe_exner_dt. */
 int pxh_roe_exner_ssprk2(const pxh_grid *grid, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
                          const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc, double cfl,
-                         void *dt_dev, void *workspace, void *stream);
+                         void *dt_dev, void *workspace, int dt_ready, void *stream);
+
+/* The two fused sweeps a step is built from, one launch each, for callers that
+ * sequence the step themselves (ring fills or halo exchanges in between):
+ *   hydro sweep: roe_solve_2D + momentum_corrections (roe_exner.py:535-612, 14-62):
+ *                out.{h,hu,hv} from in.
+ *   bed sweep:   compute_G + exner_solve_2D (roe_exner.py:85-144, 665-717) on the
+ *                filled hyd.{h,hu,hv}, bed from bed.{z_b,z_b_lo,G}:
+ *                old == NULL: out.{z_b,z_b_lo,G};
+ *                old != NULL: out = 0.5 old + 0.5 result over h,hu,hv,z_b,G
+ *                (integrators/SSPRK2.py:40), and when dt_partial != NULL the
+ *                per-block CFL minima of the combined state, which
+ *                pxh_dt_finalize turns into compute_dt_fn's value. */
+int pxh_roe_exner_hydro_sweep(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
+                              const uint8_t *mask0, const void *dt_dev, void *stream);
+int pxh_roe_exner_bed_sweep(const pxh_grid *grid, const pxh_fields *hyd, const pxh_fields *bed,
+                            const pxh_fields *out, const pxh_fields *old, const uint8_t *mask0,
+                            const pxh_sediments *sed, const void *dt_dev, void *dt_partial, void *stream);
+int pxh_dt_finalize(const pxh_grid *grid, const void *dt_partial, double cfl, void *dt_out, void *stream);
 
 /* compute_dt_fn: *dt_out = max(cfl * min over active edges, 1e-6) written to
  * device memory.  workspace must hold pxh_dt_workspace_size(grid) bytes. */

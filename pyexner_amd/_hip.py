@@ -77,7 +77,14 @@ _SIGS = {
     "pxh_bc_apply": (ctypes.c_int, [_P(Grid), _P(Fields), _P(BC), ctypes.c_void_p]),
     "pxh_roe_exner_ssprk2": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
                                             _P(Sediments), _P(BC), ctypes.c_double, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_void_p]),
+                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "pxh_roe_exner_hydro_sweep": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_roe_exner_bed_sweep": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), _P(Fields),
+                                               ctypes.c_void_p, _P(Sediments), ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
+    "pxh_dt_finalize": (ctypes.c_int, [_P(Grid), ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
     "pxh_roe_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
     "pxh_momentum_corrections": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),

@@ -57,6 +57,39 @@ struct F {
     T *z_b_lo;  // compensated-bed low part (float32 only) or null
 };
 
+// Global-memory access with a 32-bit element offset.  Field pointers reach the
+// kernels inside structs, where the compiler cannot see they are global, and
+// would otherwise emit flat loads (which share the LDS wait counter with the
+// cross-lane shuffles) and 64-bit address arithmetic.
+#define PXH_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T ld(const T *p, unsigned i) {
+    return ((const PXH_GLOBAL T *)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void st(T *p, unsigned i, T v) {
+    ((PXH_GLOBAL T *)p)[i] = v;
+}
+// Row-based access for sweeps where every lane of a wavefront is on the same
+// row: the row offset is wavefront-uniform (scalar registers, any size), the
+// column a 32-bit per-lane offset.
+__device__ __forceinline__ unsigned long uni64(unsigned long a) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(a));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(a >> 32));
+    return (static_cast<unsigned long>(hi) << 32) | lo;
+}
+// row_off must be the same in every active lane of the wavefront.
+template <typename T>
+__device__ __forceinline__ T ldr(const T *p, long row_off, unsigned col) {
+    const unsigned long base = uni64(reinterpret_cast<unsigned long>(p + row_off));
+    return *(const PXH_GLOBAL T *)((const PXH_GLOBAL char *)base + col * unsigned(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void str(T *p, long row_off, unsigned col, T v) {
+    const unsigned long base = uni64(reinterpret_cast<unsigned long>(p + row_off));
+    *(PXH_GLOBAL T *)((PXH_GLOBAL char *)base + col * unsigned(sizeof(T))) = v;
+}
+
 inline int launch_status() {
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PXH_OK : static_cast<int>(e);

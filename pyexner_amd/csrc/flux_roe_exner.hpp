@@ -191,7 +191,32 @@ __device__ __forceinline__ T const_G(T h, T Gprev, bool nodata) {
     return (h > C<T>(DRY_TOL)) ? G : T(0);
 }
 
-// _get_lambda (roe_exner.py:163-209), evaluated in double.
+// Largest and smallest of the three values cos((acos(x) + 2 pi k) / 3), with
+// acos's argument clipped to [-1, 1] as _get_lambda does (roe_exner.py:185).
+// They are the extreme roots of 4 t^3 - 3 t - x = 0.  For x >= 0 the largest
+// root lies in [sqrt(3)/2, 1], where the derivative 12 t^2 - 3 >= 6, so three
+// Newton steps from a quadratic fit (error < 2e-3) reach double precision; the
+// smallest root follows from dividing out that factor:
+// t' = (-t - sqrt(3 (1 - t)(1 + t))) / 2, accurate also next to the double
+// root at x -> 1.  Negative x uses the mirror t(-x) = -t(x).
+__host__ __device__ __forceinline__ void trisect(double x, double &tmin, double &tmax) {
+    if (x >= 1.0) { tmax = 1.0; tmin = -0.5; return; }
+    if (x <= -1.0) { tmax = 0.5; tmin = -1.0; return; }
+    const bool neg = x < 0.0;
+    const double xa = neg ? -x : x;
+    double t = 0.8660254037844386 + xa * (0.16666666666666666 - 0.03269207045111198 * xa);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double t2 = t * t;
+        t = t - ((4.0 * t2 - 3.0) * t - xa) / (12.0 * t2 - 3.0);
+    }
+    const double other = 0.5 * (-t - __builtin_sqrt(3.0 * (1.0 - t) * (1.0 + t)));
+    if (neg) { tmin = -t; tmax = -other; }
+    else { tmax = t; tmin = other; }
+}
+
+// _get_lambda (roe_exner.py:163-209), evaluated in double: the coded cubic's
+// depressed form, lambda = 2 sqrt(-p/3) t + shift with t from trisect.
 __device__ __forceinline__ void get_lambda(double ut, double at, double ct, double &lmin, double &lmax) {
     if (at < 1e-5) {
         lmin = ut - ct;
@@ -205,12 +230,12 @@ __device__ __forceinline__ void get_lambda(double ut, double at, double ct, doub
     double p = ((3.0 * a) * c - b * b) / 3.0;
     double q = ((2.0 * cube(b) - ((9.0 * a) * b) * c) + 27.0 * d) / -27.0;
     double x = (3.0 * q) / (2.0 * p) * __builtin_sqrt(-3.0 / p);
-    x = pmin(pmax(x, -1.0), 1.0);
-    double th = acos(x);
+    double tmin, tmax;
+    trisect(x, tmin, tmax);
     double st = 2.0 * __builtin_sqrt(-p / 3.0);
     double sh = -b / (3.0 * a);
-    lmax = st * cos(th / 3.0) + sh;
-    lmin = st * cos((th + 6.283185307179586) / 3.0) + sh;
+    lmax = st * tmax + sh;
+    lmin = st * tmin + sh;
 }
 
 // compute_dt per edge (roe_exner.py:211-255).

@@ -436,20 +436,25 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, in, out, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
-    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0>), sg, sb, 0, st, out, in.z_b, in.z_b_lo, in.G, out, out, m0, dtp,
-                       ny, nx, dx, *sd, 1);
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), sg, sb, 0, st, out, in.z_b, in.z_b_lo, in.G, out, out,
+                       m0, dtp, ny, nx, dx, *sd, 1, nullptr);
     if (int e = launch_status()) return e;
     return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
 }
 
 // One SSPRK2 step (integrators/SSPRK2.py:32-40) on three buffer sets:
-// dt from a; b = S(a); c = S(b) with the combine a = 0.5 a + 0.5 c fused into
-// the second bed sweep.  In MPM mode the stage-1 G is never read, so it is
-// not stored.
+// dt from a (unless dt_ready); b = S(a); c = S(b) with the combine
+// a = 0.5 a + 0.5 c fused into the second bed sweep, which also reduces the
+// CFL dt of the new a, so dt holds compute_dt_fn(a) on return.  In MPM mode
+// the stage-1 G is never read, so it is not stored.  The fused dt reads G of
+// ring cells before the final (z_b, G) fill; every handler kind here either
+// copies G together with h, hu, hv from one source (Reflective, Periodic) or
+// leaves it to compute_G of the filled h, hu, hv (Transmissive), and compute_G
+// depends on the velocity only through |u|^2, so the fill cannot change it.
 template <typename T, bool FAST>
 static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields *b_, const pxh_fields *c_,
                        const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl, void *dt, void *ws,
-                       hipStream_t st) {
+                       int dt_ready, hipStream_t st) {
     F<T> a = cast_fields<T>(a_), b = cast_fields<T>(b_), c = cast_fields<T>(c_);
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
@@ -457,26 +462,82 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     const dim3 sg = strip_grid(ny, nx), sb(SW * SWPB);
     const int nb = int(sg.x * sg.y);
     T *part = static_cast<T *>(ws);
-    hipLaunchKernelGGL(k_rx_dt_strip<T>, sg, sb, 0, st, a, m0, ny, nx, dx, part);
-    if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
-    if (int e = launch_status()) return e;
+    if (!dt_ready) {
+        hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), sg, sb, 0, st, a, m0, ny, nx, dx, part);
+        if (int e = launch_status()) return e;
+        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
+        if (int e = launch_status()) return e;
+    }
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (sd->mpm ? 0u : (1u << PXH_G_G));
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, a, b, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, HY, st)) return e;
-    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0>), sg, sb, 0, st, b, a.z_b, a.z_b_lo, a.G, b, b, m0, dtp, ny, nx,
-                       dx, *sd, sd->mpm ? 0 : 1);
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), sg, sb, 0, st, b, a.z_b, a.z_b_lo, a.G, b, b, m0, dtp,
+                       ny, nx, dx, *sd, sd->mpm ? 0 : 1, nullptr);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, b, c, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
-    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1>), sg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny, nx,
-                       dx, *sd, 1);
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), sg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny,
+                       nx, dx, *sd, 1, part);
     if (int e = launch_status()) return e;
-    return bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
+    if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
+    return launch_status();
+}
+
+// The two sweeps of a stage, one launch each.
+template <typename T, bool FAST>
+static int rx_hydro_sweep_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
+                            const void *dt, hipStream_t st) {
+    F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), strip_grid(g->ny, g->nx), dim3(SW * SWPB), 0, st, in, out, m0,
+                       static_cast<const T *>(dt), g->ny, g->nx, static_cast<T>(g->dx), 1);
+    return launch_status();
+}
+
+template <typename T, bool FAST>
+static int rx_bed_sweep_t(const pxh_grid *g, const pxh_fields *hyd_, const pxh_fields *bed_, const pxh_fields *out_,
+                          const pxh_fields *old_, const uint8_t *m0, const pxh_sediments *sd, const void *dt,
+                          void *partial, hipStream_t st) {
+    F<T> hyd = cast_fields<T>(hyd_), bed = cast_fields<T>(bed_), out = cast_fields<T>(out_);
+    const dim3 sg = strip_grid(g->ny, g->nx), sb(SW * SWPB);
+    const T *dtp = static_cast<const T *>(dt);
+    const T dx = static_cast<T>(g->dx);
+    if (!old_) {
+        hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G, out,
+                           out, m0, dtp, g->ny, g->nx, dx, *sd, 1, nullptr);
+    } else {
+        F<T> old = cast_fields<T>(old_);
+        if (partial)
+            hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
+                               out, old, m0, dtp, g->ny, g->nx, dx, *sd, 1, static_cast<T *>(partial));
+        else
+            hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, false>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
+                               out, old, m0, dtp, g->ny, g->nx, dx, *sd, 1, nullptr);
+    }
+    return launch_status();
+}
+
+template <typename T>
+static int rx_hydro_sweep(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                          const void *dt, hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (!(g->flags & PXH_EXACT_F32)) return rx_hydro_sweep_t<T, true>(g, in, out, m0, dt, st);
+    }
+    return rx_hydro_sweep_t<T, false>(g, in, out, m0, dt, st);
+}
+
+template <typename T>
+static int rx_bed_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_fields *bed, const pxh_fields *out,
+                        const pxh_fields *old, const uint8_t *m0, const pxh_sediments *sd, const void *dt,
+                        void *partial, hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (!(g->flags & PXH_EXACT_F32)) return rx_bed_sweep_t<T, true>(g, hyd, bed, out, old, m0, sd, dt, partial, st);
+    }
+    return rx_bed_sweep_t<T, false>(g, hyd, bed, out, old, m0, sd, dt, partial, st);
 }
 
 template <typename T>
@@ -491,26 +552,36 @@ static int rx_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields *ou
 template <typename T>
 static int rx_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
                      const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl, void *dt, void *ws,
-                     hipStream_t st) {
+                     int dt_ready, hipStream_t st) {
     if constexpr (sizeof(T) == 4) {
-        if (!(g->flags & PXH_EXACT_F32)) return rx_ssprk2_t<T, true>(g, a, b, c, m0, sd, bc, cfl, dt, ws, st);
+        if (!(g->flags & PXH_EXACT_F32))
+            return rx_ssprk2_t<T, true>(g, a, b, c, m0, sd, bc, cfl, dt, ws, dt_ready, st);
     }
-    return rx_ssprk2_t<T, false>(g, a, b, c, m0, sd, bc, cfl, dt, ws, st);
+    return rx_ssprk2_t<T, false>(g, a, b, c, m0, sd, bc, cfl, dt, ws, dt_ready, st);
 }
 
-template <typename T>
-static int rx_dt(const pxh_grid *g, const pxh_fields *s_, const uint8_t *m0, double cfl, void *dt_out, void *ws,
-                 hipStream_t st) {
+template <typename T, bool FAST>
+static int rx_dt_t(const pxh_grid *g, const pxh_fields *s_, const uint8_t *m0, double cfl, void *dt_out, void *ws,
+                   hipStream_t st) {
     F<T> s = cast_fields<T>(s_);
     const dim3 sg = strip_grid(g->ny, g->nx);
     const int nb = int(sg.x * sg.y);
     T *part = static_cast<T *>(ws);
-    hipLaunchKernelGGL(k_rx_dt_strip<T>, sg, dim3(SW * SWPB), 0, st, s, m0, g->ny, g->nx, static_cast<T>(g->dx),
-                       part);
+    hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), sg, dim3(SW * SWPB), 0, st, s, m0, g->ny, g->nx,
+                       static_cast<T>(g->dx), part);
     if (int e = launch_status()) return e;
     hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1,
                        static_cast<T *>(dt_out));
     return launch_status();
+}
+
+template <typename T>
+static int rx_dt(const pxh_grid *g, const pxh_fields *s, const uint8_t *m0, double cfl, void *dt_out, void *ws,
+                 hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (!(g->flags & PXH_EXACT_F32)) return rx_dt_t<T, true>(g, s, m0, cfl, dt_out, ws, st);
+    }
+    return rx_dt_t<T, false>(g, s, m0, cfl, dt_out, ws, st);
 }
 
 template <typename T>
@@ -649,6 +720,42 @@ int pxh_exner_solve_2D(const pxh_grid *g, const pxh_fields *in, const pxh_fields
     return DISPATCH(g, rx_part, 3, g, in, out, m0, nullptr, dt_dev, static_cast<hipStream_t>(stream));
 }
 
+int pxh_roe_exner_hydro_sweep(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const uint8_t *m0,
+                              const void *dt_dev, void *stream) {
+    if (!part_args_ok(g, in, out, m0) || !dt_dev || in->h == out->h || in->hu == out->hu || in->hv == out->hv)
+        return PXH_E_ARG;
+    return DISPATCH(g, rx_hydro_sweep, g, in, out, m0, dt_dev, static_cast<hipStream_t>(stream));
+}
+
+int pxh_roe_exner_bed_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_fields *bed, const pxh_fields *out,
+                            const pxh_fields *old, const uint8_t *m0, const pxh_sediments *sd, const void *dt_dev,
+                            void *dt_partial, void *stream) {
+    if (!grid_ok(g) || !fields_ok(hyd, true) || !fields_ok(bed, true) || !fields_ok(out, true) || !m0 || !sd ||
+        !dt_dev)
+        return PXH_E_ARG;
+    if (old && !fields_ok(old, true)) return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    if (bed->z_b == out->z_b || (bed->z_b_lo && bed->z_b_lo == out->z_b_lo)) return PXH_E_ARG;
+    if (!lo_ok(g, {bed, out}) || (old && !lo_ok(g, {bed, out, old}))) return PXH_E_ARG;
+    if (old && (hyd->h == out->h || hyd->hu == out->hu || hyd->hv == out->hv)) return PXH_E_ARG;
+    return DISPATCH(g, rx_bed_sweep, g, hyd, bed, out, old, m0, sd, dt_dev, dt_partial,
+                    static_cast<hipStream_t>(stream));
+}
+
+int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void *dt_out, void *stream) {
+    if (!grid_ok(g) || !dt_partial || !dt_out) return PXH_E_ARG;
+    const dim3 sg = strip_grid(g->ny, g->nx);
+    const int nb = int(sg.x * sg.y);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (g->dtype == PXH_F64)
+        hipLaunchKernelGGL(k_dt_final<double>, dim3(1), dim3(DT_THREADS), 0, st, static_cast<const double *>(dt_partial),
+                           nb, cfl, 1, static_cast<double *>(dt_out));
+    else
+        hipLaunchKernelGGL(k_dt_final<float>, dim3(1), dim3(DT_THREADS), 0, st, static_cast<const float *>(dt_partial),
+                           nb, static_cast<float>(cfl), 1, static_cast<float *>(dt_out));
+    return launch_status();
+}
+
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
     const dim3 sg = strip_grid(g->ny, g->nx);
@@ -658,7 +765,7 @@ int64_t pxh_dt_workspace_size(const pxh_grid *g) {
 
 int pxh_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
                          const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl, void *dt_dev,
-                         void *ws, void *stream) {
+                         void *ws, int dt_ready, void *stream) {
     if (!grid_ok(g) || !fields_ok(a, true) || !fields_ok(b, true) || !fields_ok(c, true) || !m0 || !sd || !dt_dev ||
         !ws)
         return PXH_E_ARG;
@@ -671,7 +778,8 @@ int pxh_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_field
     if (!lo_ok(g, {a, b, c}) || (a->z_b_lo && (a->z_b_lo == b->z_b_lo || a->z_b_lo == c->z_b_lo || b->z_b_lo == c->z_b_lo)))
         return PXH_E_ARG;
     if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
-    return DISPATCH(g, rx_ssprk2, g, a, b, c, m0, sd, bc, cfl, dt_dev, ws, static_cast<hipStream_t>(stream));
+    return DISPATCH(g, rx_ssprk2, g, a, b, c, m0, sd, bc, cfl, dt_dev, ws, dt_ready,
+                    static_cast<hipStream_t>(stream));
 }
 
 int pxh_roe_exner_dt(const pxh_grid *g, const pxh_fields *s, const uint8_t *m0, double cfl, void *dt_out,

@@ -32,7 +32,7 @@ struct StripPos {
 __device__ __forceinline__ StripPos strip_pos(int ny, int nx) {
     StripPos p;
     p.lane = threadIdx.x & (SW - 1);
-    const int w = threadIdx.x / SW;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SW);  // uniform: rows are scalar
     p.c = blockIdx.x * SOUT - 1 + p.lane;
     p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
     p.out = p.lane >= 1 && p.lane <= SOUT && p.c < nx;
@@ -85,13 +85,27 @@ struct HD {          // derived per-cell values for the Roe-Exner edge
     int flags;       // bit0 wet (h > DRY_TOL), bit1 nodata
 };
 
+template <typename T>
+struct HR {          // raw per-cell inputs of the hydro sweep
+    T h, hu, hv, zb, z, n;
+    int m;
+};
+
+template <typename T>
+__device__ __forceinline__ HR<T> hr_load(const F<T> &s, const uint8_t *m0, long ro, unsigned c) {
+    HR<T> q;
+    q.h = ldr(s.h, ro, c); q.hu = ldr(s.hu, ro, c); q.hv = ldr(s.hv, ro, c);
+    q.zb = ldr(s.z_b, ro, c); q.z = ldr(s.z, ro, c); q.n = ldr(s.n_b, ro, c);
+    q.m = ldr(m0, ro, c);
+    return q;
+}
+
 template <typename T, bool FAST>
-__device__ __forceinline__ HD<T> hd_load(const F<T> &s, const uint8_t *m0, long i) {
+__device__ __forceinline__ HD<T> hd_derive(const HR<T> &q) {
     const T T_ = C<T>(DRY_TOL), S_ = C<T>(SED_TOL), g = C<T>(GRAV), zero = T(0);
     HD<T> d;
-    d.h = s.h[i]; d.hu = s.hu[i]; d.hv = s.hv[i];
-    const T zb = s.z_b[i], z = s.z[i];
-    d.n = s.n_b[i];
+    d.h = q.h; d.hu = q.hu; d.hv = q.hv;
+    d.n = q.n;
     d.hf = pmax(d.h, T_);
     const bool wet = d.hf > T_;
     if constexpr (FAST) {
@@ -107,10 +121,15 @@ __device__ __forceinline__ HD<T> hd_load(const F<T> &s, const uint8_t *m0, long 
         d.s = d_sqrt(d.hf);
         d.e = d_sqrt(g * d.hf);
     }
-    d.zt = z + (zb > S_ ? zb : C<T>(SED_FLOOR));
-    d.zr = z + zb;
-    d.flags = (d.h > T_ ? 1 : 0) | (m0[i] ? 2 : 0);
+    d.zt = q.z + (q.zb > S_ ? q.zb : C<T>(SED_FLOOR));
+    d.zr = q.z + q.zb;
+    d.flags = (d.h > T_ ? 1 : 0) | (q.m ? 2 : 0);
     return d;
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ HD<T> hd_load(const F<T> &s, const uint8_t *m0, long ro, unsigned c) {
+    return hd_derive<T, FAST>(hr_load(s, m0, ro, c));
 }
 
 template <typename T>
@@ -261,25 +280,28 @@ __device__ __forceinline__ void rx_edge_d(const HD<T> &I, const HD<T> &J, T dx, 
 
 // The updated depth of one cell, from scratch (roe_solve_2D for that cell);
 // used only when momentum corrections need a neighbour outside the window.
+// Inlined once, in a loop over the pending neighbours, rather than called: a
+// call would make the whole sweep keep its live registers across the ABI.
 template <typename T, bool FAST>
-__device__ __noinline__ T h1_at(const F<T> &in, const uint8_t *m0, T dt, int ny, int nx, T dx, int r, int c) {
-    const long i = long(r) * nx + c;
-    const HD<T> me = hd_load<T, FAST>(in, m0, i);
+__device__ __forceinline__ T h1_at(const F<T> &in, const uint8_t *m0, T dt, int ny, int nx, T dx, int r, int c) {
+    const long ro = long(r) * nx;
+    const unsigned uc = unsigned(c);
+    const HD<T> me = hd_load<T, FAST>(in, m0, ro, uc);
     T acc = T(0), P[3], M[3];
     if (c + 1 < nx) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, i + 1);
+        const HD<T> o = hd_load<T, FAST>(in, m0, ro, uc + 1);
         if (hd_active(me, o)) { rx_edge_d<0, T, FAST>(me, o, dx, P, M); acc = acc + M[0]; }
     }
     if (c > 0) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, i - 1);
+        const HD<T> o = hd_load<T, FAST>(in, m0, ro, uc - 1);
         if (hd_active(o, me)) { rx_edge_d<0, T, FAST>(o, me, dx, P, M); acc = acc + P[0]; }
     }
     if (r + 1 < ny) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, i + nx);
+        const HD<T> o = hd_load<T, FAST>(in, m0, ro + nx, uc);
         if (hd_active(me, o)) { rx_edge_d<1, T, FAST>(me, o, dx, P, M); acc = acc + M[0]; }
     }
     if (r > 0) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, i - nx);
+        const HD<T> o = hd_load<T, FAST>(in, m0, ro - nx, uc);
         if (hd_active(o, me)) { rx_edge_d<1, T, FAST>(o, me, dx, P, M); acc = acc + P[0]; }
     }
     return me.h - dt * acc / dx;
@@ -296,12 +318,13 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
     const T dt = *dtp;
     const T dtdx = dt / dx;
     const int c = p.c;
-    HD<T> cur = hd_load<T, FAST>(in, m0, long(p.r0) * nx + p.cc);
+    const unsigned ucc = unsigned(p.cc);
+    HD<T> cur = hd_load<T, FAST>(in, m0, long(p.r0) * nx, ucc);
     T pn0 = zero, pn1 = zero, pn2 = zero;  // upwP of the edge (r-1 | r)
     T zr_n = zero;                          // z + z_b of row r-1
     int nod_n = 0;
     if (p.r0 > 0) {
-        const HD<T> up = hd_load<T, FAST>(in, m0, long(p.r0 - 1) * nx + p.cc);
+        const HD<T> up = hd_load<T, FAST>(in, m0, long(p.r0 - 1) * nx, ucc);
         if (hd_active(up, cur)) {
             T P[3], M[3];
             rx_edge_d<1, T, FAST>(up, cur, dx, P, M);
@@ -311,10 +334,15 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
         nod_n = up.flags & 2;
     }
     T h1_n = zero;  // updated depth of row r-1 (valid from the second row on)
+    // rows r+1 and r+2 are in flight while row r is computed
+    const int rlast = p.r1 < ny - 1 ? p.r1 : ny - 1;  // last row any edge of the strip reads
+    HR<T> q1 = hr_load(in, m0, long(p.r0 + 1 <= rlast ? p.r0 + 1 : rlast) * nx, ucc);
+    HR<T> q2 = hr_load(in, m0, long(p.r0 + 2 <= rlast ? p.r0 + 2 : rlast) * nx, ucc);
     for (int r = p.r0; r < p.r1; ++r) {
         const bool hasS = r + 1 < ny;
-        HD<T> nxt = cur;
-        if (hasS) nxt = hd_load<T, FAST>(in, m0, long(r + 1) * nx + p.cc);
+        const HD<T> nxt = hd_derive<T, FAST>(q1);
+        q1 = q2;
+        if (r + 3 <= rlast) q2 = hr_load(in, m0, long(r + 3) * nx, ucc);
         // x edge (c | c+1)
         const HD<T> east = hd_shdn(cur);
         T Px[3] = {zero, zero, zero}, Mx[3] = {zero, zero, zero};
@@ -350,31 +378,44 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
                 hu1 = zero;
                 hv1 = zero;
             } else {
+                // blocking by a dry neighbour whose bed is above this water
+                // surface; neighbours outside the sweep window (lanes 0 and 63,
+                // the rows above and below) are updated from scratch below
                 const T eta = h1 + cur.zr;
+                const bool inE = p.lane <= SOUT - 1, inW = p.lane >= 2, inN = r > p.r0;
                 bool su = (fE & 2) || (fW & 2);
-                if (!su && eta < zrE) {
-                    const T hE = (p.lane <= SOUT - 1) ? h1E : h1_at<T, FAST>(in, m0, dt, ny, nx, dx, r, c + 1);
-                    su = hE < T_;
-                }
-                if (!su && eta < zrW) {
-                    const T hW = (p.lane >= 2) ? h1W : h1_at<T, FAST>(in, m0, dt, ny, nx, dx, r, c - 1);
-                    su = hW < T_;
-                }
                 bool sv = nod_n || (nxt.flags & 2);
-                if (!sv && eta < zr_n) {
-                    const T hN = (r > p.r0) ? h1_n : h1_at<T, FAST>(in, m0, dt, ny, nx, dx, r - 1, c);
-                    sv = hN < T_;
+                const bool bE = eta < zrE, bW = eta < zrW, bN = eta < zr_n, bS = eta < nxt.zr;
+                if (bE && inE && h1E < T_) su = true;
+                if (bW && inW && h1W < T_) su = true;
+                if (bN && inN && h1_n < T_) sv = true;
+                int far = (bE && !inE ? 1 : 0) | (bW && !inW ? 2 : 0) | (bN && !inN ? 4 : 0) | (bS ? 8 : 0);
+                if (su) far &= ~3;
+                if (sv) far &= ~12;
+                while (far) {
+                    const int k = __builtin_ctz(far);
+                    far &= far - 1;
+                    const int rr = r + (k == 2 ? -1 : (k == 3 ? 1 : 0));
+                    const int cc = c + (k == 0 ? 1 : (k == 1 ? -1 : 0));
+                    if (h1_at<T, FAST>(in, m0, dt, ny, nx, dx, rr, cc) < T_) {
+                        if (k < 2) {
+                            su = true;
+                            far &= ~3;
+                        } else {
+                            sv = true;
+                            far &= ~12;
+                        }
+                    }
                 }
-                if (!sv && eta < nxt.zr) sv = h1_at<T, FAST>(in, m0, dt, ny, nx, dx, r + 1, c) < T_;
                 if (su) hu1 = zero;
                 if (sv) hv1 = zero;
             }
         }
         if (p.out) {
-            const long i = long(r) * nx + c;
-            out.h[i] = h1;
-            out.hu[i] = hu1;
-            out.hv[i] = hv1;
+            const long ro = long(r) * nx;
+            str(out.h, ro, unsigned(c), h1);
+            str(out.hu, ro, unsigned(c), hu1);
+            str(out.hv, ro, unsigned(c), hv1);
         }
         pn0 = Py[0]; pn1 = Py[1]; pn2 = Py[2];
         zr_n = cur.zr;
@@ -421,32 +462,54 @@ __device__ __forceinline__ float mpm_G_fast(float h, float hu, float hv, bool no
     return G;
 }
 
+template <typename T>
+struct ER {          // raw per-cell inputs of the bed sweep
+    T h, hu, hv, zb, G;  // G: the stage input's G (read in constant-G mode only)
+    int m;
+};
+
+template <typename T>
+__device__ __forceinline__ ER<T> er_load(const F<T> &hyd, const T *zb_in, const T *G_in, const uint8_t *m0,
+                                         const pxh_sediments &sd, long ro, unsigned c) {
+    ER<T> q;
+    q.h = ldr(hyd.h, ro, c); q.hu = ldr(hyd.hu, ro, c); q.hv = ldr(hyd.hv, ro, c);
+    q.zb = ldr(zb_in, ro, c);
+    q.G = sd.mpm ? T(0) : ldr(G_in, ro, c);
+    q.m = ldr(m0, ro, c);
+    return q;
+}
+
 template <typename T, bool FAST>
-__device__ __forceinline__ ED<T> ed_load(const F<T> &hyd, const T *zb_in, const T *G_in, const uint8_t *m0,
-                                         const pxh_sediments &sd, long i) {
+__device__ __forceinline__ ED<T> ed_derive(const ER<T> &q, const pxh_sediments &sd) {
     const T T_ = C<T>(DRY_TOL), S_ = C<T>(SED_TOL), zero = T(0);
     ED<T> d;
-    d.h = hyd.h[i]; d.hu = hyd.hu[i]; d.hv = hyd.hv[i];
-    const bool nod = m0[i] != 0;
+    d.h = q.h; d.hu = q.hu; d.hv = q.hv;
+    const bool nod = q.m != 0;
     const bool wet = d.h > T_;
     const T hf = wet ? d.h : C<T>(DRY_FLOOR);
     if constexpr (FAST) {
         using Mf = Mth<float, true>;
-        d.G = sd.mpm ? mpm_G_fast(d.h, d.hu, d.hv, nod, sd) : const_G<T>(d.h, G_in[i], nod);
+        d.G = sd.mpm ? mpm_G_fast(d.h, d.hu, d.hv, nod, sd) : const_G<T>(d.h, q.G, nod);
         const float ih = Mf::rcp(hf);
         d.u = wet ? d.hu * ih : zero;
         d.v = wet ? d.hv * ih : zero;
         d.s = Mf::sqrt(hf);
     } else {
-        d.G = sd.mpm ? mpm_G<T>(d.h, d.hu, d.hv, nod, sd) : const_G<T>(d.h, G_in[i], nod);
+        d.G = sd.mpm ? mpm_G<T>(d.h, d.hu, d.hv, nod, sd) : const_G<T>(d.h, q.G, nod);
         d.u = wet ? d.hu / hf : zero;
         d.v = wet ? d.hv / hf : zero;
         d.s = d_sqrt(hf);
     }
-    d.zb = zb_in[i];
+    d.zb = q.zb;
     d.zbf = d.zb > S_ ? d.zb : C<T>(SED_FLOOR);
     d.flags = (wet ? 1 : 0) | (nod ? 2 : 0);
     return d;
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ ED<T> ed_load(const F<T> &hyd, const T *zb_in, const T *G_in, const uint8_t *m0,
+                                         const pxh_sediments &sd, long ro, unsigned c) {
+    return ed_derive<T, FAST>(er_load(hyd, zb_in, G_in, m0, sd, ro, c), sd);
 }
 
 template <typename T>
@@ -513,117 +576,244 @@ __device__ __forceinline__ void cb_half_sum(float aH, float aL, float bH, float 
     ohi = h2;
 }
 
+// ---------------------------------------------------------------------------
+// CFL speed per edge (roe_exner.py:211-255) on per-cell values
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DD {          // per-cell values for the dt edge
+    T h;             // raw depth (active test)
+    T hf, u, v, s;   // max(h, DRY_TOL), hu/hf, hv/hf (no wet guard, as coded), sqrt(hf)
+    T G;
+    int nod;
+};
+
+template <typename T, bool FAST>
+__device__ __forceinline__ DD<T> dd_make(T h, T hu, T hv, T G, int nod) {
+    DD<T> d;
+    d.h = h;
+    d.hf = pmax(h, C<T>(DRY_TOL));
+    if constexpr (FAST) {
+        using Mf = Mth<float, true>;
+        const float ih = Mf::rcp(d.hf);
+        d.u = hu * ih;
+        d.v = hv * ih;
+        d.s = Mf::sqrt(d.hf);
+    } else {
+        d.u = hu / d.hf;
+        d.v = hv / d.hf;
+        d.s = d_sqrt(d.hf);
+    }
+    d.G = G;
+    d.nod = nod;
+    return d;
+}
+
+template <typename T>
+__device__ __forceinline__ DD<T> dd_shdn(const DD<T> &a) {
+    DD<T> b;
+    b.h = shdn(a.h); b.hf = shdn(a.hf); b.u = shdn(a.u); b.v = shdn(a.v); b.s = shdn(a.s);
+    b.G = shdn(a.G); b.nod = shdn(a.nod);
+    return b;
+}
+
+template <typename T>
+__device__ __forceinline__ bool dd_active(const DD<T> &a, const DD<T> &b) {
+    const T T_ = C<T>(DRY_TOL);
+    return ((a.h > T_) || (b.h > T_)) && !(a.nod || b.nod);
+}
+
+template <int AX, typename T, bool FAST>
+__device__ __forceinline__ T dt_edge_d(const DD<T> &I, const DD<T> &J, T dx) {
+    const T g = C<T>(GRAV);
+    T uhi, vhi, uhj, vhj;
+    rot<AX>(I.u, I.v, uhi, vhi);
+    rot<AX>(J.u, J.v, uhj, vhj);
+    const T ht = T(0.5) * (I.hf + J.hf);
+    const T gt = T(0.5) * (I.G + J.G);
+    const T S = sq(uhi) + uhi * uhj + sq(uhj) + vhi * vhj;
+    T ut, ct, at;
+    if constexpr (FAST) {
+        using Mf = Mth<float, true>;
+        ut = (uhi * I.s + uhj * J.s) * Mf::rcp(I.s + J.s);
+        ct = Mf::sqrt(g * ht);
+        at = gt * S * Mf::rcp(I.s * J.s);
+    } else {
+        ut = (uhi * I.s + uhj * J.s) / (I.s + J.s);
+        ct = d_sqrt(g * ht);
+        at = gt * S / (I.s * J.s);
+    }
+    const T l1 = ut - ct, l3 = ut + ct;
+    const T ss = pmax(d_abs(l1), d_abs(l3));
+    T rs;
+    if (at < C<T>(1e-5)) {
+        rs = ss;  // linear branch of _get_lambda: (ut - ct, ut + ct)
+    } else {
+        double lmin, lmax;
+        get_lambda(static_cast<double>(ut), static_cast<double>(at), static_cast<double>(ct), lmin, lmax);
+        rs = pmax(d_abs(static_cast<T>(lmin)), d_abs(static_cast<T>(lmax)));
+    }
+    const T sp = (rs < C<T>(VEL_TOL)) ? ss : rs;
+    return Mth<T, FAST>::div(dx, sp);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_block_min(T v, T *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & (SW - 1)) == 0) red[threadIdx.x / SW] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 1; k < SWPB; ++k) v = pmin(v, red[k]);
+    return v;
+}
+
 // MODE 0: out.z_b = updated bed (+ out.G when write_G).
-// MODE 1: out = 0.5 * old + 0.5 * (stage result) over h, hu, hv, z_b, G.
+// MODE 1: out = 0.5 * old + 0.5 * (stage result) over h, hu, hv, z_b, G; with
+// DT also the CFL dt partial minimum of the combined state into partial[block].
 // zb_lo_in / out.z_b_lo / old.z_b_lo: compensated bed (float32), else null.
-template <typename T, bool FAST, int MODE>
+template <typename T, bool FAST, int MODE, bool DT>
 __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__restrict__ zb_in,
                                                            const T *__restrict__ zb_lo_in, const T *__restrict__ G_in,
                                                            F<T> out, F<T> old, const uint8_t *__restrict__ m0,
                                                            const T *__restrict__ dtp, int ny, int nx, T dx,
-                                                           pxh_sediments sd, int write_G) {
+                                                           pxh_sediments sd, int write_G, T *__restrict__ partial) {
+    __shared__ T red[SWPB];
     const StripPos p = strip_pos(ny, nx);
-    if (p.r0 >= ny) return;
-    const T zero = T(0);
-    const T dt = *dtp;
-    const T dtdx = dt / dx;
+    const T zero = T(0), hf = T(0.5);
     const int c = p.c;
-    ED<T> cur = ed_load<T, FAST>(hyd, zb_in, G_in, m0, sd, long(p.r0) * nx + p.cc);
-    T fn = zero;  // flux through the edge (r-1 | r)
-    if (p.r0 > 0) {
-        const ED<T> up = ed_load<T, FAST>(hyd, zb_in, G_in, m0, sd, long(p.r0 - 1) * nx + p.cc);
-        if (ed_active(up, cur)) fn = ex_edge_d<1, T, FAST>(up, cur);
-    }
-    for (int r = p.r0; r < p.r1; ++r) {
-        const bool hasS = r + 1 < ny;
-        ED<T> nxt = cur;
-        if (hasS) nxt = ed_load<T, FAST>(hyd, zb_in, G_in, m0, sd, long(r + 1) * nx + p.cc);
-        const ED<T> east = ed_shdn(cur);
-        T fx = zero;
-        if (c + 1 < nx && ed_active(cur, east)) fx = ex_edge_d<0, T, FAST>(cur, east);
-        const T fw = shup(fx);
-        T fy = zero;
-        if (hasS && ed_active(cur, nxt)) fy = ex_edge_d<1, T, FAST>(cur, nxt);
-        T div = zero;
-        if (c > 0) div = div + (-fw);
-        if (c + 1 < nx) div = div + fx;
-        if (r > 0) div = div + (-fn);
-        if (hasS) div = div + fy;
-        if (p.out) {
-            const long i = long(r) * nx + c;
-            const T inc = FAST ? dtdx * div : dt * div / dx;
-            T zb1 = cur.zb - inc, zl1 = zero;
-            if constexpr (sizeof(T) == 4) {
-                if (zb_lo_in) cb_sub(cur.zb, zb_lo_in[i], inc, zb1, zl1);
-            }
-            if (MODE == 0) {
-                out.z_b[i] = zb1;
-                if constexpr (sizeof(T) == 4) {
-                    if (out.z_b_lo) out.z_b_lo[i] = zl1;
-                }
-                if (write_G) out.G[i] = cur.G;
-            } else {
-                const T hf = T(0.5);
-                out.h[i] = hf * old.h[i] + hf * cur.h;
-                out.hu[i] = hf * old.hu[i] + hf * cur.hu;
-                out.hv[i] = hf * old.hv[i] + hf * cur.hv;
-                out.G[i] = hf * old.G[i] + hf * cur.G;
-                T zc = hf * old.z_b[i] + hf * zb1;
-                if constexpr (sizeof(T) == 4) {
-                    if (out.z_b_lo) {
-                        T zcl;
-                        cb_half_sum(old.z_b[i], old.z_b_lo[i], zb1, zl1, zc, zcl);
-                        out.z_b_lo[i] = zcl;
-                    }
-                }
-                out.z_b[i] = zc;
-            }
+    T vmin = inf_v<T>();
+    if (p.r0 < ny) {
+        const T dt = *dtp;
+        const T dtdx = dt / dx;
+        const int rlast = p.r1 < ny - 1 ? p.r1 : ny - 1;
+        const unsigned ucc = unsigned(p.cc), uc = unsigned(c);
+        auto row = [&](int r) { return long(r <= rlast ? r : rlast) * nx; };
+        ED<T> cur = ed_load<T, FAST>(hyd, zb_in, G_in, m0, sd, row(p.r0), ucc);
+        T fn = zero;  // flux through the edge (r-1 | r)
+        if (p.r0 > 0) {
+            const ED<T> up = ed_load<T, FAST>(hyd, zb_in, G_in, m0, sd, long(p.r0 - 1) * nx, ucc);
+            if (ed_active(up, cur)) fn = ex_edge_d<1, T, FAST>(up, cur);
         }
-        fn = fy;
-        cur = nxt;
+        ER<T> q1 = er_load(hyd, zb_in, G_in, m0, sd, row(p.r0 + 1), ucc);
+        ER<T> q2 = er_load(hyd, zb_in, G_in, m0, sd, row(p.r0 + 2), ucc);
+        // previous-step values of rows r and r+1 for the combine
+        T oh0 = zero, ohu0 = zero, ohv0 = zero, oG0 = zero, oh1 = zero, ohu1 = zero, ohv1 = zero, oG1 = zero;
+        if (MODE == 1) {
+            const long i0 = row(p.r0), i1 = row(p.r0 + 1);
+            oh0 = ldr(old.h, i0, ucc); ohu0 = ldr(old.hu, i0, ucc); ohv0 = ldr(old.hv, i0, ucc);
+            oG0 = ldr(old.G, i0, ucc);
+            oh1 = ldr(old.h, i1, ucc); ohu1 = ldr(old.hu, i1, ucc); ohv1 = ldr(old.hv, i1, ucc);
+            oG1 = ldr(old.G, i1, ucc);
+        }
+        DD<T> dcur;
+        if (DT) dcur = dd_make<T, FAST>(hf * oh0 + hf * cur.h, hf * ohu0 + hf * cur.hu, hf * ohv0 + hf * cur.hv,
+                                        hf * oG0 + hf * cur.G, cur.flags & 2);
+        for (int r = p.r0; r < p.r1; ++r) {
+            const bool hasS = r + 1 < ny;
+            const ED<T> nxt = ed_derive<T, FAST>(q1, sd);
+            q1 = q2;
+            if (r + 3 <= rlast) q2 = er_load(hyd, zb_in, G_in, m0, sd, long(r + 3) * nx, ucc);
+            T oh2 = zero, ohu2 = zero, ohv2 = zero, oG2 = zero;
+            if (MODE == 1) {
+                const long i2 = row(r + 2);
+                oh2 = ldr(old.h, i2, ucc); ohu2 = ldr(old.hu, i2, ucc); ohv2 = ldr(old.hv, i2, ucc);
+                oG2 = ldr(old.G, i2, ucc);
+            }
+            const ED<T> east = ed_shdn(cur);
+            T fx = zero;
+            if (c + 1 < nx && ed_active(cur, east)) fx = ex_edge_d<0, T, FAST>(cur, east);
+            const T fw = shup(fx);
+            T fy = zero;
+            if (hasS && ed_active(cur, nxt)) fy = ex_edge_d<1, T, FAST>(cur, nxt);
+            T div = zero;
+            if (c > 0) div = div + (-fw);
+            if (c + 1 < nx) div = div + fx;
+            if (r > 0) div = div + (-fn);
+            if (hasS) div = div + fy;
+            if (p.out) {
+                const long i = long(r) * nx;  // row offset; column uc
+                const T inc = FAST ? dtdx * div : dt * div / dx;
+                T zb1 = cur.zb - inc, zl1 = zero;
+                if constexpr (sizeof(T) == 4) {
+                    if (zb_lo_in) cb_sub(cur.zb, ldr(zb_lo_in, i, uc), inc, zb1, zl1);
+                }
+                if (MODE == 0) {
+                    str(out.z_b, i, uc, zb1);
+                    if constexpr (sizeof(T) == 4) {
+                        if (out.z_b_lo) str(out.z_b_lo, i, uc, zl1);
+                    }
+                    if (write_G) str(out.G, i, uc, cur.G);
+                } else {
+                    str(out.h, i, uc, hf * oh0 + hf * cur.h);
+                    str(out.hu, i, uc, hf * ohu0 + hf * cur.hu);
+                    str(out.hv, i, uc, hf * ohv0 + hf * cur.hv);
+                    str(out.G, i, uc, hf * oG0 + hf * cur.G);
+                    const T ozb = ldr(old.z_b, i, uc);
+                    T zc = hf * ozb + hf * zb1;
+                    if constexpr (sizeof(T) == 4) {
+                        if (out.z_b_lo) {
+                            T zcl;
+                            cb_half_sum(ozb, ldr(old.z_b_lo, i, uc), zb1, zl1, zc, zcl);
+                            str(out.z_b_lo, i, uc, zcl);
+                        }
+                    }
+                    str(out.z_b, i, uc, zc);
+                }
+            }
+            if (DT) {
+                // edges (c | c+1) of row r and (r | r+1), owned by the output lanes
+                DD<T> dnxt = dcur;
+                if (hasS)
+                    dnxt = dd_make<T, FAST>(hf * oh1 + hf * nxt.h, hf * ohu1 + hf * nxt.hu, hf * ohv1 + hf * nxt.hv,
+                                            hf * oG1 + hf * nxt.G, nxt.flags & 2);
+                const DD<T> deast = dd_shdn(dcur);
+                if (p.out && c + 1 < nx && dd_active(dcur, deast)) vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
+                if (p.out && hasS && dd_active(dcur, dnxt)) vmin = pmin(vmin, dt_edge_d<1, T, FAST>(dcur, dnxt, dx));
+                dcur = dnxt;
+            }
+            fn = fy;
+            cur = nxt;
+            oh0 = oh1; ohu0 = ohu1; ohv0 = ohv1; oG0 = oG1;
+            oh1 = oh2; ohu1 = ohu2; ohv1 = ohv2; oG1 = oG2;
+        }
+    }
+    if (DT) {
+        vmin = wave_block_min(vmin, red);
+        if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = vmin;
     }
 }
 
 // ---------------------------------------------------------------------------
 // CFL dt partial minima (roe_exner.py:257-310), one partial per block
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool FAST>
 __global__ __launch_bounds__(SW *SWPB) void k_rx_dt_strip(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx,
                                                           T dx, T *__restrict__ partial) {
     __shared__ T red[SWPB];
     const StripPos p = strip_pos(ny, nx);
-    const T T_ = C<T>(DRY_TOL);
     T v = inf_v<T>();
     if (p.r0 < ny) {
-        const long i0 = long(p.r0) * nx + p.cc;
-        T h = s.h[i0], hu = s.hu[i0], hv = s.hv[i0], G = s.G[i0];
-        int nod = m0[i0];
+        const unsigned ucc = unsigned(p.cc);
+        const long i0 = long(p.r0) * nx;
+        DD<T> cur = dd_make<T, FAST>(ldr(s.h, i0, ucc), ldr(s.hu, i0, ucc), ldr(s.hv, i0, ucc), ldr(s.G, i0, ucc),
+                                     ldr(m0, i0, ucc));
         for (int r = p.r0; r < p.r1; ++r) {
             const bool hasS = r + 1 < ny;
-            T hS = h, huS = hu, hvS = hv, GS = G;
-            int nodS = nod;
+            DD<T> nxt = cur;
             if (hasS) {
-                const long j = long(r + 1) * nx + p.cc;
-                hS = s.h[j]; huS = s.hu[j]; hvS = s.hv[j]; GS = s.G[j]; nodS = m0[j];
+                const long j = long(r + 1) * nx;
+                nxt = dd_make<T, FAST>(ldr(s.h, j, ucc), ldr(s.hu, j, ucc), ldr(s.hv, j, ucc), ldr(s.G, j, ucc),
+                                       ldr(m0, j, ucc));
             }
-            const T hE = shdn(h), huE = shdn(hu), hvE = shdn(hv), GE = shdn(G);
-            const int nodE = shdn(nod);
+            const DD<T> east = dd_shdn(cur);
             // lanes 1..62 own the x edge (c | c+1) and the y edge (r | r+1) of their column
-            if (p.out && p.c + 1 < nx && ((h > T_) || (hE > T_)) && !(nod || nodE))
-                v = pmin(v, dt_edge<0>(h, hu, hv, G, hE, huE, hvE, GE, dx));
-            if (p.out && hasS && ((h > T_) || (hS > T_)) && !(nod || nodS))
-                v = pmin(v, dt_edge<1>(h, hu, hv, G, hS, huS, hvS, GS, dx));
-            h = hS; hu = huS; hv = hvS; G = GS; nod = nodS;
+            if (p.out && p.c + 1 < nx && dd_active(cur, east)) v = pmin(v, dt_edge_d<0, T, FAST>(cur, east, dx));
+            if (p.out && hasS && dd_active(cur, nxt)) v = pmin(v, dt_edge_d<1, T, FAST>(cur, nxt, dx));
+            cur = nxt;
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
-    if (p.lane == 0) red[threadIdx.x / SW] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < SWPB; ++k) v = pmin(v, red[k]);
-        partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
-    }
+    v = wave_block_min(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
 }
 
 }  // namespace pxh

@@ -97,10 +97,11 @@ class ExnerBackend:
     def combine_into(self, fa, fb, fo, stream):
         _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
 
-    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream):
-        """dt, two stages and the combine; a is advanced in place."""
+    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False):
+        """dt (unless dt_ready), two stages and the combine; a is advanced in
+        place and dt_ptr receives the dt of the new a."""
         _hip.call("pxh_roe_exner_ssprk2", self.grid, fa, fb, fc, mask0_ptr, self.sed, self.bc_step, float(cfl),
-                  dt_ptr, self.workspace.data_ptr(), stream)
+                  dt_ptr, self.workspace.data_ptr(), int(bool(dt_ready)), stream)
 
 
 def _stream():

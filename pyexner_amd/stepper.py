@@ -34,6 +34,7 @@ class ExnerStepper:
         self.c = _like(self.a, names)
         self.fa, self.fb, self.fc = (fields_struct(s) for s in (self.a, self.b, self.c))
         self.dt = torch.zeros((), dtype=state.h.dtype, device=state.h.device)
+        self.dt_ready = False  # self.dt holds compute_dt_fn(self.a)
         self.graph = None
 
     @property
@@ -45,9 +46,13 @@ class ExnerStepper:
 
     def compute_dt(self):
         self.be.dt_into(self.fa, self.m0, self.cfl, self.dt.data_ptr(), self._stream())
+        self.dt_ready = True
 
     def ssprk2_step(self):
-        self.be.ssprk2_into(self.fa, self.fb, self.fc, self.m0, self.cfl, self.dt.data_ptr(), self._stream())
+        """One SSPRK2 step; afterwards self.dt holds the dt of the new state."""
+        self.be.ssprk2_into(self.fa, self.fb, self.fc, self.m0, self.cfl, self.dt.data_ptr(), self._stream(),
+                            dt_ready=self.dt_ready)
+        self.dt_ready = True
 
     def fe_step(self):
         st = self._stream()
@@ -56,6 +61,7 @@ class ExnerStepper:
         self.be.step_into(self.fa, self.fb, self.m0, dtp, st)
         self.a, self.b = self.b, self.a
         self.fa, self.fb = self.fb, self.fa
+        self.dt_ready = False
 
     def capture(self, nsteps=1, integrator="SSPRK2"):
         """Record nsteps steps into a hipGraph (FE needs an even count so the
@@ -66,7 +72,7 @@ class ExnerStepper:
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
-            fn()  # warm-up outside capture
+            fn()  # warm-up outside capture; leaves dt_ready set for the captured SSPRK2 steps
         torch.cuda.current_stream().wait_stream(s)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):

@@ -231,7 +231,8 @@ def test_stepper_ssprk2_matches_oracle_f64(cuda, shape):
     for _ in range(40):
         so, dto = osteps.ssprk2_step(prob, so)
         st.ssprk2_step()
-    assert abs(float(st.dt) - float(dto)) <= 1e-13 * float(dto)
+    dnext = float(prob.dt(so))  # the step leaves the dt of the new state (fused reduction)
+    assert abs(float(st.dt) - dnext) <= 1e-13 * dnext
     errs = compare(st.state, so, FIELDS_EXNER)
     _print(f"stepper {shape}", errs)
     assert max(errs.values()) <= F64_TOL
