@@ -47,6 +47,12 @@ static inline dim3 strip_grid(int ny, int nx) {
     return dim3(sx, (sy + SWPB - 1) / SWPB);
 }
 
+// Wait for every outstanding vector-memory load (vmcnt(0); expcnt and lgkmcnt
+// left free).  Issued once before a row loop whose software pipeline keeps
+// loads in flight across iterations: with the prologue's loads drained, the
+// waits inside the loop only cover the loads the loop itself issued.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 template <typename T>
 __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, SW); }
 template <typename T>
@@ -218,12 +224,42 @@ __device__ __forceinline__ void rx_edge_exact(const HD<T> &I, const HD<T> &J, T 
     rx_upwind<AX>(l1, l2, l3, lE1, lE3, a1, a2, a3, b1, vt, ct, P, M);
 }
 
+// Upwinding by eigenvector coefficients: every wave's flux lies along one
+// column of P (r1 = (1, l1, vt), r2 = (0, 0, ct), r3 = (1, l3, vt)), so the
+// coefficients going to each side are summed first and the two vectors are
+// assembled once.  Mathematically the same split as rx_upwind, with a
+// different summation order (float32 hardware path only).
+template <int AX>
+__device__ __forceinline__ void rx_upwind_coef(float l1, float l2, float l3, float lE1, float lE3, float a1,
+                                               float a2, float a3, float b1, float vt, float ct, float P[3],
+                                               float M[3]) {
+#pragma clang fp contract(fast)
+    const float zero = 0.0f;
+    const float f1 = l1 * a1 - b1, f2 = l2 * a2, f3 = l3 * a3 + b1;
+    const float e1 = lE1 * a1, e3 = lE3 * a3;
+    // a NaN speed sends its wave to neither side, as in rx_upwind
+    const float p1 = (l1 > zero ? f1 : zero) + (lE1 > zero ? e1 : zero);
+    const float m1 = (l1 <= zero ? f1 : zero) + (lE1 <= zero ? e1 : zero);
+    const float p3 = (l3 > zero ? f3 : zero) + (lE3 > zero ? e3 : zero);
+    const float m3 = (l3 <= zero ? f3 : zero) + (lE3 <= zero ? e3 : zero);
+    const float p2 = l2 > zero ? f2 : zero;
+    const float m2 = l2 <= zero ? f2 : zero;
+    float P1 = p1 * l1 + p3 * l3, M1 = m1 * l1 + m3 * l3;
+    const float P0 = p1 + p3, M0 = m1 + m3;
+    float P2 = P0 * vt + p2 * ct, M2 = M0 * vt + m2 * ct;
+    unrot<AX>(P1, P2);
+    unrot<AX>(M1, M2);
+    P[0] = P0; P[1] = P1; P[2] = P2;
+    M[0] = M0; M[1] = M1; M[2] = M2;
+}
+
 // The same edge in float32 hardware arithmetic: one reciprocal for the Roe
 // average, one reciprocal square root for the celerity and its inverse, and
 // h~^(4/3) as exp2(4/3 log2 h~).
 template <int AX>
 __device__ __forceinline__ void rx_edge_fast(const HD<float> &I, const HD<float> &J, float dx, float P[3],
                                              float M[3]) {
+#pragma clang fp contract(fast)
     using Mf = Mth<float, true>;
     const float T_ = float(DRY_TOL), g = float(GRAV), V = float(VEL_TOL), zero = 0.0f;
     const float hi = I.hf, hj = J.hf;
@@ -269,7 +305,7 @@ __device__ __forceinline__ void rx_edge_fast(const HD<float> &I, const HD<float>
         src = src - g * ht * sf * dx;
     }
     const float b1 = src * ih2;
-    rx_upwind<AX>(l1, l2, l3, lE1, lE3, a1, a2, a3, b1, vt, ct, P, M);
+    rx_upwind_coef<AX>(l1, l2, l3, lE1, lE3, a1, a2, a3, b1, vt, ct, P, M);
 }
 
 template <int AX, typename T, bool FAST>
@@ -338,6 +374,7 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
     const int rlast = p.r1 < ny - 1 ? p.r1 : ny - 1;  // last row any edge of the strip reads
     HR<T> q1 = hr_load(in, m0, long(p.r0 + 1 <= rlast ? p.r0 + 1 : rlast) * nx, ucc);
     HR<T> q2 = hr_load(in, m0, long(p.r0 + 2 <= rlast ? p.r0 + 2 : rlast) * nx, ucc);
+    wait_vm_all();
     for (int r = p.r0; r < p.r1; ++r) {
         const bool hasS = r + 1 < ny;
         const HD<T> nxt = hd_derive<T, FAST>(q1);
