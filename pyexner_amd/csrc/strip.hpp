@@ -20,7 +20,7 @@
 namespace pxh {
 
 constexpr int SW = 64;       // lanes per strip
-constexpr int SOUT = 62;     // output columns per strip
+constexpr int SOUT = 48;     // output columns per strip
 constexpr int SROWS = 32;    // rows per strip
 constexpr int SWPB = 4;      // strips (wavefronts) per block, stacked in y
 
@@ -53,10 +53,30 @@ static inline dim3 strip_grid(int ny, int nx) {
 // waits inside the loop only cover the loads the loop itself issued.
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Whole-wavefront shifts by one lane on the DPP path (no LDS round trip):
+// wave_shl:1 gives lane i the value of lane i+1, wave_shr:1 that of lane i-1;
+// the lane with no source (63, resp. 0) keeps its own value, as __shfl_down /
+// __shfl_up do.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false); }
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_shift(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lane values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, dpp_i32<CTRL>(__builtin_bit_cast(int, v)));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = dpp_i32<CTRL>(int(b & 0xffffffffLL));
+        const int hi = dpp_i32<CTRL>(int(b >> 32));
+        return __builtin_bit_cast(T, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    }
+}
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
 template <typename T>
-__device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, SW); }
+__device__ __forceinline__ T shdn(T v) { return dpp_shift<DPP_WAVE_SHL1>(v); }
 template <typename T>
-__device__ __forceinline__ T shup(T v) { return __shfl_up(v, 1, SW); }
+__device__ __forceinline__ T shup(T v) { return dpp_shift<DPP_WAVE_SHR1>(v); }
 
 // ---------------------------------------------------------------------------
 // arithmetic policies
@@ -136,6 +156,18 @@ __device__ __forceinline__ HD<T> hd_derive(const HR<T> &q) {
 template <typename T, bool FAST>
 __device__ __forceinline__ HD<T> hd_load(const F<T> &s, const uint8_t *m0, long ro, unsigned c) {
     return hd_derive<T, FAST>(hr_load(s, m0, ro, c));
+}
+
+// The same for a cell whose row may differ from lane to lane (per-lane 64-bit
+// element index; no wavefront-uniform row base).
+template <typename T, bool FAST>
+__device__ __forceinline__ HD<T> hd_load_lane(const F<T> &s, const uint8_t *m0, long i) {
+    HR<T> q;
+    q.h = ((const PXH_GLOBAL T *)s.h)[i]; q.hu = ((const PXH_GLOBAL T *)s.hu)[i];
+    q.hv = ((const PXH_GLOBAL T *)s.hv)[i]; q.zb = ((const PXH_GLOBAL T *)s.z_b)[i];
+    q.z = ((const PXH_GLOBAL T *)s.z)[i]; q.n = ((const PXH_GLOBAL T *)s.n_b)[i];
+    q.m = ((const PXH_GLOBAL uint8_t *)m0)[i];
+    return hd_derive<T, FAST>(q);
 }
 
 template <typename T>
@@ -318,26 +350,27 @@ __device__ __forceinline__ void rx_edge_d(const HD<T> &I, const HD<T> &J, T dx, 
 // used only when momentum corrections need a neighbour outside the window.
 // Inlined once, in a loop over the pending neighbours, rather than called: a
 // call would make the whole sweep keep its live registers across the ABI.
+// (r, c) differ from lane to lane (each lane checks its own pending neighbour),
+// so every load here is addressed per lane.
 template <typename T, bool FAST>
 __device__ __forceinline__ T h1_at(const F<T> &in, const uint8_t *m0, T dt, int ny, int nx, T dx, int r, int c) {
-    const long ro = long(r) * nx;
-    const unsigned uc = unsigned(c);
-    const HD<T> me = hd_load<T, FAST>(in, m0, ro, uc);
+    const long i = long(r) * nx + c;
+    const HD<T> me = hd_load_lane<T, FAST>(in, m0, i);
     T acc = T(0), P[3], M[3];
     if (c + 1 < nx) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, ro, uc + 1);
+        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i + 1);
         if (hd_active(me, o)) { rx_edge_d<0, T, FAST>(me, o, dx, P, M); acc = acc + M[0]; }
     }
     if (c > 0) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, ro, uc - 1);
+        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i - 1);
         if (hd_active(o, me)) { rx_edge_d<0, T, FAST>(o, me, dx, P, M); acc = acc + P[0]; }
     }
     if (r + 1 < ny) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, ro + nx, uc);
+        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i + nx);
         if (hd_active(me, o)) { rx_edge_d<1, T, FAST>(me, o, dx, P, M); acc = acc + M[0]; }
     }
     if (r > 0) {
-        const HD<T> o = hd_load<T, FAST>(in, m0, ro - nx, uc);
+        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i - nx);
         if (hd_active(o, me)) { rx_edge_d<1, T, FAST>(o, me, dx, P, M); acc = acc + P[0]; }
     }
     return me.h - dt * acc / dx;
@@ -470,6 +503,7 @@ struct ED {          // derived per-cell values for the Exner edge
     T h, hu, hv;     // raw (post-fill) hydro state
     T u, v, s;       // velocities with the Exner floors, sqrt of floored depth
     T zb, zbf;       // raw and floored bed
+    T zl;            // low part of the compensated float32 bed (0 when unused)
     T G;
     int flags;       // bit0 wet, bit1 nodata
 };
@@ -502,18 +536,56 @@ __device__ __forceinline__ float mpm_G_fast(float h, float hu, float hv, bool no
 template <typename T>
 struct ER {          // raw per-cell inputs of the bed sweep
     T h, hu, hv, zb, G;  // G: the stage input's G (read in constant-G mode only)
+    T zl;                // compensated-bed low part (0 when the bed is not compensated)
     int m;
 };
 
 template <typename T>
-__device__ __forceinline__ ER<T> er_load(const F<T> &hyd, const T *zb_in, const T *G_in, const uint8_t *m0,
-                                         const pxh_sediments &sd, long ro, unsigned c) {
+__device__ __forceinline__ ER<T> er_load(const F<T> &hyd, const T *zb_in, const T *zb_lo_in, const T *G_in,
+                                         const uint8_t *m0, const pxh_sediments &sd, long ro, unsigned c) {
     ER<T> q;
     q.h = ldr(hyd.h, ro, c); q.hu = ldr(hyd.hu, ro, c); q.hv = ldr(hyd.hv, ro, c);
     q.zb = ldr(zb_in, ro, c);
     q.G = sd.mpm ? T(0) : ldr(G_in, ro, c);
+    q.zl = T(0);
+    if constexpr (sizeof(T) == 4) {
+        if (zb_lo_in) q.zl = ldr(zb_lo_in, ro, c);
+    }
     q.m = ldr(m0, ro, c);
     return q;
+}
+
+template <typename T>
+struct OR {          // previous-step values of one cell (SSPRK2 combine)
+    T h, hu, hv, G, zb, zl;
+};
+
+template <typename T>
+__device__ __forceinline__ OR<T> or_load(const F<T> &old, long ro, unsigned c) {
+    OR<T> o;
+    o.h = ldr(old.h, ro, c); o.hu = ldr(old.hu, ro, c); o.hv = ldr(old.hv, ro, c);
+    o.G = ldr(old.G, ro, c); o.zb = ldr(old.z_b, ro, c);
+    o.zl = T(0);
+    if constexpr (sizeof(T) == 4) {
+        if (old.z_b_lo) o.zl = ldr(old.z_b_lo, ro, c);
+    }
+    return o;
+}
+
+template <typename T>
+struct CB {          // combined state 0.5 u^n + 0.5 u^(2) of one cell (h, hu, hv, G)
+    T h, hu, hv, G;
+};
+
+template <typename T>
+__device__ __forceinline__ CB<T> cb_make(const OR<T> &o, T h, T hu, T hv, T G) {
+    const T hf = T(0.5);
+    CB<T> b;
+    b.h = hf * o.h + hf * h;
+    b.hu = hf * o.hu + hf * hu;
+    b.hv = hf * o.hv + hf * hv;
+    b.G = hf * o.G + hf * G;
+    return b;
 }
 
 template <typename T, bool FAST>
@@ -538,15 +610,16 @@ __device__ __forceinline__ ED<T> ed_derive(const ER<T> &q, const pxh_sediments &
         d.s = d_sqrt(hf);
     }
     d.zb = q.zb;
+    d.zl = q.zl;
     d.zbf = d.zb > S_ ? d.zb : C<T>(SED_FLOOR);
     d.flags = (wet ? 1 : 0) | (nod ? 2 : 0);
     return d;
 }
 
 template <typename T, bool FAST>
-__device__ __forceinline__ ED<T> ed_load(const F<T> &hyd, const T *zb_in, const T *G_in, const uint8_t *m0,
-                                         const pxh_sediments &sd, long ro, unsigned c) {
-    return ed_derive<T, FAST>(er_load(hyd, zb_in, G_in, m0, sd, ro, c), sd);
+__device__ __forceinline__ ED<T> ed_load(const F<T> &hyd, const T *zb_in, const T *zb_lo_in, const T *G_in,
+                                         const uint8_t *m0, const pxh_sediments &sd, long ro, unsigned c) {
+    return ed_derive<T, FAST>(er_load(hyd, zb_in, zb_lo_in, G_in, m0, sd, ro, c), sd);
 }
 
 template <typename T>
@@ -708,6 +781,8 @@ __device__ __forceinline__ T wave_block_min(T v, T *red) {
 // MODE 1: out = 0.5 * old + 0.5 * (stage result) over h, hu, hv, z_b, G; with
 // DT also the CFL dt partial minimum of the combined state into partial[block].
 // zb_lo_in / out.z_b_lo / old.z_b_lo: compensated bed (float32), else null.
+// Every per-row input (the stage rows and the previous-step rows) is loaded
+// two rows ahead of its use.
 template <typename T, bool FAST, int MODE, bool DT>
 __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__restrict__ zb_in,
                                                            const T *__restrict__ zb_lo_in, const T *__restrict__ G_in,
@@ -725,36 +800,48 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
         const int rlast = p.r1 < ny - 1 ? p.r1 : ny - 1;
         const unsigned ucc = unsigned(p.cc), uc = unsigned(c);
         auto row = [&](int r) { return long(r <= rlast ? r : rlast) * nx; };
-        ED<T> cur = ed_load<T, FAST>(hyd, zb_in, G_in, m0, sd, row(p.r0), ucc);
+        ED<T> cur = ed_load<T, FAST>(hyd, zb_in, zb_lo_in, G_in, m0, sd, row(p.r0), ucc);
         T fn = zero;  // flux through the edge (r-1 | r)
         if (p.r0 > 0) {
-            const ED<T> up = ed_load<T, FAST>(hyd, zb_in, G_in, m0, sd, long(p.r0 - 1) * nx, ucc);
+            const ED<T> up = ed_load<T, FAST>(hyd, zb_in, zb_lo_in, G_in, m0, sd, long(p.r0 - 1) * nx, ucc);
             if (ed_active(up, cur)) fn = ex_edge_d<1, T, FAST>(up, cur);
         }
-        ER<T> q1 = er_load(hyd, zb_in, G_in, m0, sd, row(p.r0 + 1), ucc);
-        ER<T> q2 = er_load(hyd, zb_in, G_in, m0, sd, row(p.r0 + 2), ucc);
-        // previous-step values of rows r and r+1 for the combine
-        T oh0 = zero, ohu0 = zero, ohv0 = zero, oG0 = zero, oh1 = zero, ohu1 = zero, ohv1 = zero, oG1 = zero;
+        ER<T> q1 = er_load(hyd, zb_in, zb_lo_in, G_in, m0, sd, row(p.r0 + 1), ucc);
+        ER<T> q2 = er_load(hyd, zb_in, zb_lo_in, G_in, m0, sd, row(p.r0 + 2), ucc);
+        // combine: the combined state and previous-step bed of row r; the
+        // previous-step rows r+1 and r+2 in flight
+        CB<T> cbc = {zero, zero, zero, zero};
+        T ozb = zero, ozl = zero;
+        OR<T> o1 = {zero, zero, zero, zero, zero, zero}, o2 = o1;
         if (MODE == 1) {
-            const long i0 = row(p.r0), i1 = row(p.r0 + 1);
-            oh0 = ldr(old.h, i0, ucc); ohu0 = ldr(old.hu, i0, ucc); ohv0 = ldr(old.hv, i0, ucc);
-            oG0 = ldr(old.G, i0, ucc);
-            oh1 = ldr(old.h, i1, ucc); ohu1 = ldr(old.hu, i1, ucc); ohv1 = ldr(old.hv, i1, ucc);
-            oG1 = ldr(old.G, i1, ucc);
+            const OR<T> o0 = or_load(old, row(p.r0), ucc);
+            cbc = cb_make(o0, cur.h, cur.hu, cur.hv, cur.G);
+            ozb = o0.zb;
+            ozl = o0.zl;
+            o1 = or_load(old, row(p.r0 + 1), ucc);
+            o2 = or_load(old, row(p.r0 + 2), ucc);
         }
         DD<T> dcur;
-        if (DT) dcur = dd_make<T, FAST>(hf * oh0 + hf * cur.h, hf * ohu0 + hf * cur.hu, hf * ohv0 + hf * cur.hv,
-                                        hf * oG0 + hf * cur.G, cur.flags & 2);
+        if (DT) dcur = dd_make<T, FAST>(cbc.h, cbc.hu, cbc.hv, cbc.G, cur.flags & 2);
+        wait_vm_all();
         for (int r = p.r0; r < p.r1; ++r) {
             const bool hasS = r + 1 < ny;
             const ED<T> nxt = ed_derive<T, FAST>(q1, sd);
-            q1 = q2;
-            if (r + 3 <= rlast) q2 = er_load(hyd, zb_in, G_in, m0, sd, long(r + 3) * nx, ucc);
-            T oh2 = zero, ohu2 = zero, ohv2 = zero, oG2 = zero;
+            // Row r+1's previous-step values are consumed here, before the
+            // refill: a register set still read after its refill is issued
+            // would be copied at the loop latch, waiting for the new loads.
+            CB<T> cbn = cbc;
+            T ozb_n = zero, ozl_n = zero;
             if (MODE == 1) {
-                const long i2 = row(r + 2);
-                oh2 = ldr(old.h, i2, ucc); ohu2 = ldr(old.hu, i2, ucc); ohv2 = ldr(old.hv, i2, ucc);
-                oG2 = ldr(old.G, i2, ucc);
+                cbn = cb_make(o1, nxt.h, nxt.hu, nxt.hv, nxt.G);
+                ozb_n = o1.zb;
+                ozl_n = o1.zl;
+                o1 = o2;
+            }
+            q1 = q2;
+            if (r + 3 <= rlast) {
+                q2 = er_load(hyd, zb_in, zb_lo_in, G_in, m0, sd, long(r + 3) * nx, ucc);
+                if (MODE == 1) o2 = or_load(old, long(r + 3) * nx, ucc);
             }
             const ED<T> east = ed_shdn(cur);
             T fx = zero;
@@ -772,7 +859,7 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
                 const T inc = FAST ? dtdx * div : dt * div / dx;
                 T zb1 = cur.zb - inc, zl1 = zero;
                 if constexpr (sizeof(T) == 4) {
-                    if (zb_lo_in) cb_sub(cur.zb, ldr(zb_lo_in, i, uc), inc, zb1, zl1);
+                    if (zb_lo_in) cb_sub(cur.zb, cur.zl, inc, zb1, zl1);
                 }
                 if (MODE == 0) {
                     str(out.z_b, i, uc, zb1);
@@ -781,37 +868,38 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
                     }
                     if (write_G) str(out.G, i, uc, cur.G);
                 } else {
-                    str(out.h, i, uc, hf * oh0 + hf * cur.h);
-                    str(out.hu, i, uc, hf * ohu0 + hf * cur.hu);
-                    str(out.hv, i, uc, hf * ohv0 + hf * cur.hv);
-                    str(out.G, i, uc, hf * oG0 + hf * cur.G);
-                    const T ozb = ldr(old.z_b, i, uc);
+                    str(out.h, i, uc, cbc.h);
+                    str(out.hu, i, uc, cbc.hu);
+                    str(out.hv, i, uc, cbc.hv);
+                    str(out.G, i, uc, cbc.G);
                     T zc = hf * ozb + hf * zb1;
                     if constexpr (sizeof(T) == 4) {
                         if (out.z_b_lo) {
                             T zcl;
-                            cb_half_sum(ozb, ldr(old.z_b_lo, i, uc), zb1, zl1, zc, zcl);
+                            cb_half_sum(ozb, ozl, zb1, zl1, zc, zcl);
                             str(out.z_b_lo, i, uc, zcl);
                         }
                     }
                     str(out.z_b, i, uc, zc);
                 }
             }
-            if (DT) {
-                // edges (c | c+1) of row r and (r | r+1), owned by the output lanes
-                DD<T> dnxt = dcur;
-                if (hasS)
-                    dnxt = dd_make<T, FAST>(hf * oh1 + hf * nxt.h, hf * ohu1 + hf * nxt.hu, hf * ohv1 + hf * nxt.hv,
-                                            hf * oG1 + hf * nxt.G, nxt.flags & 2);
-                const DD<T> deast = dd_shdn(dcur);
-                if (p.out && c + 1 < nx && dd_active(dcur, deast)) vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
-                if (p.out && hasS && dd_active(dcur, dnxt)) vmin = pmin(vmin, dt_edge_d<1, T, FAST>(dcur, dnxt, dx));
-                dcur = dnxt;
+            if (MODE == 1) {
+                if (DT) {
+                    // edges (c | c+1) of row r and (r | r+1), owned by the output lanes
+                    DD<T> dnxt = dcur;
+                    if (hasS) dnxt = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, nxt.flags & 2);
+                    const DD<T> deast = dd_shdn(dcur);
+                    if (p.out && c + 1 < nx && dd_active(dcur, deast))
+                        vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
+                    if (p.out && hasS && dd_active(dcur, dnxt)) vmin = pmin(vmin, dt_edge_d<1, T, FAST>(dcur, dnxt, dx));
+                    dcur = dnxt;
+                }
+                cbc = cbn;
+                ozb = ozb_n;
+                ozl = ozl_n;
             }
             fn = fy;
             cur = nxt;
-            oh0 = oh1; ohu0 = ohu1; ohv0 = ohv1; oG0 = oG1;
-            oh1 = oh2; ohu1 = ohu2; ohv1 = ohv2; oG1 = oG2;
         }
     }
     if (DT) {
@@ -823,6 +911,25 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
 // ---------------------------------------------------------------------------
 // CFL dt partial minima (roe_exner.py:257-310), one partial per block
 // ---------------------------------------------------------------------------
+template <typename T>
+struct DR {          // raw per-cell inputs of the dt sweep
+    T h, hu, hv, G;
+    int m;
+};
+
+template <typename T>
+__device__ __forceinline__ DR<T> dr_load(const F<T> &s, const uint8_t *m0, long ro, unsigned c) {
+    DR<T> q;
+    q.h = ldr(s.h, ro, c); q.hu = ldr(s.hu, ro, c); q.hv = ldr(s.hv, ro, c); q.G = ldr(s.G, ro, c);
+    q.m = ldr(m0, ro, c);
+    return q;
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ DD<T> dd_from(const DR<T> &q) {
+    return dd_make<T, FAST>(q.h, q.hu, q.hv, q.G, q.m);
+}
+
 template <typename T, bool FAST>
 __global__ __launch_bounds__(SW *SWPB) void k_rx_dt_strip(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx,
                                                           T dx, T *__restrict__ partial) {
@@ -831,17 +938,19 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_dt_strip(F<T> s, const uint8_t 
     T v = inf_v<T>();
     if (p.r0 < ny) {
         const unsigned ucc = unsigned(p.cc);
-        const long i0 = long(p.r0) * nx;
-        DD<T> cur = dd_make<T, FAST>(ldr(s.h, i0, ucc), ldr(s.hu, i0, ucc), ldr(s.hv, i0, ucc), ldr(s.G, i0, ucc),
-                                     ldr(m0, i0, ucc));
+        const int rlast = p.r1 < ny - 1 ? p.r1 : ny - 1;
+        auto row = [&](int r) { return long(r <= rlast ? r : rlast) * nx; };
+        DD<T> cur = dd_from<T, FAST>(dr_load(s, m0, row(p.r0), ucc));
+        // rows r+1 and r+2 in flight while row r is evaluated
+        DR<T> q1 = dr_load(s, m0, row(p.r0 + 1), ucc);
+        DR<T> q2 = dr_load(s, m0, row(p.r0 + 2), ucc);
+        wait_vm_all();
         for (int r = p.r0; r < p.r1; ++r) {
             const bool hasS = r + 1 < ny;
             DD<T> nxt = cur;
-            if (hasS) {
-                const long j = long(r + 1) * nx;
-                nxt = dd_make<T, FAST>(ldr(s.h, j, ucc), ldr(s.hu, j, ucc), ldr(s.hv, j, ucc), ldr(s.G, j, ucc),
-                                       ldr(m0, j, ucc));
-            }
+            if (hasS) nxt = dd_from<T, FAST>(q1);
+            q1 = q2;
+            if (r + 3 <= rlast) q2 = dr_load(s, m0, long(r + 3) * nx, ucc);
             const DD<T> east = dd_shdn(cur);
             // lanes 1..62 own the x edge (c | c+1) and the y edge (r | r+1) of their column
             if (p.out && p.c + 1 < nx && dd_active(cur, east)) v = pmin(v, dt_edge_d<0, T, FAST>(cur, east, dx));
