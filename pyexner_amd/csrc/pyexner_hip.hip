@@ -432,11 +432,11 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     const T *dtp = static_cast<const T *>(dt);
-    const dim3 sg = strip_grid(ny, nx), sb(SW * SWPB);
-    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, in, out, m0, dtp, ny, nx, dx, 1);
+    const dim3 hg = strip_grid<SO_WIDE, SR_HYDRO>(ny, nx), bg = strip_grid<SO_WIDE, SR_BED>(ny, nx), sb(SW * SWPB);
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, in, out, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
-    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), sg, sb, 0, st, out, in.z_b, in.z_b_lo, in.G, out, out,
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), bg, sb, 0, st, out, in.z_b, in.z_b_lo, in.G, out, out,
                        m0, dtp, ny, nx, dx, *sd, 1, nullptr);
     if (int e = launch_status()) return e;
     return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
@@ -459,28 +459,30 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     T *dtp = static_cast<T *>(dt);
-    const dim3 sg = strip_grid(ny, nx), sb(SW * SWPB);
-    const int nb = int(sg.x * sg.y);
+    const dim3 hg = strip_grid<SO_WIDE, SR_HYDRO>(ny, nx), bg = strip_grid<SO_WIDE, SR_BED>(ny, nx), sb(SW * SWPB);
+    const dim3 dg = strip_grid<SO_WIDE, SR_BED>(ny, nx);
+    const int nb = int(bg.x * bg.y);  // partial minima of the combine sweep
     T *part = static_cast<T *>(ws);
     if (!dt_ready) {
-        hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), sg, sb, 0, st, a, m0, ny, nx, dx, part);
+        hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), dg, sb, 0, st, a, m0, ny, nx, dx, part);
         if (int e = launch_status()) return e;
-        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
+        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, int(dg.x * dg.y),
+                           static_cast<T>(cfl), 1, dtp);
         if (int e = launch_status()) return e;
     }
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (sd->mpm ? 0u : (1u << PXH_G_G));
-    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, a, b, m0, dtp, ny, nx, dx, 1);
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, a, b, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, HY, st)) return e;
-    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), sg, sb, 0, st, b, a.z_b, a.z_b_lo, a.G, b, b, m0, dtp,
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), bg, sb, 0, st, b, a.z_b, a.z_b_lo, a.G, b, b, m0, dtp,
                        ny, nx, dx, *sd, sd->mpm ? 0 : 1, nullptr);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
-    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), sg, sb, 0, st, b, c, m0, dtp, ny, nx, dx, 1);
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, b, c, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
-    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), sg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny,
+    hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), bg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny,
                        nx, dx, *sd, 1, part);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
@@ -493,8 +495,9 @@ template <typename T, bool FAST>
 static int rx_hydro_sweep_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
                             const void *dt, hipStream_t st) {
     F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
-    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), strip_grid(g->ny, g->nx), dim3(SW * SWPB), 0, st, in, out, m0,
-                       static_cast<const T *>(dt), g->ny, g->nx, static_cast<T>(g->dx), 1);
+    const dim3 hg = strip_grid<SO_WIDE, SR_HYDRO>(g->ny, g->nx);
+    hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, dim3(SW * SWPB), 0, st, in, out, m0, static_cast<const T *>(dt),
+                       g->ny, g->nx, static_cast<T>(g->dx), 1);
     return launch_status();
 }
 
@@ -503,7 +506,7 @@ static int rx_bed_sweep_t(const pxh_grid *g, const pxh_fields *hyd_, const pxh_f
                           const pxh_fields *old_, const uint8_t *m0, const pxh_sediments *sd, const void *dt,
                           void *partial, hipStream_t st) {
     F<T> hyd = cast_fields<T>(hyd_), bed = cast_fields<T>(bed_), out = cast_fields<T>(out_);
-    const dim3 sg = strip_grid(g->ny, g->nx), sb(SW * SWPB);
+    const dim3 sg = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), sb(SW * SWPB);
     const T *dtp = static_cast<const T *>(dt);
     const T dx = static_cast<T>(g->dx);
     if (!old_) {
@@ -564,7 +567,7 @@ template <typename T, bool FAST>
 static int rx_dt_t(const pxh_grid *g, const pxh_fields *s_, const uint8_t *m0, double cfl, void *dt_out, void *ws,
                    hipStream_t st) {
     F<T> s = cast_fields<T>(s_);
-    const dim3 sg = strip_grid(g->ny, g->nx);
+    const dim3 sg = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx);
     const int nb = int(sg.x * sg.y);
     T *part = static_cast<T *>(ws);
     hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), sg, dim3(SW * SWPB), 0, st, s, m0, g->ny, g->nx,
@@ -744,7 +747,7 @@ int pxh_roe_exner_bed_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_
 
 int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void *dt_out, void *stream) {
     if (!grid_ok(g) || !dt_partial || !dt_out) return PXH_E_ARG;
-    const dim3 sg = strip_grid(g->ny, g->nx);
+    const dim3 sg = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx);  // the partials of pxh_roe_exner_bed_sweep
     const int nb = int(sg.x * sg.y);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (g->dtype == PXH_F64)
@@ -758,8 +761,9 @@ int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void 
 
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
-    const dim3 sg = strip_grid(g->ny, g->nx);
-    const int64_t nb = int64_t(sg.x) * sg.y;
+    const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), sw = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx);
+    int64_t nb = int64_t(sa.x) * sa.y;
+    if (int64_t(sw.x) * sw.y > nb) nb = int64_t(sw.x) * sw.y;
     return (nb > DT_MAX_BLOCKS ? nb : DT_MAX_BLOCKS) * 8;
 }
 

@@ -1,48 +1,78 @@
 // Wavefront-strip sweeps for the Roe-Exner stage.
 //
 // A 64-lane wavefront owns a strip of 62 output columns (lanes 1..62; lanes 0
-// and 63 are the west/east halo columns) and walks down SROWS rows keeping
-// the current and next row in registers.  Every edge flux is evaluated once:
-// the x edge (c | c+1) by lane c with lane c+1's values from a cross-lane
-// shuffle (its upwP reaches lane c+1 by the reverse shuffle), the y edge
-// (r | r+1) by the lane that owns the column, carried to the next row in a
-// register.  Row loads are coalesced; there is no integer division.
+// and 63 are the west/east halo columns) and walks down a strip of rows
+// keeping the current and next row in registers, with the two rows after
+// them in flight.  Every edge flux is evaluated once: the x edge (c | c+1) by
+// lane c with lane c+1's values from a cross-lane DPP shift (its upwP reaches
+// lane c+1 by the reverse shift), the y edge (r | r+1) by the lane that owns
+// the column, carried to the next row in a register.  Row loads are
+// coalesced and addressed from a wavefront-uniform row base; there is no
+// integer division.
 //
 // FAST selects the float32 arithmetic: hardware reciprocal, reciprocal square
 // root, square root, log2 and exp2 (about 1 ulp each) with reciprocals shared
-// across an edge.  FAST = false keeps the reference's operation sequence with
-// IEEE division and square root (the double build, and float32 when exact
-// agreement with a float32 restatement is wanted).
+// across an edge, and the CFL cubic in float32.  FAST = false keeps the
+// reference's operation sequence with IEEE division and square root (the
+// double build, and float32 when exact agreement with a float32 restatement is
+// wanted).
 #pragma once
 #include "common.hpp"
 #include "flux_roe_exner.hpp"
 
 namespace pxh {
 
-constexpr int SW = 64;       // lanes per strip
-constexpr int SOUT = 48;     // output columns per strip
-constexpr int SROWS = 32;    // rows per strip
-constexpr int SWPB = 4;      // strips (wavefronts) per block, stacked in y
+constexpr int SW = 64;         // lanes per strip
+constexpr int SO_WIDE = 62;    // output columns per strip (lanes 1..62)
+constexpr int SR_HYDRO = 16;   // rows per strip, hydro sweep
+constexpr int SR_BED = 16;     // rows per strip, bed and dt sweeps
+constexpr int SWPB = 2;        // strips (wavefronts) per block, stacked in y
 
+// Strip geometry: lanes 1..SO own columns SO s .. SO s + SO - 1; lane 0 holds
+// the west neighbour column and lane SO + 1 the east one.  With SO = 62 the
+// row stores of two neighbouring strips split the 64-byte block at their
+// common edge; the XCD-aware block order below keeps the two wavefronts on
+// one L2, where the halves merge (split writes that reach HBM separately cost
+// about 40% of a store-heavy sweep).  The strip heights and two wavefronts
+// per block were chosen by timing the SSPRK2 step on an 8192 x 8192 grid.
 struct StripPos {
     int lane, c, cc, r0, r1;
     bool out;
 };
 
+// XCD-aware block order.  The hardware hands linear block L to XCD L % 8;
+// XCD k takes the logical blocks [k n / 8, (k + 1) n / 8) in order, a band of
+// strip rows, so the strips on either side of a shared column edge run on one
+// XCD (one L2) at nearly the same time, and their partial writes of the
+// 64-byte blocks at the edge merge in that L2 instead of reaching HBM apart.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void xcd_block(int &bx, int &by) {
+    const int gx = gridDim.x, n = gx * gridDim.y;
+    const int L = blockIdx.x + gx * blockIdx.y;
+    const int k = L % NXCD, j = L / NXCD, q = n / NXCD, rm = n % NXCD;
+    const int lb = k * q + (k < rm ? k : rm) + j;
+    bx = lb % gx;
+    by = lb / gx;
+}
+
+template <int SO, int SROWS>
 __device__ __forceinline__ StripPos strip_pos(int ny, int nx) {
     StripPos p;
     p.lane = threadIdx.x & (SW - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SW);  // uniform: rows are scalar
-    p.c = blockIdx.x * SOUT - 1 + p.lane;
+    int bx, by;
+    xcd_block(bx, by);
+    p.c = bx * SO - 1 + p.lane;
     p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
-    p.out = p.lane >= 1 && p.lane <= SOUT && p.c < nx;
-    p.r0 = (blockIdx.y * SWPB + w) * SROWS;
+    p.out = p.lane >= 1 && p.lane <= SO && p.c < nx;
+    p.r0 = (by * SWPB + w) * SROWS;
     p.r1 = p.r0 + SROWS < ny ? p.r0 + SROWS : ny;
     return p;
 }
 
+template <int SO, int SROWS>
 static inline dim3 strip_grid(int ny, int nx) {
-    const int sx = (nx + SOUT - 1) / SOUT;
+    const int sx = (nx + SO - 1) / SO;
     const int sy = (ny + SROWS - 1) / SROWS;
     return dim3(sx, (sy + SWPB - 1) / SWPB);
 }
@@ -381,7 +411,7 @@ template <typename T, bool FAST>
 __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, const uint8_t *__restrict__ m0,
                                                              const T *__restrict__ dtp, int ny, int nx, T dx,
                                                              int momcorr) {
-    const StripPos p = strip_pos(ny, nx);
+    const StripPos p = strip_pos<SO_WIDE, SR_HYDRO>(ny, nx);
     if (p.r0 >= ny) return;  // whole wavefront
     const T T_ = C<T>(DRY_TOL), zero = T(0);
     const T dt = *dtp;
@@ -452,7 +482,7 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
                 // surface; neighbours outside the sweep window (lanes 0 and 63,
                 // the rows above and below) are updated from scratch below
                 const T eta = h1 + cur.zr;
-                const bool inE = p.lane <= SOUT - 1, inW = p.lane >= 2, inN = r > p.r0;
+                const bool inE = p.lane <= SO_WIDE - 1, inW = p.lane >= 2, inN = r > p.r0;
                 bool su = (fE & 2) || (fW & 2);
                 bool sv = nod_n || (nxt.flags & 2);
                 const bool bE = eta < zrE, bW = eta < zrW, bN = eta < zr_n, bS = eta < nxt.zr;
@@ -732,6 +762,51 @@ __device__ __forceinline__ bool dd_active(const DD<T> &a, const DD<T> &b) {
     return ((a.h > T_) || (b.h > T_)) && !(a.nod || b.nod);
 }
 
+// get_lambda in float32 hardware arithmetic (float32 FAST path only).  The
+// coded cubic (a = -1, b = -2 ut, c = (1 + at) ct^2 - ut^2, d = -ct^2 at ut)
+// has the depressed form p = -((1 + at) ct^2 + ut^2 / 3) < 0,
+// q = ut (ct^2 (18 + 45 at) - 2 ut^2) / 27 and shift -b / (3 a) = -2 ut / 3,
+// so with P = -p: x = -(3 sqrt(3) / 2) q / P^(3/2) and the scale
+// 2 sqrt(-p / 3) = (2 / sqrt(3)) sqrt(P).  The largest root of
+// 4 t^3 - 3 t = |x| takes two Newton steps from the quadratic fit (error
+// below 2e-3, quadratic convergence with factor <= 2, so < 1e-10 after two);
+// the other extreme root follows by deflation, as in trisect.  Next to
+// |x| = 1 the root pair is nearly double and its roots go with the square
+// root of the error in x, so there (|x| >= 0.999, and the clipped |x| >= 1)
+// it returns false and the caller evaluates the double form; below, max|lambda|
+// stays within 3e-6 of the double form.
+__device__ __forceinline__ bool get_lambda_f32(float ut, float at, float ct, float &lmin, float &lmax) {
+#pragma clang fp contract(fast)
+    using Mf = Mth<float, true>;
+    const float ct2 = ct * ct;
+    const float P = (1.0f + at) * ct2 + ut * ut * 0.33333333333333333f;
+    const float q = ut * (ct2 * (18.0f + 45.0f * at) - 2.0f * ut * ut) * 0.037037037037037035f;
+    const float rP = Mf::rsq(P);
+    const float x = -2.598076211353316f * q * (rP * rP * rP);
+    const float st = 1.1547005383792515f * P * rP;
+    const float sh = -0.66666666666666667f * ut;
+    const float xa = __builtin_fabsf(x);
+    if (!(xa < 0.999f)) return false;
+    float t = 0.8660254037844386f + xa * (0.16666666666666666f - 0.03269207045111198f * xa);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float t2 = t * t;
+        t = t - ((4.0f * t2 - 3.0f) * t - xa) * Mf::rcp(12.0f * t2 - 3.0f);
+    }
+    const float other = 0.5f * (-t - Mf::sqrt(3.0f * (1.0f - t) * (1.0f + t)));
+    float tmin, tmax;
+    if (x < 0.0f) {
+        tmin = -t;
+        tmax = -other;
+    } else {
+        tmax = t;
+        tmin = other;
+    }
+    lmax = st * tmax + sh;
+    lmin = st * tmin + sh;
+    return true;
+}
+
 template <int AX, typename T, bool FAST>
 __device__ __forceinline__ T dt_edge_d(const DD<T> &I, const DD<T> &J, T dx) {
     const T g = C<T>(GRAV);
@@ -757,6 +832,15 @@ __device__ __forceinline__ T dt_edge_d(const DD<T> &I, const DD<T> &J, T dx) {
     T rs;
     if (at < C<T>(1e-5)) {
         rs = ss;  // linear branch of _get_lambda: (ut - ct, ut + ct)
+    } else if constexpr (FAST) {
+        float lmin, lmax;
+        if (get_lambda_f32(ut, at, ct, lmin, lmax)) {
+            rs = pmax(d_abs(lmin), d_abs(lmax));
+        } else {
+            double dmin, dmax;
+            get_lambda(static_cast<double>(ut), static_cast<double>(at), static_cast<double>(ct), dmin, dmax);
+            rs = pmax(d_abs(static_cast<float>(dmin)), d_abs(static_cast<float>(dmax)));
+        }
     } else {
         double lmin, lmax;
         get_lambda(static_cast<double>(ut), static_cast<double>(at), static_cast<double>(ct), lmin, lmax);
@@ -790,7 +874,7 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
                                                            const T *__restrict__ dtp, int ny, int nx, T dx,
                                                            pxh_sediments sd, int write_G, T *__restrict__ partial) {
     __shared__ T red[SWPB];
-    const StripPos p = strip_pos(ny, nx);
+    const StripPos p = strip_pos<SO_WIDE, SR_BED>(ny, nx);
     const T zero = T(0), hf = T(0.5);
     const int c = p.c;
     T vmin = inf_v<T>();
@@ -934,7 +1018,7 @@ template <typename T, bool FAST>
 __global__ __launch_bounds__(SW *SWPB) void k_rx_dt_strip(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx,
                                                           T dx, T *__restrict__ partial) {
     __shared__ T red[SWPB];
-    const StripPos p = strip_pos(ny, nx);
+    const StripPos p = strip_pos<SO_WIDE, SR_BED>(ny, nx);
     T v = inf_v<T>();
     if (p.r0 < ny) {
         const unsigned ucc = unsigned(p.cc);

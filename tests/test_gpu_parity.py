@@ -5,7 +5,8 @@ Tolerances (written here, checked below):
 * float64 build: relative L-inf per field <= 1e-12 (against max |oracle field|),
   after 1000 SSPRK2 steps on a small channel and after full runs of the other
   cases.
-* float32 build (hardware arithmetic, cubic in float64, compensated bed):
+* float32 build (hardware arithmetic, CFL cubic in float32 with a float64
+  fallback next to its double root, compensated bed):
   h, hu <= 1e-5 relative; hv <= 1e-5 of max|hu| (hv is a small cross-flow);
   z_b <= 1e-4 of max |z_b - z_b(0)|, all after 1000 SSPRK2 steps.
 """
@@ -38,6 +39,31 @@ def test_init_and_dt_f64(cuda):
     dto = prob.dt(so)
     dtg = float(solver.compute_dt_fn(sg, prob.cfl, mask, sc))
     assert abs(dtg - float(dto)) <= 1e-14 * float(dto)
+
+
+@pytest.mark.parametrize("at", [1e-4, 1e-2, 0.3, 1.0])
+def test_dt_f32_fast_cubic_matches_f64(cuda, at):
+    """The float32 path solves the CFL cubic in float32 (float64 next to the
+    double root, |x| >= 0.999).  On uniform states every edge carries the same
+    (ut, at, ct), so the domain dt is the one edge's: float32 dt within 5e-6 of
+    the float64 dt across subcritical, critical and supercritical flow."""
+    mesh, arr, params = cases.periodic_box(nx=24, ny=20)
+    got = {}
+    for dtype in (torch.float64, torch.float32):
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
+        for fr in (0.3, 0.7, 0.95, 1.0, 1.3, 2.0, 3.5):
+            h = 1.0
+            u = fr * np.sqrt(9.81 * h)
+            G = at * h / (3.0 * u * u)  # at = G (3 u^2) / h on a uniform state
+            sg.h.fill_(h)
+            sg.hu.fill_(h * u)
+            sg.hv.fill_(0.0)
+            sg.G.fill_(G)
+            got.setdefault(fr, {})[dtype] = float(solver.compute_dt_fn(sg, 0.5, mask, sc))
+    for fr, d in got.items():
+        e = abs(d[torch.float32] - d[torch.float64]) / d[torch.float64]
+        print(f"at={at:g} Fr={fr:g} dt64={d[torch.float64]:.9e} rel(f32)={e:.2e}", flush=True)
+        assert e <= 5e-6, (at, fr, e)
 
 
 def test_bc_apply_matches_oracle(cuda):
