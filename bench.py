@@ -31,8 +31,9 @@ sys.path.insert(0, ROOT)
 ALG_BYTES = {
     # roe_solve_2D + momentum_corrections: read h hu hv z_b z n_b + mask, write h hu hv
     "hydro_sweep": 6 * 4 + 1 + 3 * 4,
-    # stage-1 compute_G + exner_solve_2D: read h hu hv z_b z_b_lo + mask, write z_b z_b_lo
-    "bed_sweep": 5 * 4 + 1 + 2 * 4,
+    # stage-1 compute_G + exner_solve_2D: read h hu hv z_b z_b_lo + mask, write z_b z_b_lo G
+    # (the public sweep stores G; the fused SSPRK2 step skips that store in MPM mode)
+    "bed_sweep": 5 * 4 + 1 + 3 * 4,
     # stage-2 bed sweep + SSPRK2 combine + next dt: read h hu hv z_b z_b_lo + mask and
     # h hu hv z_b z_b_lo G of u^n, write h hu hv z_b z_b_lo G
     "bed_sweep_combine_dt": 5 * 4 + 1 + 6 * 4 + 6 * 4,
@@ -43,13 +44,32 @@ PER_STEP = {"hydro_sweep": 2, "bed_sweep": 1, "bed_sweep_combine_dt": 1, "comput
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ssprk2_radial_dambreak_8192_fp32.txt")
+
+
+def measured_traffic(n, kernel):
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor
+    that makes the counter read the known bytes of the SSPRK2 combine kernel),
+    or None when the summary does not cover this grid."""
+    if n != 8192 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            w = line.split()
+            if len(w) >= 4 and w[0] == kernel and not line.startswith("#"):
+                return int(float(w[3]))
+    return None
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=500)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--size", type=int, default=8192)
-    p.add_argument("--kernel-reps", type=int, default=10)
+    p.add_argument("--kernel-steps", type=int, default=10,
+                   help="SSPRK2 steps issued sweep by sweep to time each kernel inside the step")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-size", type=int, default=1024)
     p.add_argument("--cpu-steps", type=int, default=30)
@@ -143,40 +163,67 @@ def main():
     t_end = float(st.dt)  # next dt (sanity)
     log(f"next dt {t_end:.4e} s")
 
-    # per-kernel durations: each kernel alone, HIP events on the stream it runs on
+    # Per-kernel durations inside the step: SSPRK2 steps issued sweep by sweep
+    # through the C-ABI (the same launches pxh_roe_exner_ssprk2 makes, in the
+    # same order, on the same buffers), with HIP events recorded on the stream
+    # the kernels run on around each strip-sweep launch.
     be, m0 = sc.backend, st.m0
     stream = torch.cuda.current_stream().cuda_stream
     g = be.grid
     fa, fb, fc = st.fa, st.fb, st.fc
     dtp = st.dt.data_ptr()
     part = be.workspace.data_ptr()
-    calls = {
-        "hydro_sweep": lambda: _hip.call("pxh_roe_exner_hydro_sweep", g, fa, fb, m0, dtp, stream),
-        "bed_sweep": lambda: _hip.call("pxh_roe_exner_bed_sweep", g, fb, fa, fb, None, m0, be.sed, dtp, None,
-                                       stream),
-        "bed_sweep_combine_dt": lambda: _hip.call("pxh_roe_exner_bed_sweep", g, fc, fb, fa, fa, m0, be.sed, dtp,
-                                                  part, stream),
-        "compute_dt": lambda: be.dt_into(fa, m0, st.cfl, dtp, stream),
-    }
+    G_H, G_HUV, G_ZB, G_G = _hip.G_H, _hip.G_HUV, _hip.G_ZB, _hip.G_G
+    bc_hy = sc.boundaries.program((G_H, G_HUV))
+    bc_bed1 = sc.boundaries.program((G_ZB,) if be.sed.mpm else (G_ZB, G_G))
+    bc_end = sc.boundaries.program((G_ZB, G_G))
+    cfl = float(params["cfl"])
+    sz = be.sed
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    spans = {k: [] for k in ALG_BYTES}
+    torch.cuda.synchronize()
+    e0 = ev()
+    be.dt_into(fa, m0, cfl, dtp, stream)
+    spans["compute_dt"].append((e0, ev()))
+    for _ in range(a.kernel_steps):
+        e0 = ev()
+        _hip.call("pxh_roe_exner_hydro_sweep", g, fa, fb, m0, dtp, stream)
+        spans["hydro_sweep"].append((e0, ev()))
+        _hip.call("pxh_bc_apply", g, fb, bc_hy, stream)
+        e0 = ev()
+        _hip.call("pxh_roe_exner_bed_sweep", g, fb, fa, fb, None, m0, sz, dtp, None, stream)
+        spans["bed_sweep"].append((e0, ev()))
+        _hip.call("pxh_bc_apply", g, fb, bc_bed1, stream)
+        e0 = ev()
+        _hip.call("pxh_roe_exner_hydro_sweep", g, fb, fc, m0, dtp, stream)
+        spans["hydro_sweep"].append((e0, ev()))
+        _hip.call("pxh_bc_apply", g, fc, bc_hy, stream)
+        e0 = ev()
+        _hip.call("pxh_roe_exner_bed_sweep", g, fc, fb, fa, fa, m0, sz, dtp, part, stream)
+        spans["bed_sweep_combine_dt"].append((e0, ev()))
+        _hip.call("pxh_bc_apply", g, fa, bc_end, stream)
+        _hip.call("pxh_dt_finalize", g, part, cfl, dtp, stream)
+    # one dispatch of known size for the byte-counter calibration (profiling
+    # runs only read it): the SSPRK2 combine over u^n and u^(1) into u^(2)
+    be.combine_into(fa, fb, fc, stream)
+    torch.cuda.synchronize()
     kern = {}
-    for name, fn in calls.items():
-        fn()
-        torch.cuda.synchronize()
-        k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        k0.record()
-        for _ in range(a.kernel_reps):
-            fn()
-        k1.record()
-        torch.cuda.synchronize()
-        us = k0.elapsed_time(k1) * 1e3 / a.kernel_reps
+    for name, lst in spans.items():
+        us = sum(x.elapsed_time(y) for x, y in lst) * 1e3 / len(lst)
         gbs = ALG_BYTES[name] * n * n / (us * 1e-6) / 1e9
-        kern[name] = {"us": round(us, 2), "alg_bytes_per_cell": ALG_BYTES[name], "alg_GBs": round(gbs, 1),
-                      "per_step": PER_STEP[name]}
-        log(f"  {name:22s} {us:9.1f} us  {gbs:8.1f} GB/s algorithmic")
+        kern[name] = {"us": round(us, 2), "launches": len(lst), "alg_bytes_per_cell": ALG_BYTES[name],
+                      "alg_GBs": round(gbs, 1), "per_step": PER_STEP[name]}
+        log(f"  {name:22s} {us:9.1f} us  {gbs:8.1f} GB/s algorithmic  ({len(lst)} launches in the step sequence)")
     dom = max(kern, key=lambda k: kern[k]["us"] * PER_STEP[k])
     achieved = kern[dom]["alg_GBs"]
+    traffic = measured_traffic(n, dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "alg_bytes_per_cell": ALG_BYTES[dom], "cells_per_launch": n * n}
     step_bytes = sum(ALG_BYTES[k] * c for k, c in PER_STEP.items())
 

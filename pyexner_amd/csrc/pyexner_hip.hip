@@ -297,13 +297,35 @@ __global__ __launch_bounds__(DT_THREADS) void k_roe_dt_partial(F<T> s, long ncel
     if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
+// The final min over the partials (tens of thousands for the strip sweeps):
+// one block of 1024 threads with eight loads in flight per thread.  min is
+// exact, so the order of the reduction does not change the result (a NaN
+// anywhere still wins through pmin).
+constexpr int FIN_THREADS = 1024;
+constexpr int FIN_UNROLL = 8;
+
 template <typename T>
-__global__ __launch_bounds__(DT_THREADS) void k_dt_final(const T *__restrict__ partial, int nblk, T cfl, int clamp,
-                                                         T *__restrict__ out) {
-    T v = inf_v<T>();
-    for (int k = threadIdx.x; k < nblk; k += DT_THREADS) v = pmin(v, partial[k]);
-    v = block_min(v);
+__global__ __launch_bounds__(FIN_THREADS) void k_dt_final(const T *__restrict__ partial, int nblk, T cfl, int clamp,
+                                                          T *__restrict__ out) {
+    __shared__ T red[FIN_THREADS / 64];
+    T acc[FIN_UNROLL];
+#pragma unroll
+    for (int u = 0; u < FIN_UNROLL; ++u) acc[u] = inf_v<T>();
+    int k = threadIdx.x;
+    for (; k + (FIN_UNROLL - 1) * FIN_THREADS < nblk; k += FIN_UNROLL * FIN_THREADS) {
+#pragma unroll
+        for (int u = 0; u < FIN_UNROLL; ++u) acc[u] = pmin(acc[u], partial[k + u * FIN_THREADS]);
+    }
+    for (; k < nblk; k += FIN_THREADS) acc[0] = pmin(acc[0], partial[k]);
+    T v = acc[0];
+#pragma unroll
+    for (int u = 1; u < FIN_UNROLL; ++u) v = pmin(v, acc[u]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int w = 1; w < FIN_THREADS / 64; ++w) v = pmin(v, red[w]);
         T d = cfl * v;
         if (clamp) d = pmax(d, C<T>(1e-6));
         *out = d;
@@ -466,7 +488,7 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     if (!dt_ready) {
         hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), dg, sb, 0, st, a, m0, ny, nx, dx, part);
         if (int e = launch_status()) return e;
-        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, int(dg.x * dg.y),
+        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, int(dg.x * dg.y),
                            static_cast<T>(cfl), 1, dtp);
         if (int e = launch_status()) return e;
     }
@@ -486,7 +508,7 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
                        nx, dx, *sd, 1, part);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
     return launch_status();
 }
 
@@ -573,7 +595,7 @@ static int rx_dt_t(const pxh_grid *g, const pxh_fields *s_, const uint8_t *m0, d
     hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), sg, dim3(SW * SWPB), 0, st, s, m0, g->ny, g->nx,
                        static_cast<T>(g->dx), part);
     if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1,
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1,
                        static_cast<T *>(dt_out));
     return launch_status();
 }
@@ -595,7 +617,7 @@ static int roe_dt(const pxh_grid *g, const pxh_fields *s_, double cfl, void *dt_
     hipLaunchKernelGGL(k_roe_dt_partial<T>, dim3(nb), dim3(DT_THREADS), 0, st, s, long(g->ny) * g->nx,
                        static_cast<T>(g->dx), part);
     if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(DT_THREADS), 0, st, part, nb, static_cast<T>(cfl), 0,
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nb, static_cast<T>(cfl), 0,
                        static_cast<T *>(dt_out));
     return launch_status();
 }
@@ -751,10 +773,10 @@ int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void 
     const int nb = int(sg.x * sg.y);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (g->dtype == PXH_F64)
-        hipLaunchKernelGGL(k_dt_final<double>, dim3(1), dim3(DT_THREADS), 0, st, static_cast<const double *>(dt_partial),
+        hipLaunchKernelGGL(k_dt_final<double>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const double *>(dt_partial),
                            nb, cfl, 1, static_cast<double *>(dt_out));
     else
-        hipLaunchKernelGGL(k_dt_final<float>, dim3(1), dim3(DT_THREADS), 0, st, static_cast<const float *>(dt_partial),
+        hipLaunchKernelGGL(k_dt_final<float>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const float *>(dt_partial),
                            nb, static_cast<float>(cfl), 1, static_cast<float *>(dt_out));
     return launch_status();
 }
