@@ -85,10 +85,11 @@ __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F7
 
 // Whole-wavefront shifts by one lane on the DPP path (no LDS round trip):
 // wave_shl:1 gives lane i the value of lane i+1, wave_shr:1 that of lane i-1;
-// the lane with no source (63, resp. 0) keeps its own value, as __shfl_down /
-// __shfl_up do.
+// the lane with no source (63, resp. 0) reads 0 (bound_ctrl).  Those are the
+// strips' halo lanes, whose shifted-in values no output uses, and the zero
+// fill lets the compiler fold the shift into the instruction that consumes it.
 template <int CTRL>
-__device__ __forceinline__ int dpp_i32(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_i32(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true); }
 template <int CTRL, typename T>
 __device__ __forceinline__ T dpp_shift(T v) {
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lane values");
@@ -376,6 +377,27 @@ __device__ __forceinline__ void rx_edge_d(const HD<T> &I, const HD<T> &J, T dx, 
     else rx_edge_exact<AX>(I, J, dx, P, M);
 }
 
+// The edge's upwP/upwM where the edge is active, zeros elsewhere.  The float32
+// path evaluates the edge in every lane and selects (inactive lanes see finite
+// floored values, and whatever they compute is discarded): no exec-mask
+// branch and no zero-filled register copies around it.
+template <int AX, typename T, bool FAST>
+__device__ __forceinline__ void edge_or_zero(bool active, const HD<T> &I, const HD<T> &J, T dx, T P[3], T M[3]) {
+    const T zero = T(0);
+    if constexpr (FAST) {
+        rx_edge_d<AX, T, FAST>(I, J, dx, P, M);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            P[k] = active ? P[k] : zero;
+            M[k] = active ? M[k] : zero;
+        }
+    } else {
+        P[0] = P[1] = P[2] = zero;
+        M[0] = M[1] = M[2] = zero;
+        if (active) rx_edge_d<AX, T, FAST>(I, J, dx, P, M);
+    }
+}
+
 // The updated depth of one cell, from scratch (roe_solve_2D for that cell);
 // used only when momentum corrections need a neighbour outside the window.
 // Inlined once, in a loop over the pending neighbours, rather than called: a
@@ -443,14 +465,12 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
         const HD<T> nxt = hd_derive<T, FAST>(q1);
         q1 = q2;
         if (r + 3 <= rlast) q2 = hr_load(in, m0, long(r + 3) * nx, ucc);
-        // x edge (c | c+1)
+        // x edge (c | c+1) and y edge (r | r+1)
         const HD<T> east = hd_shdn(cur);
-        T Px[3] = {zero, zero, zero}, Mx[3] = {zero, zero, zero};
-        if (c + 1 < nx && hd_active(cur, east)) rx_edge_d<0, T, FAST>(cur, east, dx, Px, Mx);
+        T Px[3], Mx[3], Py[3], My[3];
+        edge_or_zero<0, T, FAST>(c + 1 < nx && hd_active(cur, east), cur, east, dx, Px, Mx);
         const T pw0 = shup(Px[0]), pw1 = shup(Px[1]), pw2 = shup(Px[2]);
-        // y edge (r | r+1)
-        T Py[3] = {zero, zero, zero}, My[3] = {zero, zero, zero};
-        if (hasS && hd_active(cur, nxt)) rx_edge_d<1, T, FAST>(cur, nxt, dx, Py, My);
+        edge_or_zero<1, T, FAST>(hasS && hd_active(cur, nxt), cur, nxt, dx, Py, My);
         T f0 = zero, f1 = zero, f2 = zero;
         if (c + 1 < nx) { f0 = f0 + Mx[0]; f1 = f1 + Mx[1]; f2 = f2 + Mx[2]; }
         if (c > 0) { f0 = f0 + pw0; f1 = f1 + pw1; f2 = f2 + pw2; }
