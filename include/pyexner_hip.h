@@ -104,12 +104,19 @@ typedef struct {
     double nx, ny;                  /* outward normal as given in the YAML */
 } pxh_bc_handler;
 
+/* pxh_bc.flags: PXH_BC_FRAME promises that every entry's target lies on the
+ * outer ring of cells (row 0 or ny-1, column 0 or nx-1) and its source within
+ * two cells of the grid edge.  The step entries then fuse the hydro and bed
+ * updates of each stage and update the two-cell frame after the fill; without
+ * the flag they run the two sweeps with the fill between them. */
+#define PXH_BC_FRAME 1
+
 typedef struct {
     const pxh_bc_entry *entries;    /* device array, n_entries long */
     int32_t n_entries;
     int32_t n_handlers;
     uint32_t group_mask;            /* which PXH_G_* groups this call writes (bit i = group i) */
-    int32_t reserved;
+    int32_t flags;                  /* PXH_BC_* flags */
     pxh_bc_handler handlers[PXH_MAX_HANDLERS];
 } pxh_bc;
 

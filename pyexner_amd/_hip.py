@@ -16,6 +16,7 @@ MAX_HANDLERS = 16
 MAX_CHAIN = 6
 
 BC_COPY, BC_REFLECT_RX, BC_REFLECT_ROE = 0, 1, 2
+BC_FRAME = 1  # pxh_bc.flags: every target on the outer ring, every source within two cells of the edge
 G_H, G_HUV, G_Z, G_ZB, G_N, G_NB, G_G = range(7)
 GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G_N: ("n",),
                 G_NB: ("n_b",), G_G: ("G",)}
@@ -54,7 +55,7 @@ class BCHandler(ctypes.Structure):
 
 class BC(ctypes.Structure):
     _fields_ = [("entries", ctypes.c_void_p), ("n_entries", ctypes.c_int32), ("n_handlers", ctypes.c_int32),
-                ("group_mask", ctypes.c_uint32), ("reserved", ctypes.c_int32),
+                ("group_mask", ctypes.c_uint32), ("flags", ctypes.c_int32),
                 ("handlers", BCHandler * MAX_HANDLERS)]
 
 

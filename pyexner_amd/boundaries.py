@@ -209,6 +209,23 @@ def compose(handlers, shape):
     return ent, idempotent
 
 
+def frame_local(entries, shape):
+    """True when every entry's target lies on the outer ring of cells and its
+    source within two cells of the grid edge: the ring fill then reads and
+    writes only the two-cell frame, so the bed update of every cell off the
+    frame can run before the fill (the fused stage)."""
+    ny, nx = shape
+    if len(entries) == 0:
+        return True
+    t = entries["target"].astype(np.int64)
+    s = entries["src"].astype(np.int64)
+    tr, tc = t // nx, t % nx
+    sr, sc = s // nx, s % nx
+    ring = (tr == 0) | (tr == ny - 1) | (tc == 0) | (tc == nx - 1)
+    frame = (sr <= 1) | (sr >= ny - 2) | (sc <= 1) | (sc >= nx - 2)
+    return bool(np.all(ring) and np.all(frame))
+
+
 class BoundaryManager:
     """BoundaryManager(params, X, Y) (boundary_registry.py:84-187)."""
 
@@ -234,6 +251,7 @@ class BoundaryManager:
             raise NotImplementedError(f"more than {_hip.MAX_HANDLERS} boundary handlers")
         entries, self.idempotent = compose(self.boundary_handlers, shape)
         self.n_entries = len(entries)
+        self.frame_local = frame_local(entries, shape)
         self.device = device
         self._entries_dev = torch.from_numpy(entries.view(np.uint8).copy()).to(device) if len(entries) else None
         self.groups_present = sorted({int(g) for g in entries["group"]}) if len(entries) else []
@@ -248,6 +266,7 @@ class BoundaryManager:
         for g_ in (range(7) if groups is None else groups):
             mask |= 1 << g_
         bc.group_mask = mask
+        bc.flags = _hip.BC_FRAME if self.frame_local else 0
         for k, hd in enumerate(self.boundary_handlers):
             bc.handlers[k].kind = hd.kind
             bc.handlers[k].nx = float(hd.normal[0])

@@ -12,6 +12,7 @@
 #include "flux_roe.hpp"
 #include "flux_roe_exner.hpp"
 #include "strip.hpp"
+#include "stage.hpp"
 
 namespace pxh {
 
@@ -447,6 +448,27 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
 // One Forward-Euler step (roe_exner_solver.py:80-116) as two strip sweeps and
 // two ring fills: hydro + momentum corrections -> fill (h, hu, hv) ->
 // G + Exner -> fill (z_b, G).
+// Fused stage (stage.hpp) for frame-local boundary programs: fused sweep ->
+// fill (h, hu, hv) -> the frame's bed update.  MODE 0 writes the whole stage
+// result to out; the caller fills (z_b, G) afterwards.
+template <typename T, bool FAST>
+static int rx_stage0_fused(const pxh_grid *g, const F<T> &in, const F<T> &out, const uint8_t *m0,
+                           const pxh_sediments *sd, const pxh_bc *bc, const T *dtp, int write_G, hipStream_t st) {
+    const int ny = g->ny, nx = g->nx;
+    const T dx = static_cast<T>(g->dx);
+    const F<T> none{};
+    launch_stage<T, FAST, 0, false>(stage_grid(ny, nx), dim3(SW * SWPB), st, in, out, none, none, m0, dtp, ny, nx, dx,
+                                    *sd, write_G, static_cast<T *>(nullptr));
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
+    const long nband = frame_cells(ny, nx);
+    hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 0>), dim3(unsigned((nband + 255) / 256)), dim3(256), 0, st, out, in,
+                       out, none, m0, dtp, ny, nx, dx, *sd, write_G, nband);
+    return launch_status();
+}
+
+static bool frame_fused(const pxh_bc *bc) { return bc && (bc->flags & PXH_BC_FRAME); }
+
 template <typename T, bool FAST>
 static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
                      const pxh_sediments *sd, const pxh_bc *bc, const void *dt, hipStream_t st) {
@@ -454,6 +476,10 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     const T *dtp = static_cast<const T *>(dt);
+    if (frame_fused(bc)) {
+        if (int e = rx_stage0_fused<T, FAST>(g, in, out, m0, sd, bc, dtp, 1, st)) return e;
+        return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
+    }
     const dim3 hg = strip_grid<SO_WIDE, SR_HYDRO>(ny, nx), bg = strip_grid<SO_WIDE, SR_BED>(ny, nx), sb(SW * SWPB);
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, in, out, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
@@ -494,6 +520,30 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     }
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (sd->mpm ? 0u : (1u << PXH_G_G));
+    if (frame_fused(bc)) {
+        // stage 1: b = S(a)
+        if (int e = rx_stage0_fused<T, FAST>(g, a, b, m0, sd, bc, dtp, sd->mpm ? 0 : 1, st)) return e;
+        if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
+        // stage 2: c = S(b), a = 0.5 a + 0.5 c, and the dt minima of the new a
+        const dim3 fg = stage_grid(ny, nx);
+        const int nf = int(fg.x * fg.y);
+        const long nband = frame_cells(ny, nx);
+        const int nbb = int((nband + 255) / 256);
+        const F<T> none{};
+        launch_stage<T, FAST, 1, true>(fg, sb, st, b, none, c, a, m0, dtp, ny, nx, dx, *sd, 1, part);
+        if (int e = launch_status()) return e;
+        if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
+        hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 1>), dim3(nbb), dim3(256), 0, st, c, b, none, a, m0, dtp, ny, nx,
+                           dx, *sd, 1, nband);
+        if (int e = launch_status()) return e;
+        hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbb), dim3(256), 0, st, a, m0, ny, nx, dx, nband,
+                           part + nf);
+        if (int e = launch_status()) return e;
+        if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
+        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nf + nbb, static_cast<T>(cfl), 1,
+                           dtp);
+        return launch_status();
+    }
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, a, b, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, HY, st)) return e;
@@ -783,9 +833,10 @@ int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void 
 
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
-    const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), sw = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx);
+    const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx);
     int64_t nb = int64_t(sa.x) * sa.y;
-    if (int64_t(sw.x) * sw.y > nb) nb = int64_t(sw.x) * sw.y;
+    const int64_t nfused = int64_t(fg.x) * fg.y + (frame_cells(g->ny, g->nx) + 255) / 256;
+    if (nfused > nb) nb = nfused;
     return (nb > DT_MAX_BLOCKS ? nb : DT_MAX_BLOCKS) * 8;
 }
 

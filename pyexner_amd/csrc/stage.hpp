@@ -1,0 +1,443 @@
+// Fused Roe-Exner stage: roe_solve_2D + momentum_corrections + compute_G +
+// exner_solve_2D (+ the SSPRK2 combine and the next dt's per-block minima) in
+// one pass over the state, for boundary programs whose ring fill stays in the
+// frame of the grid.
+//
+// A stage of the reference (roe_exner_solver.py:80-116) fills the ghost ring
+// between the hydro update and the bed update.  When every target of the fill
+// lies on the outer ring of cells and every source within two cells of the
+// grid edge (Reflective, Transmissive and TransmissiveBed rings do; Periodic
+// does not), only cells of the two-cell frame read filled values in the bed
+// update.  The fused kernel then updates the bed of every cell off the frame
+// right behind the hydro update, from the updated rows it holds in registers,
+// and the frame follows after the fill in small band kernels (k_rx_band_*).
+//
+// Strip geometry: lanes 2..61 own 60 columns; the hydro update is complete on
+// lanes 1..62 (both of their x edges are in the wavefront), which the bed
+// edges of the owned columns need.  A strip of SR_STAGE owned rows updates the
+// hydro of one row above and one below it.
+#pragma once
+#include "strip.hpp"
+
+namespace pxh {
+
+constexpr int FO = 60;          // owned columns per fused strip
+constexpr int FHW = 2;          // halo lanes on each side
+constexpr int SR_STAGE = 32;    // owned rows per fused strip
+
+struct StagePos {
+    int lane, c, cc, r0, r1;
+    bool out;   // owned column
+    bool hval;  // updated depth complete in this lane
+};
+
+__device__ __forceinline__ StagePos stage_pos(int ny, int nx) {
+    StagePos p;
+    p.lane = threadIdx.x & (SW - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SW);
+    int bx, by;
+    xcd_block(bx, by);
+    p.c = bx * FO - FHW + p.lane;
+    p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
+    p.out = p.lane >= FHW && p.lane < FHW + FO && p.c < nx;
+    p.hval = p.lane >= 1 && p.lane <= SW - 2 && p.c >= 0 && p.c < nx;
+    p.r0 = (by * SWPB + w) * SR_STAGE;
+    p.r1 = p.r0 + SR_STAGE < ny ? p.r0 + SR_STAGE : ny;
+    return p;
+}
+
+static inline dim3 stage_grid(int ny, int nx) {
+    const int sx = (nx + FO - 1) / FO;
+    const int sy = (ny + SR_STAGE - 1) / SR_STAGE;
+    return dim3(sx, (sy + SWPB - 1) / SWPB);
+}
+
+// The two-cell frame: rows [0, top) and [bot, ny), and in the rows between,
+// columns [0, lft) and [rgt, nx).  Cells off the frame are "interior".
+struct Frame {
+    int top, bot, lft, rgt;
+};
+
+__host__ __device__ __forceinline__ Frame frame_of(int ny, int nx) {
+    Frame f;
+    f.top = ny < 2 ? ny : 2;
+    f.bot = ny - 2 > f.top ? ny - 2 : f.top;
+    f.lft = nx < 2 ? nx : 2;
+    f.rgt = nx - 2 > f.lft ? nx - 2 : f.lft;
+    return f;
+}
+
+__host__ __device__ __forceinline__ long frame_cells(int ny, int nx) {
+    const Frame f = frame_of(ny, nx);
+    return long(f.top + (ny - f.bot)) * nx + long(f.bot - f.top) * (f.lft + (nx - f.rgt));
+}
+
+// Cell (r, c) of frame index i (0 <= i < frame_cells).
+__device__ __forceinline__ void frame_cell(long i, int ny, int nx, int &r, int &c) {
+    const Frame f = frame_of(ny, nx);
+    const long ntb = long(f.top + (ny - f.bot)) * nx;
+    if (i < ntb) {
+        const int k = int(i / nx);
+        c = int(i - long(k) * nx);
+        r = k < f.top ? k : f.bot + (k - f.top);
+    } else {
+        i -= ntb;
+        const int per = f.lft + (nx - f.rgt);
+        const int k = int(i / per), j = int(i - long(k) * per);
+        r = f.top + k;
+        c = j < f.lft ? j : f.rgt + (j - f.lft);
+    }
+}
+
+// Raw row of the fused stage: the hydro inputs, and the bed inputs the hydro
+// sweep does not read (compensated-bed low part; G in constant mode).
+template <typename T>
+struct SRow {
+    HR<T> h;
+    T zl, G;
+};
+
+// LO: the compensated-bed low part is present; MPM: G is computed, not read.
+// Both are template parameters so that the row refill has no branches inside
+// it (a branch there makes the compiler copy the fresh mask byte behind a wait
+// for every outstanding load).
+template <typename T, bool LO, bool MPM>
+__device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, const T *G_in, const uint8_t *m0,
+                                           long ro, unsigned c) {
+    SRow<T> q;
+    q.h = hr_load(in, m0, ro, c);
+    q.zl = T(0);
+    if constexpr (LO) q.zl = ldr(zb_lo_in, ro, c);
+    q.G = T(0);
+    if constexpr (!MPM) q.G = ldr(G_in, ro, c);
+    return q;
+}
+
+// MODE 0 (a stage on its own): out.{h,hu,hv} of every owned cell;
+//   out.{z_b, z_b_lo (, G when write_G)} of the interior owned cells.
+// MODE 1 (the second SSPRK2 stage): cb.{h,hu,hv} of the owned cells within
+//   three cells of the grid edge (what the frame's bed update and the ring
+//   fill read); old = 0.5 old + 0.5 result over h, hu, hv, G, z_b, z_b_lo of
+//   the interior owned cells; with DT, the CFL minimum over the edges between
+//   interior cells of the combined state into partial[block].
+// in: the stage input (hydro fields, z, n_b, z_b, z_b_lo, G).
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
+__global__ __launch_bounds__(SW *SWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
+                                                             const uint8_t *__restrict__ m0,
+                                                             const T *__restrict__ dtp, int ny, int nx, T dx,
+                                                             pxh_sediments sd, int write_G,
+                                                             T *__restrict__ partial) {
+    __shared__ T red[SWPB];
+    const StagePos p = stage_pos(ny, nx);
+    const Frame fr = frame_of(ny, nx);
+    const T zero = T(0), hf = T(0.5);
+    const int c = p.c;
+    T vmin = inf_v<T>();
+    if (p.r0 < ny) {
+        const T dt = *dtp;
+        const T dtdx = dt / dx;
+        const unsigned ucc = unsigned(p.cc), uc = unsigned(c);
+        const T *zb_lo_in = in.z_b_lo;
+        const T *G_in = in.G;
+        const bool inE = p.lane <= SW - 3, inW = p.lane >= 2;
+        const bool colI = c >= fr.lft && c < fr.rgt;           // interior column
+        const bool colIE = c >= fr.lft && c + 1 < fr.rgt;      // x edge (c | c+1) between interior cells
+        const bool frame3 = c <= 2 || c >= nx - 3;             // MODE 1: store the stage hydro here
+        // hydro rows [hr0, hr1): the owned rows and one either side
+        const int hr0 = p.r0 > 0 ? p.r0 - 1 : 0;
+        const int hr1 = p.r1 + 1 < ny ? p.r1 + 1 : ny;
+        const int rlast = hr1 < ny - 1 ? hr1 : ny - 1;  // last row any load reads
+        auto row = [&](int r) { return long(r <= rlast ? r : rlast) * nx; };
+        HCarry<T> hc = hcarry_init<T, FAST>(in, m0, hr0, nx, ucc, dx);
+        T cur_zl = zero, cur_G = zero;  // bed inputs of the current hydro row
+        {
+            const SRow<T> q0 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, long(hr0) * nx, ucc);
+            cur_zl = q0.zl;
+            cur_G = q0.G;
+        }
+        SRow<T> q1 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, row(hr0 + 1), ucc);
+        SRow<T> q2 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, row(hr0 + 2), ucc);
+        // MODE 1: previous-step rows k-1 and k in flight at hydro row k
+        OR<T> o1 = {zero, zero, zero, zero, zero, zero}, o2 = o1;
+        if (MODE == 1) {
+            o1 = or_load(old, row(hr0 > 0 ? hr0 - 1 : 0), ucc);
+            o2 = or_load(old, row(hr0), ucc);
+        }
+        ED<T> edp{};         // bed values of hydro row k-1
+        T fn = zero;         // bed flux through the edge (k-2 | k-1)
+        DD<T> dprev{};       // combined dt values of row k-2 (DT)
+        bool have_dprev = false;
+        wait_vm_all();
+        for (int k = hr0; k < hr1; ++k) {
+            // previous-step values of row k-1, consumed before their refill
+            CB<T> cbn = {zero, zero, zero, zero};
+            T ozb = zero, ozl = zero;
+            if (MODE == 1) {
+                cbn = cb_make(o1, edp.h, edp.hu, edp.hv, edp.G);
+                ozb = o1.zb;
+                ozl = o1.zl;
+                o1 = o2;
+            }
+            const HD<T> nxt = hd_derive<T, FAST>(q1.h);
+            const T nxt_zl = q1.zl, nxt_G = q1.G;
+            q1 = q2;
+            if (k + 3 <= rlast) q2 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, long(k + 3) * nx, ucc);
+            if (MODE == 1) o2 = or_load(old, row(k + 1), ucc);  // clamped: unconditional
+            // hydro row k
+            const T zb_k = hc.cur.zb;
+            const int nod_k = hc.cur.flags & 2;
+            T h1, hu1, hv1;
+            hydro_row<T, FAST>(hc, nxt, k, hr0, c, p.hval, inE, inW, ny, nx, dx, dt, dtdx, in, m0, h1, hu1, hv1);
+            const bool own_k = p.out && k >= p.r0 && k < p.r1;
+            if (own_k) {
+                const long ro = long(k) * nx;
+                if (MODE == 0) {
+                    str(out.h, ro, uc, h1);
+                    str(out.hu, ro, uc, hu1);
+                    str(out.hv, ro, uc, hv1);
+                } else if (frame3 || k <= 2 || k >= ny - 3) {
+                    str(cb.h, ro, uc, h1);
+                    str(cb.hu, ro, uc, hu1);
+                    str(cb.hv, ro, uc, hv1);
+                }
+            }
+            ER<T> er;
+            er.h = h1; er.hu = hu1; er.hv = hv1;
+            er.zb = zb_k; er.G = cur_G; er.zl = cur_zl; er.m = nod_k;
+            const ED<T> edk = ed_derive<T, FAST>(er, sd);
+            cur_zl = nxt_zl;
+            cur_G = nxt_G;
+            if (k > hr0) {
+                // bed edge (k-1 | k), and the bed of row k-1 when it is interior and owned
+                T fy = zero;
+                if (ed_active(edp, edk)) fy = ex_edge_d<1, T, FAST>(edp, edk);
+                const int rb = k - 1;
+                const bool rowI = rb >= fr.top && rb < fr.bot && rb >= p.r0 && rb < p.r1;
+                if (rowI) {
+                    const ED<T> east = ed_shdn(edp);
+                    T fx = zero;
+                    if (c + 1 < nx && ed_active(edp, east)) fx = ex_edge_d<0, T, FAST>(edp, east);
+                    const T fw = shup(fx);
+                    T div = zero;
+                    if (c > 0) div = div + (-fw);
+                    if (c + 1 < nx) div = div + fx;
+                    div = div + (-fn);  // rb >= 2
+                    div = div + fy;     // rb + 1 < ny
+                    if (p.out && colI) {
+                        const long i = long(rb) * nx;
+                        const T inc = FAST ? dtdx * div : dt * div / dx;
+                        T zb1 = edp.zb - inc, zl1 = zero;
+                        if constexpr (LO) cb_sub(edp.zb, edp.zl, inc, zb1, zl1);
+                        if (MODE == 0) {
+                            str(out.z_b, i, uc, zb1);
+                            if constexpr (LO) str(out.z_b_lo, i, uc, zl1);
+                            if (write_G) str(out.G, i, uc, edp.G);
+                        } else {
+                            str(old.h, i, uc, cbn.h);
+                            str(old.hu, i, uc, cbn.hu);
+                            str(old.hv, i, uc, cbn.hv);
+                            str(old.G, i, uc, cbn.G);
+                            T zc = hf * ozb + hf * zb1;
+                            if constexpr (LO) {
+                                T zcl;
+                                cb_half_sum(ozb, ozl, zb1, zl1, zc, zcl);
+                                str(old.z_b_lo, i, uc, zcl);
+                            }
+                            str(old.z_b, i, uc, zc);
+                        }
+                    }
+                    if (MODE == 1 && DT) {
+                        // edges between interior cells: (c | c+1) of row rb,
+                        // and (rb-1 | rb) when row rb-1 is interior too
+                        const DD<T> dcur = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, edp.flags & 2);
+                        const DD<T> deast = dd_shdn(dcur);
+                        if (p.out && colIE && dd_active(dcur, deast))
+                            vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
+                        if (p.out && colI && have_dprev && dd_active(dprev, dcur))
+                            vmin = pmin(vmin, dt_edge_d<1, T, FAST>(dprev, dcur, dx));
+                        dprev = dcur;
+                        have_dprev = true;
+                    }
+                }
+                fn = fy;
+            }
+            edp = edk;
+        }
+    }
+    if (MODE == 1 && DT) {
+        vmin = wave_block_min(vmin, red);
+        if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = vmin;
+    }
+}
+
+// Host launcher: picks the (LO, MPM) instance for in's compensated bed and the
+// sediment mode.  In a compensated run every set (in, out, old) has z_b_lo.
+template <typename T, bool FAST, int MODE, bool DT>
+static void launch_stage(dim3 grid, dim3 block, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
+                         const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, T dx,
+                         const pxh_sediments &sd, int write_G, T *partial) {
+    const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
+#define PXH_STAGE(LO_, MPM_)                                                                                   \
+    hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_>), grid, block, 0, st, in, out, cb, old, \
+                       m0, dtp, ny, nx, dx, sd, write_G, partial)
+    if constexpr (sizeof(T) == 4) {
+        if (lo) {
+            if (sd.mpm) PXH_STAGE(true, true);
+            else PXH_STAGE(true, false);
+            return;
+        }
+    }
+    if (sd.mpm) PXH_STAGE(false, true);
+    else PXH_STAGE(false, false);
+#undef PXH_STAGE
+}
+
+// ---------------------------------------------------------------------------
+// The frame, after the ring fill of the stage's hydro fields
+// ---------------------------------------------------------------------------
+// Per-lane-addressed loads for the frame kernels (any cell in any lane).
+template <typename T>
+__device__ __forceinline__ T ldl(const T *p, long i) {
+    return ((const PXH_GLOBAL T *)p)[i];
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ ED<T> ed_load_lane(const F<T> &hyd, const T *zb_in, const T *zb_lo_in, const T *G_in,
+                                              const uint8_t *m0, const pxh_sediments &sd, long i) {
+    ER<T> q;
+    q.h = ldl(hyd.h, i); q.hu = ldl(hyd.hu, i); q.hv = ldl(hyd.hv, i);
+    q.zb = ldl(zb_in, i);
+    q.G = sd.mpm ? T(0) : ldl(G_in, i);
+    q.zl = T(0);
+    if constexpr (sizeof(T) == 4) {
+        if (zb_lo_in) q.zl = ldl(zb_lo_in, i);
+    }
+    q.m = ldl(m0, i);
+    return ed_derive<T, FAST>(q, sd);
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ ED<T> ed_at(const F<T> &hyd, const F<T> &bed, const uint8_t *m0, const pxh_sediments &sd,
+                                       long i) {
+    return ed_load_lane<T, FAST>(hyd, bed.z_b, bed.z_b_lo, bed.G, m0, sd, i);
+}
+
+// compute_G + exner_solve_2D of the frame's cells from the filled hyd and the
+// stage input bed; MODE 0: out.{z_b, z_b_lo (, G)}; MODE 1: old = 0.5 old +
+// 0.5 result over h, hu, hv, G, z_b, z_b_lo.
+template <typename T, bool FAST, int MODE>
+__global__ __launch_bounds__(256) void k_rx_band_bed(F<T> hyd, F<T> bed, F<T> out, F<T> old,
+                                                      const uint8_t *__restrict__ m0, const T *__restrict__ dtp,
+                                                      int ny, int nx, T dx, pxh_sediments sd, int write_G,
+                                                      long nband) {
+    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= nband) return;
+    int r, c;
+    frame_cell(t, ny, nx, r, c);
+    const T zero = T(0), hf = T(0.5);
+    const T dt = *dtp;
+    const long i = long(r) * nx + c;
+    const ED<T> me = ed_at<T, FAST>(hyd, bed, m0, sd, i);
+    const bool hasS = r + 1 < ny;
+    T fx = zero, fw = zero, fn = zero, fy = zero;
+    if (c + 1 < nx) {
+        const ED<T> e = ed_at<T, FAST>(hyd, bed, m0, sd, i + 1);
+        if (ed_active(me, e)) fx = ex_edge_d<0, T, FAST>(me, e);
+    }
+    if (c > 0) {
+        const ED<T> w = ed_at<T, FAST>(hyd, bed, m0, sd, i - 1);
+        if (ed_active(w, me)) fw = ex_edge_d<0, T, FAST>(w, me);
+    }
+    if (r > 0) {
+        const ED<T> n = ed_at<T, FAST>(hyd, bed, m0, sd, i - nx);
+        if (ed_active(n, me)) fn = ex_edge_d<1, T, FAST>(n, me);
+    }
+    if (hasS) {
+        const ED<T> s = ed_at<T, FAST>(hyd, bed, m0, sd, i + nx);
+        if (ed_active(me, s)) fy = ex_edge_d<1, T, FAST>(me, s);
+    }
+    T div = zero;
+    if (c > 0) div = div + (-fw);
+    if (c + 1 < nx) div = div + fx;
+    if (r > 0) div = div + (-fn);
+    if (hasS) div = div + fy;
+    const T inc = FAST ? (dt / dx) * div : dt * div / dx;
+    T zb1 = me.zb - inc, zl1 = zero;
+    if constexpr (sizeof(T) == 4) {
+        if (bed.z_b_lo) cb_sub(me.zb, me.zl, inc, zb1, zl1);
+    }
+    if (MODE == 0) {
+        out.z_b[i] = zb1;
+        if constexpr (sizeof(T) == 4) {
+            if (out.z_b_lo) out.z_b_lo[i] = zl1;
+        }
+        if (write_G) out.G[i] = me.G;
+    } else {
+        const T ozb = old.z_b[i];
+        T ozl = zero;
+        if constexpr (sizeof(T) == 4) {
+            if (old.z_b_lo) ozl = old.z_b_lo[i];
+        }
+        old.h[i] = hf * old.h[i] + hf * me.h;
+        old.hu[i] = hf * old.hu[i] + hf * me.hu;
+        old.hv[i] = hf * old.hv[i] + hf * me.hv;
+        old.G[i] = hf * old.G[i] + hf * me.G;
+        T zc = hf * ozb + hf * zb1;
+        if constexpr (sizeof(T) == 4) {
+            if (old.z_b_lo) {
+                T zcl;
+                cb_half_sum(ozb, ozl, zb1, zl1, zc, zcl);
+                old.z_b_lo[i] = zcl;
+            }
+        }
+        old.z_b[i] = zc;
+    }
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ DD<T> dd_at(const F<T> &s, const uint8_t *m0, long i) {
+    return dd_make<T, FAST>(s.h[i], s.hu[i], s.hv[i], s.G[i], m0[i]);
+}
+
+// CFL minimum over every edge with a frame cell on either side (an edge
+// between two frame cells may be evaluated twice; the minimum does not mind),
+// of the combined state s; one partial per block.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
+                                                     long nband, T *__restrict__ partial) {
+    __shared__ T red[4];
+    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    T v = inf_v<T>();
+    if (t < nband) {
+        int r, c;
+        frame_cell(t, ny, nx, r, c);
+        const long i = long(r) * nx + c;
+        const DD<T> me = dd_at<T, FAST>(s, m0, i);
+        if (c + 1 < nx) {
+            const DD<T> e = dd_at<T, FAST>(s, m0, i + 1);
+            if (dd_active(me, e)) v = pmin(v, dt_edge_d<0, T, FAST>(me, e, dx));
+        }
+        if (c > 0) {
+            const DD<T> w = dd_at<T, FAST>(s, m0, i - 1);
+            if (dd_active(w, me)) v = pmin(v, dt_edge_d<0, T, FAST>(w, me, dx));
+        }
+        if (r + 1 < ny) {
+            const DD<T> so = dd_at<T, FAST>(s, m0, i + nx);
+            if (dd_active(me, so)) v = pmin(v, dt_edge_d<1, T, FAST>(me, so, dx));
+        }
+        if (r > 0) {
+            const DD<T> n = dd_at<T, FAST>(s, m0, i - nx);
+            if (dd_active(n, me)) v = pmin(v, dt_edge_d<1, T, FAST>(n, me, dx));
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) v = pmin(v, red[k]);
+        partial[blockIdx.x] = v;
+    }
+}
+
+}  // namespace pxh

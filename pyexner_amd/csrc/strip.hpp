@@ -139,6 +139,7 @@ struct HD {          // derived per-cell values for the Roe-Exner edge
     T n;             // n_b
     T zt;            // z + floored z_b (bed thrust)
     T zr;            // z + z_b (momentum corrections)
+    T zb;            // raw z_b (the fused stage's bed update)
     int flags;       // bit0 wet (h > DRY_TOL), bit1 nodata
 };
 
@@ -180,6 +181,7 @@ __device__ __forceinline__ HD<T> hd_derive(const HR<T> &q) {
     }
     d.zt = q.z + (q.zb > S_ ? q.zb : C<T>(SED_FLOOR));
     d.zr = q.z + q.zb;
+    d.zb = q.zb;
     d.flags = (d.h > T_ ? 1 : 0) | (q.m ? 2 : 0);
     return d;
 }
@@ -428,6 +430,126 @@ __device__ __forceinline__ T h1_at(const F<T> &in, const uint8_t *m0, T dt, int 
     return me.h - dt * acc / dx;
 }
 
+// Hydro values carried down a strip from one row to the next.
+template <typename T>
+struct HCarry {
+    HD<T> cur;        // derived values of the current row
+    T pn0, pn1, pn2;  // upwP of the edge above the current row
+    T zr_n;           // z + z_b of the row above
+    int nod_n;        // nodata bit of the row above
+    T h1_n;           // updated depth of the row above (once this wavefront updated it)
+};
+
+// The carry at row r0: row r0 derived, and the edge (r0-1 | r0).
+template <typename T, bool FAST>
+__device__ __forceinline__ HCarry<T> hcarry_init(const F<T> &in, const uint8_t *m0, int r0, int nx, unsigned ucc,
+                                                 T dx) {
+    const T zero = T(0);
+    HCarry<T> hc;
+    hc.cur = hd_load<T, FAST>(in, m0, long(r0) * nx, ucc);
+    hc.pn0 = hc.pn1 = hc.pn2 = zero;
+    hc.zr_n = zero;
+    hc.nod_n = 0;
+    hc.h1_n = zero;
+    if (r0 > 0) {
+        const HD<T> up = hd_load<T, FAST>(in, m0, long(r0 - 1) * nx, ucc);
+        if (hd_active(up, hc.cur)) {
+            T P[3], M[3];
+            rx_edge_d<1, T, FAST>(up, hc.cur, dx, P, M);
+            hc.pn0 = P[0]; hc.pn1 = P[1]; hc.pn2 = P[2];
+        }
+        hc.zr_n = up.zr;
+        hc.nod_n = up.flags & 2;
+    }
+    return hc;
+}
+
+// roe_solve_2D (roe_exner.py:535-612) and momentum_corrections (:14-62) of row
+// r, with nxt the derived row r+1; returns the updated h, hu, hv and moves the
+// carry to row r+1.  mc: this lane's cell takes the momentum corrections (its
+// updated depth is complete here); inE / inW: the east / west neighbour's
+// updated depth is held by this wavefront; rfirst: the first row this
+// wavefront updated (the row above it is not held).
+template <typename T, bool FAST>
+__device__ __forceinline__ void hydro_row(HCarry<T> &hc, const HD<T> &nxt, int r, int rfirst, int c, bool mc,
+                                          bool inE, bool inW, int ny, int nx, T dx, T dt, T dtdx, const F<T> &in,
+                                          const uint8_t *m0, T &h1, T &hu1, T &hv1) {
+    const T T_ = C<T>(DRY_TOL), zero = T(0);
+    const HD<T> &cur = hc.cur;
+    const bool hasS = r + 1 < ny;
+    // x edge (c | c+1) and y edge (r | r+1)
+    const HD<T> east = hd_shdn(cur);
+    T Px[3], Mx[3], Py[3], My[3];
+    edge_or_zero<0, T, FAST>(c + 1 < nx && hd_active(cur, east), cur, east, dx, Px, Mx);
+    const T pw0 = shup(Px[0]), pw1 = shup(Px[1]), pw2 = shup(Px[2]);
+    edge_or_zero<1, T, FAST>(hasS && hd_active(cur, nxt), cur, nxt, dx, Py, My);
+    T f0 = zero, f1 = zero, f2 = zero;
+    if (c + 1 < nx) { f0 = f0 + Mx[0]; f1 = f1 + Mx[1]; f2 = f2 + Mx[2]; }
+    if (c > 0) { f0 = f0 + pw0; f1 = f1 + pw1; f2 = f2 + pw2; }
+    if (hasS) { f0 = f0 + My[0]; f1 = f1 + My[1]; f2 = f2 + My[2]; }
+    if (r > 0) { f0 = f0 + hc.pn0; f1 = f1 + hc.pn1; f2 = f2 + hc.pn2; }
+    if constexpr (FAST) {
+        h1 = cur.h - dtdx * f0;
+        hu1 = cur.hu - dtdx * f1;
+        hv1 = cur.hv - dtdx * f2;
+    } else {
+        h1 = cur.h - dt * f0 / dx;
+        hu1 = cur.hu - dt * f1 / dx;
+        hv1 = cur.hv - dt * f2 / dx;
+    }
+    // momentum_corrections, interior cells only
+    const T h1E = shdn(h1), h1W = shup(h1);
+    const T zrE = east.zr, zrW = shup(cur.zr);
+    const int fE = east.flags, fW = shup(cur.flags);
+    if (mc && r >= 1 && r <= ny - 2 && c >= 1 && c <= nx - 2) {
+        // hu/hv survive only in active cells; the blocking tests (which may
+        // need a neighbour's updated depth) run only there
+        const bool active = (h1 > T_) && !(cur.flags & 2);
+        if (!active) {
+            hu1 = zero;
+            hv1 = zero;
+        } else {
+            // blocking by a dry neighbour whose bed is above this water
+            // surface; neighbours whose updated depth this wavefront does not
+            // hold (outside the lanes, above the first row, the row below)
+            // are updated from scratch
+            const T eta = h1 + cur.zr;
+            const bool inN = r > rfirst;
+            bool su = (fE & 2) || (fW & 2);
+            bool sv = hc.nod_n || (nxt.flags & 2);
+            const bool bE = eta < zrE, bW = eta < zrW, bN = eta < hc.zr_n, bS = eta < nxt.zr;
+            if (bE && inE && h1E < T_) su = true;
+            if (bW && inW && h1W < T_) su = true;
+            if (bN && inN && hc.h1_n < T_) sv = true;
+            int far = (bE && !inE ? 1 : 0) | (bW && !inW ? 2 : 0) | (bN && !inN ? 4 : 0) | (bS ? 8 : 0);
+            if (su) far &= ~3;
+            if (sv) far &= ~12;
+            while (far) {
+                const int k = __builtin_ctz(far);
+                far &= far - 1;
+                const int rr = r + (k == 2 ? -1 : (k == 3 ? 1 : 0));
+                const int cc = c + (k == 0 ? 1 : (k == 1 ? -1 : 0));
+                if (h1_at<T, FAST>(in, m0, dt, ny, nx, dx, rr, cc) < T_) {
+                    if (k < 2) {
+                        su = true;
+                        far &= ~3;
+                    } else {
+                        sv = true;
+                        far &= ~12;
+                    }
+                }
+            }
+            if (su) hu1 = zero;
+            if (sv) hv1 = zero;
+        }
+    }
+    hc.pn0 = Py[0]; hc.pn1 = Py[1]; hc.pn2 = Py[2];
+    hc.zr_n = cur.zr;
+    hc.nod_n = cur.flags & 2;
+    hc.h1_n = h1;
+    hc.cur = nxt;
+}
+
 // roe_solve_2D + momentum_corrections for the rows of one strip.
 template <typename T, bool FAST>
 __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, const uint8_t *__restrict__ m0,
@@ -435,113 +557,29 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
                                                              int momcorr) {
     const StripPos p = strip_pos<SO_WIDE, SR_HYDRO>(ny, nx);
     if (p.r0 >= ny) return;  // whole wavefront
-    const T T_ = C<T>(DRY_TOL), zero = T(0);
     const T dt = *dtp;
     const T dtdx = dt / dx;
     const int c = p.c;
     const unsigned ucc = unsigned(p.cc);
-    HD<T> cur = hd_load<T, FAST>(in, m0, long(p.r0) * nx, ucc);
-    T pn0 = zero, pn1 = zero, pn2 = zero;  // upwP of the edge (r-1 | r)
-    T zr_n = zero;                          // z + z_b of row r-1
-    int nod_n = 0;
-    if (p.r0 > 0) {
-        const HD<T> up = hd_load<T, FAST>(in, m0, long(p.r0 - 1) * nx, ucc);
-        if (hd_active(up, cur)) {
-            T P[3], M[3];
-            rx_edge_d<1, T, FAST>(up, cur, dx, P, M);
-            pn0 = P[0]; pn1 = P[1]; pn2 = P[2];
-        }
-        zr_n = up.zr;
-        nod_n = up.flags & 2;
-    }
-    T h1_n = zero;  // updated depth of row r-1 (valid from the second row on)
+    const bool mc = momcorr && p.out, inE = p.lane <= SO_WIDE - 1, inW = p.lane >= 2;
+    HCarry<T> hc = hcarry_init<T, FAST>(in, m0, p.r0, nx, ucc, dx);
     // rows r+1 and r+2 are in flight while row r is computed
     const int rlast = p.r1 < ny - 1 ? p.r1 : ny - 1;  // last row any edge of the strip reads
     HR<T> q1 = hr_load(in, m0, long(p.r0 + 1 <= rlast ? p.r0 + 1 : rlast) * nx, ucc);
     HR<T> q2 = hr_load(in, m0, long(p.r0 + 2 <= rlast ? p.r0 + 2 : rlast) * nx, ucc);
     wait_vm_all();
     for (int r = p.r0; r < p.r1; ++r) {
-        const bool hasS = r + 1 < ny;
         const HD<T> nxt = hd_derive<T, FAST>(q1);
         q1 = q2;
         if (r + 3 <= rlast) q2 = hr_load(in, m0, long(r + 3) * nx, ucc);
-        // x edge (c | c+1) and y edge (r | r+1)
-        const HD<T> east = hd_shdn(cur);
-        T Px[3], Mx[3], Py[3], My[3];
-        edge_or_zero<0, T, FAST>(c + 1 < nx && hd_active(cur, east), cur, east, dx, Px, Mx);
-        const T pw0 = shup(Px[0]), pw1 = shup(Px[1]), pw2 = shup(Px[2]);
-        edge_or_zero<1, T, FAST>(hasS && hd_active(cur, nxt), cur, nxt, dx, Py, My);
-        T f0 = zero, f1 = zero, f2 = zero;
-        if (c + 1 < nx) { f0 = f0 + Mx[0]; f1 = f1 + Mx[1]; f2 = f2 + Mx[2]; }
-        if (c > 0) { f0 = f0 + pw0; f1 = f1 + pw1; f2 = f2 + pw2; }
-        if (hasS) { f0 = f0 + My[0]; f1 = f1 + My[1]; f2 = f2 + My[2]; }
-        if (r > 0) { f0 = f0 + pn0; f1 = f1 + pn1; f2 = f2 + pn2; }
         T h1, hu1, hv1;
-        if constexpr (FAST) {
-            h1 = cur.h - dtdx * f0;
-            hu1 = cur.hu - dtdx * f1;
-            hv1 = cur.hv - dtdx * f2;
-        } else {
-            h1 = cur.h - dt * f0 / dx;
-            hu1 = cur.hu - dt * f1 / dx;
-            hv1 = cur.hv - dt * f2 / dx;
-        }
-        // momentum_corrections (roe_exner.py:14-62), interior cells only
-        const T h1E = shdn(h1), h1W = shup(h1);
-        const T zrE = shdn(cur.zr), zrW = shup(cur.zr);
-        const int fE = shdn(cur.flags), fW = shup(cur.flags);
-        if (momcorr && p.out && r >= 1 && r <= ny - 2 && c >= 1 && c <= nx - 2) {
-            // hu/hv survive only in active cells; the blocking tests (which may
-            // need a neighbour's updated depth) run only there
-            const bool active = (h1 > T_) && !(cur.flags & 2);
-            if (!active) {
-                hu1 = zero;
-                hv1 = zero;
-            } else {
-                // blocking by a dry neighbour whose bed is above this water
-                // surface; neighbours outside the sweep window (lanes 0 and 63,
-                // the rows above and below) are updated from scratch below
-                const T eta = h1 + cur.zr;
-                const bool inE = p.lane <= SO_WIDE - 1, inW = p.lane >= 2, inN = r > p.r0;
-                bool su = (fE & 2) || (fW & 2);
-                bool sv = nod_n || (nxt.flags & 2);
-                const bool bE = eta < zrE, bW = eta < zrW, bN = eta < zr_n, bS = eta < nxt.zr;
-                if (bE && inE && h1E < T_) su = true;
-                if (bW && inW && h1W < T_) su = true;
-                if (bN && inN && h1_n < T_) sv = true;
-                int far = (bE && !inE ? 1 : 0) | (bW && !inW ? 2 : 0) | (bN && !inN ? 4 : 0) | (bS ? 8 : 0);
-                if (su) far &= ~3;
-                if (sv) far &= ~12;
-                while (far) {
-                    const int k = __builtin_ctz(far);
-                    far &= far - 1;
-                    const int rr = r + (k == 2 ? -1 : (k == 3 ? 1 : 0));
-                    const int cc = c + (k == 0 ? 1 : (k == 1 ? -1 : 0));
-                    if (h1_at<T, FAST>(in, m0, dt, ny, nx, dx, rr, cc) < T_) {
-                        if (k < 2) {
-                            su = true;
-                            far &= ~3;
-                        } else {
-                            sv = true;
-                            far &= ~12;
-                        }
-                    }
-                }
-                if (su) hu1 = zero;
-                if (sv) hv1 = zero;
-            }
-        }
+        hydro_row<T, FAST>(hc, nxt, r, p.r0, c, mc, inE, inW, ny, nx, dx, dt, dtdx, in, m0, h1, hu1, hv1);
         if (p.out) {
             const long ro = long(r) * nx;
             str(out.h, ro, unsigned(c), h1);
             str(out.hu, ro, unsigned(c), hu1);
             str(out.hv, ro, unsigned(c), hv1);
         }
-        pn0 = Py[0]; pn1 = Py[1]; pn2 = Py[2];
-        zr_n = cur.zr;
-        nod_n = cur.flags & 2;
-        h1_n = h1;
-        cur = nxt;
     }
 }
 
