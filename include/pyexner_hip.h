@@ -110,6 +110,12 @@ typedef struct {
  * updates of each stage and update the two-cell frame after the fill; without
  * the flag they run the two sweeps with the fill between them. */
 #define PXH_BC_FRAME 1
+/* pxh_bc.flags: PXH_BC_GROUPED promises that the entries are ordered by group
+ * and that group g occupies entries [group_start[g], group_start[g+1]); a call
+ * then visits only the entries of the groups in group_mask.  Without it every
+ * call visits every entry. */
+#define PXH_BC_GROUPED 2
+#define PXH_N_GROUPS 7
 
 typedef struct {
     const pxh_bc_entry *entries;    /* device array, n_entries long */
@@ -117,6 +123,7 @@ typedef struct {
     int32_t n_handlers;
     uint32_t group_mask;            /* which PXH_G_* groups this call writes (bit i = group i) */
     int32_t flags;                  /* PXH_BC_* flags */
+    int32_t group_start[PXH_N_GROUPS + 1];  /* with PXH_BC_GROUPED; group_start[7] = n_entries */
     pxh_bc_handler handlers[PXH_MAX_HANDLERS];
 } pxh_bc;
 
@@ -181,6 +188,39 @@ int pxh_roe_exner_bed_sweep(const pxh_grid *grid, const pxh_fields *hyd, const p
                             const pxh_fields *out, const pxh_fields *old, const uint8_t *mask0,
                             const pxh_sediments *sed, const void *dt_dev, void *dt_partial, void *stream);
 int pxh_dt_finalize(const pxh_grid *grid, const void *dt_partial, double cfl, void *dt_out, void *stream);
+
+/* The fused stage that pxh_roe_exner_step and pxh_roe_exner_ssprk2 run when
+ * bc->flags has PXH_BC_FRAME, for callers that sequence the step themselves.
+ * Valid only when the caller's ring fill is frame-local (PXH_BC_FRAME): a
+ * stage (roe_exner_solver.py:80-116) is then
+ *   stage sweep (one launch) -> fill (h, hu, hv) of the stage result -> frame sweep
+ * and the caller fills (z_b, G) after the frame sweep.  The "frame" is the
+ * cells within two cells of the grid edge.
+ *   stage sweep, old == NULL: roe_solve_2D + momentum_corrections of in into
+ *     out.{h,hu,hv} (every cell), compute_G + exner_solve_2D into
+ *     out.{z_b,z_b_lo,G} of the cells off the frame.
+ *   stage sweep, old != NULL (second SSPRK2 stage, integrators/SSPRK2.py:38-40):
+ *     out.{h,hu,hv} of the cells within three cells of the edge (what the fill
+ *     and the frame sweep read), old = 0.5 old + 0.5 S(in) over h, hu, hv,
+ *     z_b, z_b_lo, G off the frame; with dt_partial, the per-block CFL minima of
+ *     that combined state.
+ *   frame sweep: compute_G + exner_solve_2D of the frame's cells from the
+ *     filled hyd.{h,hu,hv} and bed.{z_b,z_b_lo,G} (the stage input):
+ *     old == NULL: into out.{z_b,z_b_lo,G}; old != NULL: combined into old as
+ *     above, and with dt_partial (the one the stage sweep filled) the frame's
+ *     minima and the final reduction: dt_dev = compute_dt_fn(old), so it holds
+ *     the stage's dt on entry and the next step's on return.
+ * flags: PXH_SWEEP_SKIP_G leaves out.G unwritten in MPM mode (the next stage
+ * recomputes G; the first SSPRK2 stage uses it).  dt_partial holds
+ * pxh_dt_workspace_size(grid) bytes. */
+#define PXH_SWEEP_SKIP_G 1
+int pxh_roe_exner_stage_sweep(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
+                              const pxh_fields *old, const uint8_t *mask0, const pxh_sediments *sed,
+                              const void *dt_dev, int flags, void *dt_partial, void *stream);
+int pxh_roe_exner_frame_sweep(const pxh_grid *grid, const pxh_fields *hyd, const pxh_fields *bed,
+                              const pxh_fields *out, const pxh_fields *old, const uint8_t *mask0,
+                              const pxh_sediments *sed, void *dt_dev, int flags, double cfl, void *dt_partial,
+                              void *stream);
 
 /* compute_dt_fn: *dt_out = max(cfl * min over active edges, 1e-6) written to
  * device memory.  workspace must hold pxh_dt_workspace_size(grid) bytes. */

@@ -17,6 +17,9 @@ MAX_CHAIN = 6
 
 BC_COPY, BC_REFLECT_RX, BC_REFLECT_ROE = 0, 1, 2
 BC_FRAME = 1  # pxh_bc.flags: every target on the outer ring, every source within two cells of the edge
+BC_GROUPED = 2  # pxh_bc.flags: entries ordered by group, group_start[] valid
+N_GROUPS = 7
+SWEEP_SKIP_G = 1  # flags of the stage/frame sweeps: leave out.G unwritten in MPM mode
 G_H, G_HUV, G_Z, G_ZB, G_N, G_NB, G_G = range(7)
 GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G_N: ("n",),
                 G_NB: ("n_b",), G_G: ("G",)}
@@ -56,6 +59,7 @@ class BCHandler(ctypes.Structure):
 class BC(ctypes.Structure):
     _fields_ = [("entries", ctypes.c_void_p), ("n_entries", ctypes.c_int32), ("n_handlers", ctypes.c_int32),
                 ("group_mask", ctypes.c_uint32), ("flags", ctypes.c_int32),
+                ("group_start", ctypes.c_int32 * (N_GROUPS + 1)),
                 ("handlers", BCHandler * MAX_HANDLERS)]
 
 
@@ -86,6 +90,12 @@ _SIGS = {
                                                ctypes.c_void_p]),
     "pxh_dt_finalize": (ctypes.c_int, [_P(Grid), ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
                                        ctypes.c_void_p]),
+    "pxh_roe_exner_stage_sweep": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
+                                                 _P(Sediments), ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
+    "pxh_roe_exner_frame_sweep": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), _P(Fields),
+                                                 ctypes.c_void_p, _P(Sediments), ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_roe_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
     "pxh_momentum_corrections": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),

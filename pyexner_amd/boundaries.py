@@ -255,6 +255,9 @@ class BoundaryManager:
         self.device = device
         self._entries_dev = torch.from_numpy(entries.view(np.uint8).copy()).to(device) if len(entries) else None
         self.groups_present = sorted({int(g) for g in entries["group"]}) if len(entries) else []
+        grp = entries["group"].astype(np.int64)
+        assert np.all(np.diff(grp) >= 0), "compose() emits the entries group by group"
+        self.group_start = [int(v) for v in np.searchsorted(grp, np.arange(_hip.N_GROUPS + 1))]
 
     def program(self, groups=None):
         """pxh_bc struct restricted to the given groups (default: all)."""
@@ -266,7 +269,9 @@ class BoundaryManager:
         for g_ in (range(7) if groups is None else groups):
             mask |= 1 << g_
         bc.group_mask = mask
-        bc.flags = _hip.BC_FRAME if self.frame_local else 0
+        bc.flags = (_hip.BC_FRAME if self.frame_local else 0) | _hip.BC_GROUPED
+        for g_ in range(_hip.N_GROUPS + 1):
+            bc.group_start[g_] = self.group_start[g_]
         for k, hd in enumerate(self.boundary_handlers):
             bc.handlers[k].kind = hd.kind
             bc.handlers[k].nx = float(hd.normal[0])

@@ -394,11 +394,21 @@ __device__ __forceinline__ void bc_transform(const pxh_bc_handler &hd, T &hu, T 
     }
 }
 
+// The entries one ring fill visits: range r is entries [start[r], start[r] +
+// off[r+1] - off[r]), taken by threads [off[r], off[r+1]).
+struct BcSel {
+    int n;
+    int start[PXH_N_GROUPS];
+    int off[PXH_N_GROUPS + 1];
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restrict__ ent, int n, pxh_bc bc) {
+__global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restrict__ ent, BcSel sel, pxh_bc bc) {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const pxh_bc_entry e = ent[k];
+    if (k >= sel.off[sel.n]) return;
+    int r = 0;
+    while (r + 1 < sel.n && k >= sel.off[r + 1]) ++r;
+    const pxh_bc_entry e = ent[sel.start[r] + (k - sel.off[r])];
     if (!((bc.group_mask >> e.group) & 1u)) return;
     const int t = e.target, src = e.src;
     switch (e.group) {
@@ -429,6 +439,20 @@ static inline bool grid_ok(const pxh_grid *g) {
            (long(g->ny) * g->nx) < (1L << 31);
 }
 
+// A boundary program (or none) the ring-fill launch can run: with
+// PXH_BC_GROUPED, group_start must be non-decreasing from 0 to n_entries.
+static inline bool bc_ok(const pxh_bc *bc) {
+    if (!bc) return true;
+    if (bc->n_handlers < 0 || bc->n_handlers > PXH_MAX_HANDLERS || bc->n_entries < 0) return false;
+    if (bc->n_entries > 0 && !bc->entries) return false;
+    if (bc->flags & PXH_BC_GROUPED) {
+        if (bc->group_start[0] != 0 || bc->group_start[PXH_N_GROUPS] != bc->n_entries) return false;
+        for (int q = 0; q < PXH_N_GROUPS; ++q)
+            if (bc->group_start[q + 1] < bc->group_start[q]) return false;
+    }
+    return true;
+}
+
 static inline int dt_blocks(const pxh_grid *g) {
     long n = (long(g->ny) * g->nx + DT_THREADS - 1) / DT_THREADS;
     return int(n < DT_MAX_BLOCKS ? n : DT_MAX_BLOCKS);
@@ -440,31 +464,91 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
     pxh_bc b = *bc;
     b.group_mask &= groups;
     if (!b.group_mask) return PXH_OK;
-    const int n = bc->n_entries;
-    hipLaunchKernelGGL(k_bc<T>, dim3((n + 255) / 256), dim3(256), 0, st, s, bc->entries, n, b);
+    BcSel sel{};
+    if (bc->flags & PXH_BC_GROUPED) {
+        for (int q = 0; q < PXH_N_GROUPS; ++q) {
+            const int cnt = bc->group_start[q + 1] - bc->group_start[q];
+            if (!((b.group_mask >> q) & 1u) || cnt <= 0) continue;
+            sel.start[sel.n] = bc->group_start[q];
+            sel.off[sel.n + 1] = sel.off[sel.n] + cnt;
+            ++sel.n;
+        }
+        if (sel.n == 0) return PXH_OK;
+    } else {
+        sel.n = 1;
+        sel.off[1] = bc->n_entries;
+    }
+    const int n = sel.off[sel.n];
+    hipLaunchKernelGGL(k_bc<T>, dim3((n + 255) / 256), dim3(256), 0, st, s, bc->entries, sel, b);
+    return launch_status();
+}
+
+// The fused stage (stage.hpp) for frame-local boundary programs, as two
+// launch groups the caller separates with the (h, hu, hv) ring fill of the
+// stage's hydro result:
+//   stage sweep, old == none: out.{h,hu,hv} of every cell, out.{z_b, z_b_lo
+//     (, G when write_G)} of the cells off the two-cell frame;
+//   stage sweep, old set: out.{h,hu,hv} of the cells within three cells of the
+//     edge, old = 0.5 old + 0.5 S(in) off the frame, and with partial the
+//     per-block dt minima of that combined state in partial[0, nf);
+//   frame sweep: the frame's bed update from the filled hyd and in's bed
+//     (MODE 0 into out, MODE 1 combined into old), and with partial the frame's
+//     dt minima in partial[nf, nf + nbb) followed by the final reduction of
+//     all nf + nbb minima into *dtp.
+template <typename T, bool FAST>
+static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, const F<T> *old, const uint8_t *m0,
+                         const pxh_sediments *sd, const T *dtp, int write_G, T *part, hipStream_t st) {
+    const int ny = g->ny, nx = g->nx;
+    const T dx = static_cast<T>(g->dx);
+    const F<T> none{};
+    const dim3 fg = stage_grid(ny, nx);
+    if (!old)
+        launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, dx, *sd, write_G,
+                                        static_cast<T *>(nullptr));
+    else if (part)
+        launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, *sd, 1, part);
+    else
+        launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, *sd, 1,
+                                        static_cast<T *>(nullptr));
+    return launch_status();
+}
+
+template <typename T, bool FAST>
+static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, const F<T> &out, const F<T> *old,
+                         const uint8_t *m0, const pxh_sediments *sd, T *dtp, int write_G, double cfl, T *part,
+                         hipStream_t st) {
+    const int ny = g->ny, nx = g->nx;
+    const T dx = static_cast<T>(g->dx);
+    const F<T> none{};
+    const long nband = frame_cells(ny, nx);
+    const int nbb = int((nband + 255) / 256);  // >= 1: a non-empty grid has a frame
+    if (!old) {
+        hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 0>), dim3(nbb), dim3(256), 0, st, hyd, bed, out, none, m0, dtp, ny,
+                           nx, dx, *sd, write_G, nband);
+        return launch_status();
+    }
+    hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 1>), dim3(nbb), dim3(256), 0, st, hyd, bed, none, *old, m0, dtp, ny, nx,
+                       dx, *sd, 1, nband);
+    if (int e = launch_status()) return e;
+    if (!part) return PXH_OK;
+    const dim3 fg = stage_grid(ny, nx);
+    const int nf = int(fg.x * fg.y);
+    hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbb), dim3(256), 0, st, *old, m0, ny, nx, dx, nband, part + nf);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nf + nbb, static_cast<T>(cfl), 1, dtp);
     return launch_status();
 }
 
 // One Forward-Euler step (roe_exner_solver.py:80-116) as two strip sweeps and
 // two ring fills: hydro + momentum corrections -> fill (h, hu, hv) ->
-// G + Exner -> fill (z_b, G).
-// Fused stage (stage.hpp) for frame-local boundary programs: fused sweep ->
-// fill (h, hu, hv) -> the frame's bed update.  MODE 0 writes the whole stage
-// result to out; the caller fills (z_b, G) afterwards.
+// G + Exner -> fill (z_b, G).  With a frame-local boundary program: stage
+// sweep -> fill (h, hu, hv) -> frame sweep; the caller fills (z_b, G) after.
 template <typename T, bool FAST>
 static int rx_stage0_fused(const pxh_grid *g, const F<T> &in, const F<T> &out, const uint8_t *m0,
                            const pxh_sediments *sd, const pxh_bc *bc, const T *dtp, int write_G, hipStream_t st) {
-    const int ny = g->ny, nx = g->nx;
-    const T dx = static_cast<T>(g->dx);
-    const F<T> none{};
-    launch_stage<T, FAST, 0, false>(stage_grid(ny, nx), dim3(SW * SWPB), st, in, out, none, none, m0, dtp, ny, nx, dx,
-                                    *sd, write_G, static_cast<T *>(nullptr));
-    if (int e = launch_status()) return e;
+    if (int e = stage_sweep_f<T, FAST>(g, in, out, nullptr, m0, sd, dtp, write_G, nullptr, st)) return e;
     if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
-    const long nband = frame_cells(ny, nx);
-    hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 0>), dim3(unsigned((nband + 255) / 256)), dim3(256), 0, st, out, in,
-                       out, none, m0, dtp, ny, nx, dx, *sd, write_G, nband);
-    return launch_status();
+    return frame_sweep_f<T, FAST>(g, out, in, out, nullptr, m0, sd, const_cast<T *>(dtp), write_G, 0.0, nullptr, st);
 }
 
 static bool frame_fused(const pxh_bc *bc) { return bc && (bc->flags & PXH_BC_FRAME); }
@@ -524,25 +608,11 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
         // stage 1: b = S(a)
         if (int e = rx_stage0_fused<T, FAST>(g, a, b, m0, sd, bc, dtp, sd->mpm ? 0 : 1, st)) return e;
         if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
-        // stage 2: c = S(b), a = 0.5 a + 0.5 c, and the dt minima of the new a
-        const dim3 fg = stage_grid(ny, nx);
-        const int nf = int(fg.x * fg.y);
-        const long nband = frame_cells(ny, nx);
-        const int nbb = int((nband + 255) / 256);
-        const F<T> none{};
-        launch_stage<T, FAST, 1, true>(fg, sb, st, b, none, c, a, m0, dtp, ny, nx, dx, *sd, 1, part);
-        if (int e = launch_status()) return e;
+        // stage 2: c = S(b), a = 0.5 a + 0.5 c, and the dt of the new a
+        if (int e = stage_sweep_f<T, FAST>(g, b, c, &a, m0, sd, dtp, 1, part, st)) return e;
         if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
-        hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 1>), dim3(nbb), dim3(256), 0, st, c, b, none, a, m0, dtp, ny, nx,
-                           dx, *sd, 1, nband);
-        if (int e = launch_status()) return e;
-        hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbb), dim3(256), 0, st, a, m0, ny, nx, dx, nband,
-                           part + nf);
-        if (int e = launch_status()) return e;
-        if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
-        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nf + nbb, static_cast<T>(cfl), 1,
-                           dtp);
-        return launch_status();
+        if (int e = frame_sweep_f<T, FAST>(g, c, b, c, &a, m0, sd, dtp, 1, cfl, part, st)) return e;
+        return bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
     }
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, a, b, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
@@ -613,6 +683,39 @@ static int rx_bed_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_fiel
         if (!(g->flags & PXH_EXACT_F32)) return rx_bed_sweep_t<T, true>(g, hyd, bed, out, old, m0, sd, dt, partial, st);
     }
     return rx_bed_sweep_t<T, false>(g, hyd, bed, out, old, m0, sd, dt, partial, st);
+}
+
+template <typename T>
+static int rx_stage_sweep(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const pxh_fields *old_,
+                          const uint8_t *m0, const pxh_sediments *sd, const void *dt, int write_G, void *partial,
+                          hipStream_t st) {
+    const F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
+    F<T> old{};
+    if (old_) old = cast_fields<T>(old_);
+    const T *dtp = static_cast<const T *>(dt);
+    T *part = static_cast<T *>(partial);
+    if constexpr (sizeof(T) == 4) {
+        if (!(g->flags & PXH_EXACT_F32))
+            return stage_sweep_f<T, true>(g, in, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, part, st);
+    }
+    return stage_sweep_f<T, false>(g, in, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, part, st);
+}
+
+template <typename T>
+static int rx_frame_sweep(const pxh_grid *g, const pxh_fields *hyd_, const pxh_fields *bed_, const pxh_fields *out_,
+                          const pxh_fields *old_, const uint8_t *m0, const pxh_sediments *sd, void *dt, int write_G,
+                          double cfl, void *partial, hipStream_t st) {
+    const F<T> hyd = cast_fields<T>(hyd_), bed = cast_fields<T>(bed_);
+    F<T> out{}, old{};
+    if (out_) out = cast_fields<T>(out_);
+    if (old_) old = cast_fields<T>(old_);
+    T *dtp = static_cast<T *>(dt);
+    T *part = static_cast<T *>(partial);
+    if constexpr (sizeof(T) == 4) {
+        if (!(g->flags & PXH_EXACT_F32))
+            return frame_sweep_f<T, true>(g, hyd, bed, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, cfl, part, st);
+    }
+    return frame_sweep_f<T, false>(g, hyd, bed, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, cfl, part, st);
 }
 
 template <typename T>
@@ -765,7 +868,7 @@ int pxh_roe_exner_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields
     if (in->h == out->h || in->hu == out->hu || in->hv == out->hv || in->z_b == out->z_b || in->G == out->G)
         return PXH_E_ARG;
     if (!lo_ok(g, {in, out}) || (in->z_b_lo && in->z_b_lo == out->z_b_lo)) return PXH_E_ARG;
-    if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
+    if (!bc_ok(bc)) return PXH_E_ARG;
     return DISPATCH(g, rx_step, g, in, out, m0, sd, bc, dt_dev, static_cast<hipStream_t>(stream));
 }
 
@@ -831,6 +934,51 @@ int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void 
     return launch_status();
 }
 
+int pxh_roe_exner_stage_sweep(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const pxh_fields *old,
+                              const uint8_t *m0, const pxh_sediments *sd, const void *dt_dev, int flags,
+                              void *dt_partial, void *stream) {
+    if (!grid_ok(g) || !fields_ok(in, true) || !fields_ok(out, false) || !m0 || !sd || !dt_dev) return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    if (in->h == out->h || in->hu == out->hu || in->hv == out->hv) return PXH_E_ARG;
+    if (!old) {
+        if (!fields_ok(out, true) || in->z_b == out->z_b || in->G == out->G || !lo_ok(g, {in, out}) ||
+            (in->z_b_lo && in->z_b_lo == out->z_b_lo))
+            return PXH_E_ARG;
+    } else {
+        if (!fields_ok(old, true) || !lo_ok(g, {in, old})) return PXH_E_ARG;
+        const void *po[] = {old->h, old->hu, old->hv, old->z_b, old->G};
+        for (const void *q : po)
+            if (q == in->h || q == in->hu || q == in->hv || q == in->z_b || q == in->G || q == out->h ||
+                q == out->hu || q == out->hv)
+                return PXH_E_ARG;
+        if (old->z_b_lo && old->z_b_lo == in->z_b_lo) return PXH_E_ARG;
+    }
+    const int write_G = (flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
+    return DISPATCH(g, rx_stage_sweep, g, in, out, old, m0, sd, dt_dev, write_G, old ? dt_partial : nullptr,
+                    static_cast<hipStream_t>(stream));
+}
+
+int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_fields *bed, const pxh_fields *out,
+                              const pxh_fields *old, const uint8_t *m0, const pxh_sediments *sd, void *dt_dev,
+                              int flags, double cfl, void *dt_partial, void *stream) {
+    if (!grid_ok(g) || !fields_ok(hyd, false) || !fields_ok(bed, true) || !m0 || !sd || !dt_dev) return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    if (!old) {
+        if (!fields_ok(out, true) || bed->z_b == out->z_b || bed->G == out->G || !lo_ok(g, {bed, out}) ||
+            (bed->z_b_lo && bed->z_b_lo == out->z_b_lo))
+            return PXH_E_ARG;
+    } else {
+        if (!fields_ok(old, true) || !lo_ok(g, {bed, old})) return PXH_E_ARG;
+        const void *po[] = {old->h, old->hu, old->hv, old->z_b, old->G};
+        for (const void *q : po)
+            if (q == hyd->h || q == hyd->hu || q == hyd->hv || q == bed->z_b || q == bed->G) return PXH_E_ARG;
+        if (old->z_b_lo && old->z_b_lo == bed->z_b_lo) return PXH_E_ARG;
+    }
+    const int write_G = (flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
+    return DISPATCH(g, rx_frame_sweep, g, hyd, bed, old ? nullptr : out, old, m0, sd, dt_dev, write_G, cfl,
+                    old ? dt_partial : nullptr, static_cast<hipStream_t>(stream));
+}
+
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
     const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx);
@@ -854,7 +1002,7 @@ int pxh_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_field
             if (pa[i] == pb[j] || pa[i] == pc[j] || pb[i] == pc[j]) return PXH_E_ARG;
     if (!lo_ok(g, {a, b, c}) || (a->z_b_lo && (a->z_b_lo == b->z_b_lo || a->z_b_lo == c->z_b_lo || b->z_b_lo == c->z_b_lo)))
         return PXH_E_ARG;
-    if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
+    if (!bc_ok(bc)) return PXH_E_ARG;
     return DISPATCH(g, rx_ssprk2, g, a, b, c, m0, sd, bc, cfl, dt_dev, ws, dt_ready,
                     static_cast<hipStream_t>(stream));
 }
@@ -884,7 +1032,7 @@ int pxh_roe_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out,
                  const void *dt_dev, void *stream) {
     if (!grid_ok(g) || !fields_ok(in, false) || !fields_ok(out, false) || !dt_dev) return PXH_E_ARG;
     if (in->h == out->h || in->hu == out->hu || in->hv == out->hv) return PXH_E_ARG;
-    if (bc && (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries))) return PXH_E_ARG;
+    if (!bc_ok(bc)) return PXH_E_ARG;
     return DISPATCH(g, roe_step, g, in, out, bc, dt_dev, static_cast<hipStream_t>(stream));
 }
 
@@ -895,7 +1043,7 @@ int pxh_roe_dt(const pxh_grid *g, const pxh_fields *s, double cfl, void *dt_out,
 
 int pxh_bc_apply(const pxh_grid *g, const pxh_fields *s, const pxh_bc *bc, void *stream) {
     if (!grid_ok(g) || !s || !bc) return PXH_E_ARG;
-    if (bc->n_handlers > PXH_MAX_HANDLERS || (bc->n_entries > 0 && !bc->entries)) return PXH_E_ARG;
+    if (!bc_ok(bc)) return PXH_E_ARG;
     return DISPATCH(g, bc_apply_t, g, s, bc, static_cast<hipStream_t>(stream));
 }
 

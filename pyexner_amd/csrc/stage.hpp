@@ -6,8 +6,8 @@
 // A stage of the reference (roe_exner_solver.py:80-116) fills the ghost ring
 // between the hydro update and the bed update.  When every target of the fill
 // lies on the outer ring of cells and every source within two cells of the
-// grid edge (Reflective, Transmissive and TransmissiveBed rings do; Periodic
-// does not), only cells of the two-cell frame read filled values in the bed
+// grid edge (Reflective, Transmissive and Periodic rings do: their sources sit
+// in rows/columns 1 and n-2), only cells of the two-cell frame read filled values in the bed
 // update.  The fused kernel then updates the bed of every cell off the frame
 // right behind the hydro update, from the updated rows it holds in registers,
 // and the frame follows after the fill in small band kernels (k_rx_band_*).
@@ -21,9 +21,16 @@
 
 namespace pxh {
 
+#ifndef PXH_SR_STAGE
+#define PXH_SR_STAGE 32
+#endif
+#ifndef PXH_FWPB
+#define PXH_FWPB 1
+#endif
 constexpr int FO = 60;          // owned columns per fused strip
 constexpr int FHW = 2;          // halo lanes on each side
-constexpr int SR_STAGE = 32;    // owned rows per fused strip
+constexpr int SR_STAGE = PXH_SR_STAGE;  // owned rows per fused strip
+constexpr int FWPB = PXH_FWPB;  // fused strips (wavefronts) per block
 
 struct StagePos {
     int lane, c, cc, r0, r1;
@@ -41,7 +48,7 @@ __device__ __forceinline__ StagePos stage_pos(int ny, int nx) {
     p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
     p.out = p.lane >= FHW && p.lane < FHW + FO && p.c < nx;
     p.hval = p.lane >= 1 && p.lane <= SW - 2 && p.c >= 0 && p.c < nx;
-    p.r0 = (by * SWPB + w) * SR_STAGE;
+    p.r0 = (by * FWPB + w) * SR_STAGE;
     p.r1 = p.r0 + SR_STAGE < ny ? p.r0 + SR_STAGE : ny;
     return p;
 }
@@ -49,7 +56,7 @@ __device__ __forceinline__ StagePos stage_pos(int ny, int nx) {
 static inline dim3 stage_grid(int ny, int nx) {
     const int sx = (nx + FO - 1) / FO;
     const int sy = (ny + SR_STAGE - 1) / SR_STAGE;
-    return dim3(sx, (sy + SWPB - 1) / SWPB);
+    return dim3(sx, (sy + FWPB - 1) / FWPB);
 }
 
 // The two-cell frame: rows [0, top) and [bot, ny), and in the rows between,
@@ -122,12 +129,12 @@ __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, co
 //   interior cells of the combined state into partial[block].
 // in: the stage input (hydro fields, z, n_b, z_b, z_b_lo, G).
 template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
-__global__ __launch_bounds__(SW *SWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
+__global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                              const uint8_t *__restrict__ m0,
                                                              const T *__restrict__ dtp, int ny, int nx, T dx,
                                                              pxh_sediments sd, int write_G,
                                                              T *__restrict__ partial) {
-    __shared__ T red[SWPB];
+    __shared__ T red[FWPB];
     const StagePos p = stage_pos(ny, nx);
     const Frame fr = frame_of(ny, nx);
     const T zero = T(0), hf = T(0.5);
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
         }
     }
     if (MODE == 1 && DT) {
-        vmin = wave_block_min(vmin, red);
+        vmin = wave_block_min_n<FWPB>(vmin, red);
         if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = vmin;
     }
 }
@@ -273,12 +280,12 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
 // Host launcher: picks the (LO, MPM) instance for in's compensated bed and the
 // sediment mode.  In a compensated run every set (in, out, old) has z_b_lo.
 template <typename T, bool FAST, int MODE, bool DT>
-static void launch_stage(dim3 grid, dim3 block, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
+static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
                          const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, T dx,
                          const pxh_sediments &sd, int write_G, T *partial) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
 #define PXH_STAGE(LO_, MPM_)                                                                                   \
-    hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_>), grid, block, 0, st, in, out, cb, old, \
+    hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_>), grid, dim3(SW * FWPB), 0, st, in, out, cb, old, \
                        m0, dtp, ny, nx, dx, sd, write_G, partial)
     if constexpr (sizeof(T) == 4) {
         if (lo) {

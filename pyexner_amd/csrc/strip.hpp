@@ -908,15 +908,21 @@ __device__ __forceinline__ T dt_edge_d(const DD<T> &I, const DD<T> &J, T dx) {
     return Mth<T, FAST>::div(dx, sp);
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_block_min(T v, T *red) {
+// Minimum over a block of NW wavefronts; the result is valid in thread 0.
+template <int NW, typename T>
+__device__ __forceinline__ T wave_block_min_n(T v, T *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & (SW - 1)) == 0) red[threadIdx.x / SW] = v;
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int k = 1; k < SWPB; ++k) v = pmin(v, red[k]);
+        for (int k = 1; k < NW; ++k) v = pmin(v, red[k]);
     return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_block_min(T v, T *red) {
+    return wave_block_min_n<SWPB>(v, red);
 }
 
 // MODE 0: out.z_b = updated bed (+ out.G when write_G).

@@ -56,7 +56,7 @@ class ExnerBackend:
     STEP_GROUPS = (_hip.G_H, _hip.G_HUV, _hip.G_ZB, _hip.G_G)
 
     def __init__(self, shape, dtype, dx, seds, constant, boundaries, device="cuda", compensated_bed=None,
-                 exact_f32=False):
+                 exact_f32=False, fused_stage=True):
         self.shape = tuple(shape)
         self.dtype = dtype
         self.np_dtype = np.float32 if dtype == torch.float32 else np.float64
@@ -75,6 +75,12 @@ class ExnerBackend:
                                       "are not supported by the fused ring fill")
         self.bc_all = boundaries.program() if boundaries is not None else None
         self.bc_step = boundaries.program(self.STEP_GROUPS) if boundaries is not None else None
+        # The step entries fuse the hydro and bed updates of a stage when the
+        # program is frame-local (pxh_bc.flags); fused_stage=False keeps the
+        # two sweeps with the ring fill between them.
+        self.fused_stage = bool(fused_stage)
+        if self.bc_step is not None and not self.fused_stage:
+            self.bc_step.flags &= ~_hip.BC_FRAME
         ws = _hip.load().pxh_dt_workspace_size(self.grid)
         if ws < 0:
             raise ValueError("invalid grid")
@@ -134,13 +140,13 @@ def get_mask(state, mpi_handler=None, b_mask=None):
 
 def config_fn_roeexner(state, mpi_handler, boundaries, dx, params):
     """roe_exner_solver.py:41-56.  Backend options (not in the reference YAML):
-    ``backend: {compensated_bed: bool, exact_f32: bool}``."""
+    ``backend: {compensated_bed: bool, exact_f32: bool, fused_stage: bool}``."""
     grass = params["erosion"]["grass_factor"]
     constant = not isinstance(grass, str)
     opts = params.get("backend", {}) or {}
     backend = ExnerBackend(state.h.shape, state.h.dtype, dx, state.seds, constant, boundaries,
                            device=state.h.device, compensated_bed=opts.get("compensated_bed"),
-                           exact_f32=opts.get("exact_f32", False))
+                           exact_f32=opts.get("exact_f32", False), fused_stage=opts.get("fused_stage", True))
     return SolverConfig(mpi_handler=mpi_handler, boundaries=boundaries, dx=dx, halo_exchange=identity_halo,
                         compute_G="constant" if constant else "MPM", compute_n="constant" if constant else "MPM",
                         backend=backend)

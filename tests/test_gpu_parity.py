@@ -16,7 +16,7 @@ import torch
 
 from oracle import steps as osteps
 from oracle.closed_forms import stoker_wet_dambreak
-from pyexner_amd import cases
+from pyexner_amd import _hip, cases
 from tests.helpers import (FIELDS_EXNER, FIELDS_ROE, compare, gpu_case, gpu_fe, gpu_ssprk2, oracle_exner,
                            oracle_roe, rel_linf)
 
@@ -107,6 +107,26 @@ def test_exner_ssprk2_1000_steps_f64(cuda):
     errs = compare(sg, so, FIELDS_EXNER)
     _print("ssprk2 x1000", errs)
     assert max(errs.values()) <= F64_TOL
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_exner_ssprk2_fused_and_two_sweep_f64(cuda, fused):
+    """The fused stage (one sweep per stage, the frame after the ring fill) and
+    the two-sweep stage both match the oracle: 200 SSPRK2 steps of the channel
+    (Transmissive + Reflective ring) and of the periodic box."""
+    for mk in (lambda: cases.channel(nx=70, ny=36, n_bumps=8, seed=12),
+               lambda: cases.periodic_box(nx=40, ny=34, seed=13)):
+        mesh, arr, params = mk()
+        params = dict(params, backend={"fused_stage": fused})
+        so, prob = oracle_exner(mesh, arr, params)
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+        assert (sc.backend.bc_step.flags & _hip.BC_FRAME) == (_hip.BC_FRAME if fused else 0)
+        for _ in range(200):
+            so, _ = osteps.ssprk2_step(prob, so)
+            sg, _ = gpu_ssprk2(sg, mask, solver, sc, prob.cfl)
+        errs = compare(sg, so, FIELDS_EXNER)
+        _print(f"fused={fused} x200", errs)
+        assert max(errs.values()) <= F64_TOL
 
 
 def test_exner_radial_dambreak_fe_f64(cuda):
@@ -279,3 +299,44 @@ def test_stepper_graph_replay_matches_eager_f32(cuda):
         torch.cuda.synchronize()
         for f in FIELDS_EXNER:
             assert torch.equal(getattr(graphed.state, f), getattr(eager.state, f)), (k, f)
+
+
+@pytest.mark.parametrize("dtype,grass", [(torch.float32, "MPM"), (torch.float64, "MPM"), (torch.float32, 0.004)])
+def test_stage_and_frame_sweeps_equal_fused_ssprk2(cuda, dtype, grass):
+    """pxh_roe_exner_stage_sweep / pxh_roe_exner_frame_sweep with the ring fills
+    in between, issued the way a caller that sequences the step itself issues
+    them, give the state and next dt of pxh_roe_exner_ssprk2 bit for bit."""
+    from pyexner_amd import _hip
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = cases.radial_dambreak(nx=133, ny=101, dx=0.05, n_bumps=6, radius=1.5)
+    params = dict(params, erosion=dict(params["erosion"], grass_factor=grass))
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
+    assert sc.boundaries.frame_local
+    ref = ExnerStepper(sc.backend, sg, mask, 0.5)
+    seq = ExnerStepper(sc.backend, sg, mask, 0.5)
+    be, m0, g, sz = sc.backend, seq.m0, sc.backend.grid, sc.backend.sed
+    stream = torch.cuda.current_stream().cuda_stream
+    part = be.workspace.data_ptr()
+    bc_hy = sc.boundaries.program((_hip.G_H, _hip.G_HUV))
+    bc_bed1 = sc.boundaries.program((_hip.G_ZB,) if sz.mpm else (_hip.G_ZB, _hip.G_G))
+    bc_end = sc.boundaries.program((_hip.G_ZB, _hip.G_G))
+    skip = _hip.SWEEP_SKIP_G
+    dtp = seq.dt.data_ptr()
+    ref.compute_dt()
+    seq.compute_dt()
+    for k in range(25):
+        ref.ssprk2_step()
+        fa, fb, fc = seq.fa, seq.fb, seq.fc
+        _hip.call("pxh_roe_exner_stage_sweep", g, fa, fb, None, m0, sz, dtp, skip, None, stream)
+        _hip.call("pxh_bc_apply", g, fb, bc_hy, stream)
+        _hip.call("pxh_roe_exner_frame_sweep", g, fb, fa, fb, None, m0, sz, dtp, skip, 0.0, None, stream)
+        _hip.call("pxh_bc_apply", g, fb, bc_bed1, stream)
+        _hip.call("pxh_roe_exner_stage_sweep", g, fb, fc, fa, m0, sz, dtp, 0, part, stream)
+        _hip.call("pxh_bc_apply", g, fc, bc_hy, stream)
+        _hip.call("pxh_roe_exner_frame_sweep", g, fc, fb, None, fa, m0, sz, dtp, 0, 0.5, part, stream)
+        _hip.call("pxh_bc_apply", g, fa, bc_end, stream)
+        torch.cuda.synchronize()
+        assert torch.equal(ref.dt, seq.dt), k
+        for f in FIELDS_EXNER + (("z_b_lo",) if ref.state.z_b_lo is not None else ()):
+            assert torch.equal(getattr(ref.state, f), getattr(seq.state, f)), (k, f)
+    assert float(ref.dt) > 0

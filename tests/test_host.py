@@ -195,3 +195,54 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_frame_local_programs():
+    """The fused stage needs every ring-fill target on the outer ring and every
+    source within two cells of the edge: true for Reflective, Transmissive and
+    Periodic rings, false when a boundary polygon reaches into the interior."""
+    from pyexner_amd.boundaries import BoundaryManager
+    for mk in (lambda: cases.channel(nx=40, ny=20, n_bumps=2), lambda: cases.radial_dambreak(nx=32, ny=30, n_bumps=2,
+                                                                                            radius=0.4),
+               lambda: cases.periodic_box(nx=24, ny=20)):
+        mesh, arr, params = mk()
+        bm = BoundaryManager(params, mesh.local_X, mesh.local_Y, device="cpu")
+        assert bm.frame_local
+    mesh, arr, params = cases.channel(nx=40, ny=20, n_bumps=2)
+    X, Y, dh = mesh.local_X, mesh.local_Y, mesh.dh
+    x0 = X[0, 5]
+    # a three-cell-wide Transmissive strip: its targets leave the outer ring
+    params["boundaries"]["inlet"]["polygon"] = [[X[0, 0] - dh / 2, Y[-1, 0] - dh / 2], [x0 - dh / 2, Y[-1, 0] - dh / 2],
+                                                [x0 - dh / 2, Y[0, 0] + dh / 2], [X[0, 0] - dh / 2, Y[0, 0] + dh / 2]]
+    bm = BoundaryManager(params, mesh.local_X, mesh.local_Y, device="cpu")
+    assert not bm.frame_local
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """The ctypes mirrors in pyexner_amd._hip have the sizes and field offsets
+    the C compiler gives the header's structs."""
+    import ctypes
+    mirrors = {"pxh_grid": _hip.Grid, "pxh_fields": _hip.Fields, "pxh_sediments": _hip.Sediments,
+               "pxh_bc_entry": _hip.BCEntry, "pxh_bc_handler": _hip.BCHandler, "pxh_bc": _hip.BC}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pyexner_hip.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append(f'    printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'    printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True,
+                   timeout=60)
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=60).stdout.split("\n")
+    seen = 0
+    for ln in got:
+        if not ln:
+            continue
+        cname, what, val = ln.split()
+        cls = mirrors[cname]
+        want = ctypes.sizeof(cls) if what == "sizeof" else getattr(cls, what).offset
+        assert int(val) == want, (cname, what, int(val), want)
+        seen += 1
+    assert seen == sum(1 + len(c._fields_) for c in mirrors.values())
