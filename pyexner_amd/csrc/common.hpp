@@ -57,6 +57,25 @@ struct F {
     T *z_b_lo;  // compensated-bed low part (float32 only) or null
 };
 
+// The sediment table as the kernels take it: the C table plus its float32
+// fold for the hardware-math path, fw = 8 frac C/D / (1 - porosity), so that
+// G = sum_k fw_k h^(-1/2) sqrt(dtheta_k) dtheta_k / (theta_k^1.5 + 1e-5).
+struct SedK : pxh_sediments {
+    float fAB[PXH_MAX_SED], fthc[PXH_MAX_SED], fw[PXH_MAX_SED];
+};
+
+static inline SedK sed_k(const pxh_sediments &s) {
+    SedK k;
+    static_cast<pxh_sediments &>(k) = s;
+    for (int i = 0; i < PXH_MAX_SED; ++i) {
+        const bool on = i < s.n_sed;
+        k.fAB[i] = on ? float(s.AB[i]) : 0.0f;
+        k.fthc[i] = on ? float(s.theta_c[i]) : 0.0f;
+        k.fw[i] = on ? float(8.0 * s.frac[i] * s.CD[i] / s.one_minus_por) : 0.0f;
+    }
+    return k;
+}
+
 // Global-memory access with a 32-bit element offset.  Field pointers reach the
 // kernels inside structs, where the compiler cannot see they are global, and
 // would otherwise emit flat loads (which share the LDS wait counter with the
@@ -88,6 +107,30 @@ template <typename T>
 __device__ __forceinline__ void str(T *p, long row_off, unsigned col, T v) {
     const unsigned long base = uni64(reinterpret_cast<unsigned long>(p + row_off));
     *(PXH_GLOBAL T *)((PXH_GLOBAL char *)base + col * unsigned(sizeof(T))) = v;
+}
+// The same with a 32-bit row offset (elements) from a base the caller has moved
+// to the first row it touches, so that the whole byte offset fits 32 bits: the
+// access then takes the base in scalar registers and one vector register for
+// the offset, with no 64-bit address arithmetic per load.  The caller keeps
+// (row_off + col) * sizeof(T) below 2^32.
+template <typename T>
+__device__ __forceinline__ T ldr(const T *p, unsigned row_off, unsigned col) {
+    const unsigned long base = uni64(reinterpret_cast<unsigned long>(p));
+    const unsigned ro = __builtin_amdgcn_readfirstlane(row_off * unsigned(sizeof(T)));
+    return *(const PXH_GLOBAL T *)((const PXH_GLOBAL char *)base + (ro + col * unsigned(sizeof(T))));
+}
+template <typename T>
+__device__ __forceinline__ void str(T *p, unsigned row_off, unsigned col, T v) {
+    const unsigned long base = uni64(reinterpret_cast<unsigned long>(p));
+    const unsigned ro = __builtin_amdgcn_readfirstlane(row_off * unsigned(sizeof(T)));
+    *(PXH_GLOBAL T *)((PXH_GLOBAL char *)base + (ro + col * unsigned(sizeof(T)))) = v;
+}
+
+// f with every field pointer moved by off elements (null stays null).
+template <typename T>
+__host__ __device__ __forceinline__ F<T> shifted(const F<T> &f, long off) {
+    auto mv = [off](T *p) { return p ? p + off : p; };
+    return F<T>{mv(f.h), mv(f.hu), mv(f.hv), mv(f.z), mv(f.z_b), mv(f.n), mv(f.n_b), mv(f.G), mv(f.z_b_lo)};
 }
 
 inline int launch_status() {

@@ -503,12 +503,12 @@ static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, con
     const F<T> none{};
     const dim3 fg = stage_grid(ny, nx);
     if (!old)
-        launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, dx, *sd, write_G,
+        launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, dx, sed_k(*sd), write_G,
                                         static_cast<T *>(nullptr));
     else if (part)
-        launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, *sd, 1, part);
+        launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, sed_k(*sd), 1, part);
     else
-        launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, *sd, 1,
+        launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, sed_k(*sd), 1,
                                         static_cast<T *>(nullptr));
     return launch_status();
 }
@@ -524,11 +524,11 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
     const int nbb = int((nband + 255) / 256);  // >= 1: a non-empty grid has a frame
     if (!old) {
         hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 0>), dim3(nbb), dim3(256), 0, st, hyd, bed, out, none, m0, dtp, ny,
-                           nx, dx, *sd, write_G, nband);
+                           nx, dx, sed_k(*sd), write_G, nband);
         return launch_status();
     }
     hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 1>), dim3(nbb), dim3(256), 0, st, hyd, bed, none, *old, m0, dtp, ny, nx,
-                       dx, *sd, 1, nband);
+                       dx, sed_k(*sd), 1, nband);
     if (int e = launch_status()) return e;
     if (!part) return PXH_OK;
     const dim3 fg = stage_grid(ny, nx);
@@ -551,7 +551,16 @@ static int rx_stage0_fused(const pxh_grid *g, const F<T> &in, const F<T> &out, c
     return frame_sweep_f<T, FAST>(g, out, in, out, nullptr, m0, sd, const_cast<T *>(dtp), write_G, 0.0, nullptr, st);
 }
 
-static bool frame_fused(const pxh_bc *bc) { return bc && (bc->flags & PXH_BC_FRAME); }
+// The fused stage addresses a strip's rows with 32-bit byte offsets from the
+// strip's first row: SR_STAGE + 3 rows of nx elements must fit.
+static bool stage_fits(const pxh_grid *g) {
+    const long esz = g->dtype == PXH_F64 ? 8 : 4;
+    return long(SR_STAGE + 4) * g->nx * esz < (1L << 32);
+}
+
+static bool frame_fused(const pxh_grid *g, const pxh_bc *bc) {
+    return bc && (bc->flags & PXH_BC_FRAME) && stage_fits(g);
+}
 
 template <typename T, bool FAST>
 static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const uint8_t *m0,
@@ -560,7 +569,7 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     const T *dtp = static_cast<const T *>(dt);
-    if (frame_fused(bc)) {
+    if (frame_fused(g, bc)) {
         if (int e = rx_stage0_fused<T, FAST>(g, in, out, m0, sd, bc, dtp, 1, st)) return e;
         return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
     }
@@ -569,7 +578,7 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
     hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), bg, sb, 0, st, out, in.z_b, in.z_b_lo, in.G, out, out,
-                       m0, dtp, ny, nx, dx, *sd, 1, nullptr);
+                       m0, dtp, ny, nx, dx, sed_k(*sd), 1, nullptr);
     if (int e = launch_status()) return e;
     return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
 }
@@ -604,7 +613,7 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     }
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (sd->mpm ? 0u : (1u << PXH_G_G));
-    if (frame_fused(bc)) {
+    if (frame_fused(g, bc)) {
         // stage 1: b = S(a)
         if (int e = rx_stage0_fused<T, FAST>(g, a, b, m0, sd, bc, dtp, sd->mpm ? 0 : 1, st)) return e;
         if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
@@ -618,14 +627,14 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, HY, st)) return e;
     hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), bg, sb, 0, st, b, a.z_b, a.z_b_lo, a.G, b, b, m0, dtp,
-                       ny, nx, dx, *sd, sd->mpm ? 0 : 1, nullptr);
+                       ny, nx, dx, sed_k(*sd), sd->mpm ? 0 : 1, nullptr);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, b, c, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
     hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), bg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny,
-                       nx, dx, *sd, 1, part);
+                       nx, dx, sed_k(*sd), 1, part);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
     hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
@@ -653,15 +662,15 @@ static int rx_bed_sweep_t(const pxh_grid *g, const pxh_fields *hyd_, const pxh_f
     const T dx = static_cast<T>(g->dx);
     if (!old_) {
         hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G, out,
-                           out, m0, dtp, g->ny, g->nx, dx, *sd, 1, nullptr);
+                           out, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr);
     } else {
         F<T> old = cast_fields<T>(old_);
         if (partial)
             hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
-                               out, old, m0, dtp, g->ny, g->nx, dx, *sd, 1, static_cast<T *>(partial));
+                               out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, static_cast<T *>(partial));
         else
             hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, false>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
-                               out, old, m0, dtp, g->ny, g->nx, dx, *sd, 1, nullptr);
+                               out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr);
     }
     return launch_status();
 }
@@ -953,6 +962,7 @@ int pxh_roe_exner_stage_sweep(const pxh_grid *g, const pxh_fields *in, const pxh
                 return PXH_E_ARG;
         if (old->z_b_lo && old->z_b_lo == in->z_b_lo) return PXH_E_ARG;
     }
+    if (!stage_fits(g)) return PXH_E_RANGE;
     const int write_G = (flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
     return DISPATCH(g, rx_stage_sweep, g, in, out, old, m0, sd, dt_dev, write_G, old ? dt_partial : nullptr,
                     static_cast<hipStream_t>(stream));

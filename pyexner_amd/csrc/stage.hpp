@@ -110,7 +110,7 @@ struct SRow {
 // for every outstanding load).
 template <typename T, bool LO, bool MPM>
 __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, const T *G_in, const uint8_t *m0,
-                                           long ro, unsigned c) {
+                                           unsigned ro, unsigned c) {
     SRow<T> q;
     q.h = hr_load(in, m0, ro, c);
     q.zl = T(0);
@@ -132,7 +132,7 @@ template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
 __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                              const uint8_t *__restrict__ m0,
                                                              const T *__restrict__ dtp, int ny, int nx, T dx,
-                                                             pxh_sediments sd, int write_G,
+                                                             SedK sd, int write_G,
                                                              T *__restrict__ partial) {
     __shared__ T red[FWPB];
     const StagePos p = stage_pos(ny, nx);
@@ -144,8 +144,6 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
         const T dt = *dtp;
         const T dtdx = dt / dx;
         const unsigned ucc = unsigned(p.cc), uc = unsigned(c);
-        const T *zb_lo_in = in.z_b_lo;
-        const T *G_in = in.G;
         const bool inE = p.lane <= SW - 3, inW = p.lane >= 2;
         const bool colI = c >= fr.lft && c < fr.rgt;           // interior column
         const bool colIE = c >= fr.lft && c + 1 < fr.rgt;      // x edge (c | c+1) between interior cells
@@ -154,22 +152,26 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
         const int hr0 = p.r0 > 0 ? p.r0 - 1 : 0;
         const int hr1 = p.r1 + 1 < ny ? p.r1 + 1 : ny;
         const int rlast = hr1 < ny - 1 ? hr1 : ny - 1;  // last row any load reads
-        auto row = [&](int r) { return long(r <= rlast ? r : rlast) * nx; };
-        HCarry<T> hc = hcarry_init<T, FAST>(in, m0, hr0, nx, ucc, dx);
+        // Every row access is relative to row b0, the first row the strip
+        // touches, through pointers moved there: (rlast - b0 + 1) nx elements
+        // fit a 32-bit byte offset (the launcher checks nx).
+        const int b0 = hr0 > 0 ? hr0 - 1 : 0;
+        const long sh = long(b0) * nx;
+        const F<T> ins = shifted(in, sh), outs = shifted(out, sh), cbs = shifted(cb, sh), olds = shifted(old, sh);
+        const uint8_t *m0s = m0 + sh;
+        auto row = [&](int r) { return unsigned((r <= rlast ? r : rlast) - b0) * unsigned(nx); };
+        HCarry<T> hc = hcarry_at<T, FAST>(ins, m0s, row(hr0), row(b0), hr0 > 0, ucc, dx);
         T cur_zl = zero, cur_G = zero;  // bed inputs of the current hydro row
         {
-            const SRow<T> q0 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, long(hr0) * nx, ucc);
+            const SRow<T> q0 = sr_load<T, LO, MPM>(ins, ins.z_b_lo, ins.G, m0s, row(hr0), ucc);
             cur_zl = q0.zl;
             cur_G = q0.G;
         }
-        SRow<T> q1 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, row(hr0 + 1), ucc);
-        SRow<T> q2 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, row(hr0 + 2), ucc);
-        // MODE 1: previous-step rows k-1 and k in flight at hydro row k
-        OR<T> o1 = {zero, zero, zero, zero, zero, zero}, o2 = o1;
-        if (MODE == 1) {
-            o1 = or_load(old, row(hr0 > 0 ? hr0 - 1 : 0), ucc);
-            o2 = or_load(old, row(hr0), ucc);
-        }
+        // the next hydro row in flight (row k+1 at hydro row k)
+        SRow<T> q1 = sr_load<T, LO, MPM>(ins, ins.z_b_lo, ins.G, m0s, row(hr0 + 1), ucc);
+        // MODE 1: the previous-step row k-1 in flight at hydro row k
+        OR<T> o1 = {zero, zero, zero, zero, zero, zero};
+        if (MODE == 1) o1 = or_load(olds, row(b0), ucc);
         ED<T> edp{};         // bed values of hydro row k-1
         T fn = zero;         // bed flux through the edge (k-2 | k-1)
         DD<T> dprev{};       // combined dt values of row k-2 (DT)
@@ -183,29 +185,27 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
                 cbn = cb_make(o1, edp.h, edp.hu, edp.hv, edp.G);
                 ozb = o1.zb;
                 ozl = o1.zl;
-                o1 = o2;
+                o1 = or_load(olds, row(k), ucc);  // clamped: unconditional
             }
             const HD<T> nxt = hd_derive<T, FAST>(q1.h);
             const T nxt_zl = q1.zl, nxt_G = q1.G;
-            q1 = q2;
-            if (k + 3 <= rlast) q2 = sr_load<T, LO, MPM>(in, zb_lo_in, G_in, m0, long(k + 3) * nx, ucc);
-            if (MODE == 1) o2 = or_load(old, row(k + 1), ucc);  // clamped: unconditional
+            q1 = sr_load<T, LO, MPM>(ins, ins.z_b_lo, ins.G, m0s, row(k + 2), ucc);  // clamped
             // hydro row k
             const T zb_k = hc.cur.zb;
             const int nod_k = hc.cur.flags & 2;
             T h1, hu1, hv1;
-            hydro_row<T, FAST>(hc, nxt, k, hr0, c, p.hval, inE, inW, ny, nx, dx, dt, dtdx, in, m0, h1, hu1, hv1);
+            hydro_row<T, FAST>(hc, nxt, k, hr0, c, p.hval, inE, inW, ny, nx, dx, dt, dtdx, ins, m0s, sh, h1, hu1, hv1);
             const bool own_k = p.out && k >= p.r0 && k < p.r1;
             if (own_k) {
-                const long ro = long(k) * nx;
+                const unsigned ro = row(k);
                 if (MODE == 0) {
-                    str(out.h, ro, uc, h1);
-                    str(out.hu, ro, uc, hu1);
-                    str(out.hv, ro, uc, hv1);
+                    str(outs.h, ro, uc, h1);
+                    str(outs.hu, ro, uc, hu1);
+                    str(outs.hv, ro, uc, hv1);
                 } else if (frame3 || k <= 2 || k >= ny - 3) {
-                    str(cb.h, ro, uc, h1);
-                    str(cb.hu, ro, uc, hu1);
-                    str(cb.hv, ro, uc, hv1);
+                    str(cbs.h, ro, uc, h1);
+                    str(cbs.hu, ro, uc, hu1);
+                    str(cbs.hv, ro, uc, hv1);
                 }
             }
             ER<T> er;
@@ -231,26 +231,26 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
                     div = div + (-fn);  // rb >= 2
                     div = div + fy;     // rb + 1 < ny
                     if (p.out && colI) {
-                        const long i = long(rb) * nx;
+                        const unsigned i = row(rb);
                         const T inc = FAST ? dtdx * div : dt * div / dx;
                         T zb1 = edp.zb - inc, zl1 = zero;
                         if constexpr (LO) cb_sub(edp.zb, edp.zl, inc, zb1, zl1);
                         if (MODE == 0) {
-                            str(out.z_b, i, uc, zb1);
-                            if constexpr (LO) str(out.z_b_lo, i, uc, zl1);
-                            if (write_G) str(out.G, i, uc, edp.G);
+                            str(outs.z_b, i, uc, zb1);
+                            if constexpr (LO) str(outs.z_b_lo, i, uc, zl1);
+                            if (write_G) str(outs.G, i, uc, edp.G);
                         } else {
-                            str(old.h, i, uc, cbn.h);
-                            str(old.hu, i, uc, cbn.hu);
-                            str(old.hv, i, uc, cbn.hv);
-                            str(old.G, i, uc, cbn.G);
+                            str(olds.h, i, uc, cbn.h);
+                            str(olds.hu, i, uc, cbn.hu);
+                            str(olds.hv, i, uc, cbn.hv);
+                            str(olds.G, i, uc, cbn.G);
                             T zc = hf * ozb + hf * zb1;
                             if constexpr (LO) {
                                 T zcl;
                                 cb_half_sum(ozb, ozl, zb1, zl1, zc, zcl);
-                                str(old.z_b_lo, i, uc, zcl);
+                                str(olds.z_b_lo, i, uc, zcl);
                             }
-                            str(old.z_b, i, uc, zc);
+                            str(olds.z_b, i, uc, zc);
                         }
                     }
                     if (MODE == 1 && DT) {
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
 template <typename T, bool FAST, int MODE, bool DT>
 static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
                          const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, T dx,
-                         const pxh_sediments &sd, int write_G, T *partial) {
+                         const SedK &sd, int write_G, T *partial) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
 #define PXH_STAGE(LO_, MPM_)                                                                                   \
     hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_>), grid, dim3(SW * FWPB), 0, st, in, out, cb, old, \
@@ -310,7 +310,7 @@ __device__ __forceinline__ T ldl(const T *p, long i) {
 
 template <typename T, bool FAST>
 __device__ __forceinline__ ED<T> ed_load_lane(const F<T> &hyd, const T *zb_in, const T *zb_lo_in, const T *G_in,
-                                              const uint8_t *m0, const pxh_sediments &sd, long i) {
+                                              const uint8_t *m0, const SedK &sd, long i) {
     ER<T> q;
     q.h = ldl(hyd.h, i); q.hu = ldl(hyd.hu, i); q.hv = ldl(hyd.hv, i);
     q.zb = ldl(zb_in, i);
@@ -324,7 +324,7 @@ __device__ __forceinline__ ED<T> ed_load_lane(const F<T> &hyd, const T *zb_in, c
 }
 
 template <typename T, bool FAST>
-__device__ __forceinline__ ED<T> ed_at(const F<T> &hyd, const F<T> &bed, const uint8_t *m0, const pxh_sediments &sd,
+__device__ __forceinline__ ED<T> ed_at(const F<T> &hyd, const F<T> &bed, const uint8_t *m0, const SedK &sd,
                                        long i) {
     return ed_load_lane<T, FAST>(hyd, bed.z_b, bed.z_b_lo, bed.G, m0, sd, i);
 }
@@ -335,7 +335,7 @@ __device__ __forceinline__ ED<T> ed_at(const F<T> &hyd, const F<T> &bed, const u
 template <typename T, bool FAST, int MODE>
 __global__ __launch_bounds__(256) void k_rx_band_bed(F<T> hyd, F<T> bed, F<T> out, F<T> old,
                                                       const uint8_t *__restrict__ m0, const T *__restrict__ dtp,
-                                                      int ny, int nx, T dx, pxh_sediments sd, int write_G,
+                                                      int ny, int nx, T dx, SedK sd, int write_G,
                                                       long nband) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
     if (t >= nband) return;

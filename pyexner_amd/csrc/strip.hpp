@@ -149,8 +149,9 @@ struct HR {          // raw per-cell inputs of the hydro sweep
     int m;
 };
 
-template <typename T>
-__device__ __forceinline__ HR<T> hr_load(const F<T> &s, const uint8_t *m0, long ro, unsigned c) {
+// RO: long (absolute row offset) or unsigned (from a moved base, 32-bit).
+template <typename T, typename RO>
+__device__ __forceinline__ HR<T> hr_load(const F<T> &s, const uint8_t *m0, RO ro, unsigned c) {
     HR<T> q;
     q.h = ldr(s.h, ro, c); q.hu = ldr(s.hu, ro, c); q.hv = ldr(s.hv, ro, c);
     q.zb = ldr(s.z_b, ro, c); q.z = ldr(s.z, ro, c); q.n = ldr(s.n_b, ro, c);
@@ -186,8 +187,8 @@ __device__ __forceinline__ HD<T> hd_derive(const HR<T> &q) {
     return d;
 }
 
-template <typename T, bool FAST>
-__device__ __forceinline__ HD<T> hd_load(const F<T> &s, const uint8_t *m0, long ro, unsigned c) {
+template <typename T, bool FAST, typename RO>
+__device__ __forceinline__ HD<T> hd_load(const F<T> &s, const uint8_t *m0, RO ro, unsigned c) {
     return hd_derive<T, FAST>(hr_load(s, m0, ro, c));
 }
 
@@ -405,10 +406,12 @@ __device__ __forceinline__ void edge_or_zero(bool active, const HD<T> &I, const 
 // Inlined once, in a loop over the pending neighbours, rather than called: a
 // call would make the whole sweep keep its live registers across the ABI.
 // (r, c) differ from lane to lane (each lane checks its own pending neighbour),
-// so every load here is addressed per lane.
+// so every load here is addressed per lane.  in and m0 may be moved by sh
+// elements (to a strip's first row): cell (r, c) is then element r nx + c - sh.
 template <typename T, bool FAST>
-__device__ __forceinline__ T h1_at(const F<T> &in, const uint8_t *m0, T dt, int ny, int nx, T dx, int r, int c) {
-    const long i = long(r) * nx + c;
+__device__ __forceinline__ T h1_at(const F<T> &in, const uint8_t *m0, long sh, T dt, int ny, int nx, T dx, int r,
+                                  int c) {
+    const long i = long(r) * nx + c - sh;
     const HD<T> me = hd_load_lane<T, FAST>(in, m0, i);
     T acc = T(0), P[3], M[3];
     if (c + 1 < nx) {
@@ -440,19 +443,20 @@ struct HCarry {
     T h1_n;           // updated depth of the row above (once this wavefront updated it)
 };
 
-// The carry at row r0: row r0 derived, and the edge (r0-1 | r0).
-template <typename T, bool FAST>
-__device__ __forceinline__ HCarry<T> hcarry_init(const F<T> &in, const uint8_t *m0, int r0, int nx, unsigned ucc,
-                                                 T dx) {
+// The carry at a first row (offset ro): that row derived, and the edge from
+// the row above it (offset ro_up) when there is one.
+template <typename T, bool FAST, typename RO>
+__device__ __forceinline__ HCarry<T> hcarry_at(const F<T> &in, const uint8_t *m0, RO ro, RO ro_up, bool has_up,
+                                               unsigned ucc, T dx) {
     const T zero = T(0);
     HCarry<T> hc;
-    hc.cur = hd_load<T, FAST>(in, m0, long(r0) * nx, ucc);
+    hc.cur = hd_load<T, FAST>(in, m0, ro, ucc);
     hc.pn0 = hc.pn1 = hc.pn2 = zero;
     hc.zr_n = zero;
     hc.nod_n = 0;
     hc.h1_n = zero;
-    if (r0 > 0) {
-        const HD<T> up = hd_load<T, FAST>(in, m0, long(r0 - 1) * nx, ucc);
+    if (has_up) {
+        const HD<T> up = hd_load<T, FAST>(in, m0, ro_up, ucc);
         if (hd_active(up, hc.cur)) {
             T P[3], M[3];
             rx_edge_d<1, T, FAST>(up, hc.cur, dx, P, M);
@@ -464,6 +468,13 @@ __device__ __forceinline__ HCarry<T> hcarry_init(const F<T> &in, const uint8_t *
     return hc;
 }
 
+// The carry at row r0 (absolute rows).
+template <typename T, bool FAST>
+__device__ __forceinline__ HCarry<T> hcarry_init(const F<T> &in, const uint8_t *m0, int r0, int nx, unsigned ucc,
+                                                 T dx) {
+    return hcarry_at<T, FAST>(in, m0, long(r0) * nx, long(r0 > 0 ? r0 - 1 : 0) * nx, r0 > 0, ucc, dx);
+}
+
 // roe_solve_2D (roe_exner.py:535-612) and momentum_corrections (:14-62) of row
 // r, with nxt the derived row r+1; returns the updated h, hu, hv and moves the
 // carry to row r+1.  mc: this lane's cell takes the momentum corrections (its
@@ -473,7 +484,7 @@ __device__ __forceinline__ HCarry<T> hcarry_init(const F<T> &in, const uint8_t *
 template <typename T, bool FAST>
 __device__ __forceinline__ void hydro_row(HCarry<T> &hc, const HD<T> &nxt, int r, int rfirst, int c, bool mc,
                                           bool inE, bool inW, int ny, int nx, T dx, T dt, T dtdx, const F<T> &in,
-                                          const uint8_t *m0, T &h1, T &hu1, T &hv1) {
+                                          const uint8_t *m0, long sh, T &h1, T &hu1, T &hv1) {
     const T T_ = C<T>(DRY_TOL), zero = T(0);
     const HD<T> &cur = hc.cur;
     const bool hasS = r + 1 < ny;
@@ -529,7 +540,7 @@ __device__ __forceinline__ void hydro_row(HCarry<T> &hc, const HD<T> &nxt, int r
                 far &= far - 1;
                 const int rr = r + (k == 2 ? -1 : (k == 3 ? 1 : 0));
                 const int cc = c + (k == 0 ? 1 : (k == 1 ? -1 : 0));
-                if (h1_at<T, FAST>(in, m0, dt, ny, nx, dx, rr, cc) < T_) {
+                if (h1_at<T, FAST>(in, m0, sh, dt, ny, nx, dx, rr, cc) < T_) {
                     if (k < 2) {
                         su = true;
                         far &= ~3;
@@ -573,7 +584,7 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_hydro_strip(F<T> in, F<T> out, 
         q1 = q2;
         if (r + 3 <= rlast) q2 = hr_load(in, m0, long(r + 3) * nx, ucc);
         T h1, hu1, hv1;
-        hydro_row<T, FAST>(hc, nxt, r, p.r0, c, mc, inE, inW, ny, nx, dx, dt, dtdx, in, m0, h1, hu1, hv1);
+        hydro_row<T, FAST>(hc, nxt, r, p.r0, c, mc, inE, inW, ny, nx, dx, dt, dtdx, in, m0, 0L, h1, hu1, hv1);
         if (p.out) {
             const long ro = long(r) * nx;
             str(out.h, ro, unsigned(c), h1);
@@ -596,28 +607,37 @@ struct ED {          // derived per-cell values for the Exner edge
     int flags;       // bit0 wet, bit1 nodata
 };
 
-// compute_G (roe_exner.py:85-130) in float32 hardware arithmetic:
-// theta = (A/B) vel2 h^(-1/3), gamma1 = (C/D) h^(-1/2), theta^1.5 = theta sqrt(theta).
-__device__ __forceinline__ float mpm_G_fast(float h, float hu, float hv, bool nodata, const pxh_sediments &sd) {
+// compute_G (roe_exner.py:85-130) in float32 hardware arithmetic from the
+// velocities the bed sweep derives anyway (vel2 = u^2 + v^2 of a wet cell):
+// theta = (A/B) vel2 h^(-1/3), theta^1.5 = theta sqrt(theta), and the folded
+// weight fw = 8 frac (C/D) / (1 - porosity) in place of gamma1 and the
+// porosity divide.  One sediment class (the common case) takes a straight
+// path whose three constants stay in scalar registers.
+__device__ __forceinline__ float mpm_G_term(float vel2, float icb, float irs, float AB, float thc, float w) {
     using Mf = Mth<float, true>;
-    const float T_ = float(DRY_TOL), zero = 0.0f;
-    h = pmax(h, T_);
-    const float ih = Mf::rcp(h);
-    const float u = hu * ih, v = hv * ih;
-    const float vel2 = u * u + v * v;
-    const float icb = Mf::exp2(-0.33333333333333333f * Mf::log2(h));
-    const float irs = Mf::rsq(h);
-    float G = zero;
-    for (int k = 0; k < sd.n_sed; ++k) {
-        float th = float(sd.AB[k]) * vel2 * icb;
-        th = pmax(th, zero);
-        const float dth = pmax(th - float(sd.theta_c[k]), zero);
-        const float g1 = float(sd.CD[k]) * irs;
-        const float g2 = 8.0f * Mf::sqrt(dth) * Mf::rcp(th * Mf::sqrt(th) + 1e-5f);
-        G = G + float(sd.frac[k]) * g1 * g2 * dth;
+    float th = AB * vel2 * icb;
+    th = pmax(th, 0.0f);
+    const float dth = pmax(th - thc, 0.0f);
+    const float g2 = Mf::sqrt(dth) * Mf::rcp(th * Mf::sqrt(th) + 1e-5f);
+    return w * irs * g2 * dth;
+}
+
+// G is exactly zero where the cell is dry, nodata or still (vel2 == 0 gives
+// theta = 0 and so dtheta = 0 in every term), so the terms are evaluated only
+// where vel2 != 0 (a NaN vel2 still propagates through them).
+__device__ __forceinline__ float mpm_G_fast(float h, float vel2, bool nodata, const SedK &sd) {
+    using Mf = Mth<float, true>;
+    const float T_ = float(DRY_TOL);
+    float G = 0.0f;
+    if (vel2 != 0.0f && h > T_ && !nodata) {
+        const float icb = Mf::exp2(-0.33333333333333333f * Mf::log2(h));
+        const float irs = Mf::rsq(h);
+        if (sd.n_sed == 1) {
+            G = mpm_G_term(vel2, icb, irs, sd.fAB[0], sd.fthc[0], sd.fw[0]);
+        } else {
+            for (int k = 0; k < sd.n_sed; ++k) G = G + mpm_G_term(vel2, icb, irs, sd.fAB[k], sd.fthc[k], sd.fw[k]);
+        }
     }
-    G = G * Mf::rcp(float(sd.one_minus_por));
-    if (nodata || !(h > T_)) G = zero;
     return G;
 }
 
@@ -648,8 +668,8 @@ struct OR {          // previous-step values of one cell (SSPRK2 combine)
     T h, hu, hv, G, zb, zl;
 };
 
-template <typename T>
-__device__ __forceinline__ OR<T> or_load(const F<T> &old, long ro, unsigned c) {
+template <typename T, typename RO>
+__device__ __forceinline__ OR<T> or_load(const F<T> &old, RO ro, unsigned c) {
     OR<T> o;
     o.h = ldr(old.h, ro, c); o.hu = ldr(old.hu, ro, c); o.hv = ldr(old.hv, ro, c);
     o.G = ldr(old.G, ro, c); o.zb = ldr(old.z_b, ro, c);
@@ -677,7 +697,7 @@ __device__ __forceinline__ CB<T> cb_make(const OR<T> &o, T h, T hu, T hv, T G) {
 }
 
 template <typename T, bool FAST>
-__device__ __forceinline__ ED<T> ed_derive(const ER<T> &q, const pxh_sediments &sd) {
+__device__ __forceinline__ ED<T> ed_derive(const ER<T> &q, const SedK &sd) {
     const T T_ = C<T>(DRY_TOL), S_ = C<T>(SED_TOL), zero = T(0);
     ED<T> d;
     d.h = q.h; d.hu = q.hu; d.hv = q.hv;
@@ -686,11 +706,11 @@ __device__ __forceinline__ ED<T> ed_derive(const ER<T> &q, const pxh_sediments &
     const T hf = wet ? d.h : C<T>(DRY_FLOOR);
     if constexpr (FAST) {
         using Mf = Mth<float, true>;
-        d.G = sd.mpm ? mpm_G_fast(d.h, d.hu, d.hv, nod, sd) : const_G<T>(d.h, q.G, nod);
         const float ih = Mf::rcp(hf);
         d.u = wet ? d.hu * ih : zero;
         d.v = wet ? d.hv * ih : zero;
         d.s = Mf::sqrt(hf);
+        d.G = sd.mpm ? mpm_G_fast(d.h, d.u * d.u + d.v * d.v, nod, sd) : const_G<T>(d.h, q.G, nod);
     } else {
         d.G = sd.mpm ? mpm_G<T>(d.h, d.hu, d.hv, nod, sd) : const_G<T>(d.h, q.G, nod);
         d.u = wet ? d.hu / hf : zero;
@@ -706,7 +726,7 @@ __device__ __forceinline__ ED<T> ed_derive(const ER<T> &q, const pxh_sediments &
 
 template <typename T, bool FAST>
 __device__ __forceinline__ ED<T> ed_load(const F<T> &hyd, const T *zb_in, const T *zb_lo_in, const T *G_in,
-                                         const uint8_t *m0, const pxh_sediments &sd, long ro, unsigned c) {
+                                         const uint8_t *m0, const SedK &sd, long ro, unsigned c) {
     return ed_derive<T, FAST>(er_load(hyd, zb_in, zb_lo_in, G_in, m0, sd, ro, c), sd);
 }
 
@@ -936,7 +956,7 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
                                                            const T *__restrict__ zb_lo_in, const T *__restrict__ G_in,
                                                            F<T> out, F<T> old, const uint8_t *__restrict__ m0,
                                                            const T *__restrict__ dtp, int ny, int nx, T dx,
-                                                           pxh_sediments sd, int write_G, T *__restrict__ partial) {
+                                                           SedK sd, int write_G, T *__restrict__ partial) {
     __shared__ T red[SWPB];
     const StripPos p = strip_pos<SO_WIDE, SR_BED>(ny, nx);
     const T zero = T(0), hf = T(0.5);
