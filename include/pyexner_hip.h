@@ -19,6 +19,7 @@
  *   compute_dt_fn "Roe"    solvers/roe_solver.py:56-60         -> pxh_roe_dt
  *   BoundaryManager.apply  domain/boundary_registry.py:183-187 -> pxh_bc_apply
  *   SSPRK2 combine          integrators/SSPRK2.py:40            -> pxh_ssprk2_combine
+ *   make_halo_exchange      solvers/kernels/roe_exner.py:719-803 -> pxh_halo_pack / pxh_halo_unpack
  */
 #ifndef PYEXNER_HIP_H
 #define PYEXNER_HIP_H
@@ -242,6 +243,30 @@ int pxh_roe_dt(const pxh_grid *grid, const pxh_fields *s, double cfl, void *dt_o
 /* ---- boundaries / halos -------------------------------------------------- */
 /* Apply the composed boundary program in place (groups bc->group_mask). */
 int pxh_bc_apply(const pxh_grid *grid, const pxh_fields *s, const pxh_bc *bc, void *stream);
+
+/* Halo strips of a subdomain tile: make_halo_exchange (solvers/kernels/roe_exner.py:719-803),
+ * with the MPI sendrecv left to the caller.  The tile is the grid: its owned
+ * block plus hw ghost rings (hw = 1 or 2).  pack copies the send strips (the hw
+ * owned rows/columns next to each selected side) of the fields of `groups`
+ * (bits 1 << PXH_G_*; PXH_G_ZB carries z_b_lo too when present) into buf;
+ * unpack copies buf into the receive strips (the hw ghost rows/columns).  Depth
+ * 0 is the row/column nearest the shared boundary on both tiles, so the buffer
+ * a neighbour packed for its east side unpacks as it is into this tile's west
+ * side, and so on.  West/east strips span every row, north/south strips every
+ * column: exchange west/east first, unpack, then pack north/south, and the
+ * corner blocks travel with the second phase.  One call handles one phase:
+ * `sides` is a subset of west|east or of north|south.  Layout of buf: per field (PXH_G_*
+ * order, z_b_lo after z_b), per selected side (west, north, east, south), hw
+ * strips depth-major; pxh_halo_bytes gives its size. */
+#define PXH_HALO_W 1
+#define PXH_HALO_N 2
+#define PXH_HALO_E 4
+#define PXH_HALO_S 8
+int64_t pxh_halo_bytes(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t groups, int sides);
+int pxh_halo_pack(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t groups, int sides, void *buf,
+                  void *stream);
+int pxh_halo_unpack(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t groups, int sides,
+                    const void *buf, void *stream);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ BC_COPY, BC_REFLECT_RX, BC_REFLECT_ROE = 0, 1, 2
 BC_FRAME = 1  # pxh_bc.flags: every target on the outer ring, every source within two cells of the edge
 BC_GROUPED = 2  # pxh_bc.flags: entries ordered by group, group_start[] valid
 N_GROUPS = 7
+HALO_W, HALO_N, HALO_E, HALO_S = 1, 2, 4, 8  # pxh_halo_* sides
 SWEEP_SKIP_G = 1  # flags of the stage/frame sweeps: leave out.G unwritten in MPM mode
 G_H, G_HUV, G_Z, G_ZB, G_N, G_NB, G_G = range(7)
 GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G_N: ("n",),
@@ -96,6 +97,11 @@ _SIGS = {
     "pxh_roe_exner_frame_sweep": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), _P(Fields),
                                                  ctypes.c_void_p, _P(Sediments), ctypes.c_void_p, ctypes.c_int,
                                                  ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_halo_bytes": (ctypes.c_int64, [_P(Grid), _P(Fields), ctypes.c_int, ctypes.c_uint32, ctypes.c_int]),
+    "pxh_halo_pack": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_halo_unpack": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_roe_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
     "pxh_momentum_corrections": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),

@@ -165,6 +165,11 @@ STATIC_GROUPS = (G.G_Z, G.G_N, G.G_NB)
 # ---------------------------------------------------------------------------
 # composed program
 # ---------------------------------------------------------------------------
+# pxh_bc_entry (16 bytes)
+ENTRY_DTYPE = np.dtype([("target", "<i4"), ("src", "<i4"), ("group", "u1"), ("nchain", "u1"),
+                        ("chain", "u1", (_hip.MAX_CHAIN,))])
+
+
 def compose(handlers, shape):
     """Resolve the sequential handlers into per-target (src, chain) entries.
 
@@ -200,8 +205,7 @@ def compose(handlers, shape):
             if len(ch) > _hip.MAX_CHAIN:
                 raise NotImplementedError(f"boundary chain of {len(ch)} reflections exceeds {_hip.MAX_CHAIN}")
             rows.append((t, s, grp, ch))
-    ent = np.zeros(len(rows), dtype=np.dtype([("target", "<i4"), ("src", "<i4"), ("group", "u1"),
-                                              ("nchain", "u1"), ("chain", "u1", (_hip.MAX_CHAIN,))]))
+    ent = np.zeros(len(rows), dtype=ENTRY_DTYPE)
     for q, (t, s, grp, ch) in enumerate(rows):
         ent[q]["target"], ent[q]["src"], ent[q]["group"], ent[q]["nchain"] = t, s, grp, len(ch)
         ent[q]["chain"][:len(ch)] = ch
@@ -250,6 +254,7 @@ class BoundaryManager:
         if len(self.boundary_handlers) > _hip.MAX_HANDLERS:
             raise NotImplementedError(f"more than {_hip.MAX_HANDLERS} boundary handlers")
         entries, self.idempotent = compose(self.boundary_handlers, shape)
+        self.entries = entries
         self.n_entries = len(entries)
         self.frame_local = frame_local(entries, shape)
         self.device = device

@@ -13,6 +13,7 @@
 #include "flux_roe_exner.hpp"
 #include "strip.hpp"
 #include "stage.hpp"
+#include "halo.hpp"
 
 namespace pxh {
 
@@ -583,6 +584,16 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
     return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
 }
 
+// The seam edges of a two-sweep combine (k_rx_seam_dt) into part[0, seam_blocks).
+template <typename T, bool FAST>
+static int seam_dt(const pxh_grid *g, const F<T> &s, const uint8_t *m0, T *part, hipStream_t st) {
+    const int ns = seam_count(g->ny);
+    if (ns == 0) return PXH_OK;
+    hipLaunchKernelGGL((k_rx_seam_dt<T, FAST>), dim3(seam_blocks(g->ny, g->nx)), dim3(256), 0, st, s, m0, g->ny,
+                       g->nx, static_cast<T>(g->dx), ns, part);
+    return launch_status();
+}
+
 // One SSPRK2 step (integrators/SSPRK2.py:32-40) on three buffer sets:
 // dt from a (unless dt_ready); b = S(a); c = S(b) with the combine
 // a = 0.5 a + 0.5 c fused into the second bed sweep, which also reduces the
@@ -636,8 +647,10 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), bg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny,
                        nx, dx, sed_k(*sd), 1, part);
     if (int e = launch_status()) return e;
+    if (int e = seam_dt<T, FAST>(g, a, m0, part + nb, st)) return e;
     if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nb, static_cast<T>(cfl), 1, dtp);
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nb + seam_blocks(ny, nx),
+                       static_cast<T>(cfl), 1, dtp);
     return launch_status();
 }
 
@@ -665,10 +678,12 @@ static int rx_bed_sweep_t(const pxh_grid *g, const pxh_fields *hyd_, const pxh_f
                            out, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr);
     } else {
         F<T> old = cast_fields<T>(old_);
-        if (partial)
+        if (partial) {
             hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
                                out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, static_cast<T *>(partial));
-        else
+            if (int e = launch_status()) return e;
+            return seam_dt<T, FAST>(g, out, m0, static_cast<T *>(partial) + sg.x * sg.y, st);
+        } else
             hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, false>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
                                out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr);
     }
@@ -836,6 +851,58 @@ static int part_args_ok(const pxh_grid *g, const pxh_fields *in, const pxh_field
     return grid_ok(g) && fields_ok(in, true) && fields_ok(out, true) && m0;
 }
 
+// The halo plan of a call: the selected fields (PXH_G_* order, z_b_lo after
+// z_b) and sides; false when the arguments do not describe one.
+template <typename T>
+static bool halo_plan(const pxh_grid *g, const pxh_fields *s_, int hw, uint32_t groups, int sides, HaloPlan &plan,
+                      HaloFields<T> &hf) {
+    if (hw < 1 || hw > 2 || g->ny < 2 * hw + 1 || g->nx < 2 * hw + 1) return false;
+    // one phase per call: west/east, or north/south (their strips overlap in the corners)
+    const bool we = sides & (PXH_HALO_W | PXH_HALO_E), ns = sides & (PXH_HALO_N | PXH_HALO_S);
+    if (sides <= 0 || sides > 15 || (we && ns) || groups == 0 || (groups & ~0x7Fu)) return false;
+    const F<T> f = cast_fields<T>(s_);
+    T *list[HALO_MAX_FIELDS];
+    int n = 0;
+    auto add = [&](T *p) { list[n++] = p; };
+    if (groups & (1u << PXH_G_H)) add(f.h);
+    if (groups & (1u << PXH_G_HUV)) { add(f.hu); add(f.hv); }
+    if (groups & (1u << PXH_G_Z)) add(f.z);
+    if (groups & (1u << PXH_G_ZB)) { add(f.z_b); if (f.z_b_lo) add(f.z_b_lo); }
+    if (groups & (1u << PXH_G_N)) add(f.n);
+    if (groups & (1u << PXH_G_NB)) add(f.n_b);
+    if (groups & (1u << PXH_G_G)) add(f.G);
+    for (int k = 0; k < n; ++k)
+        if (!list[k]) return false;
+    plan.nf = n;
+    long so = 0;
+    for (int sd = 0; sd < 4; ++sd) {
+        plan.side_on[sd] = (sides >> sd) & 1;
+        plan.side_off[sd] = so;
+        if (plan.side_on[sd]) so += long(hw) * ((sd == 0 || sd == 2) ? g->ny : g->nx);
+    }
+    plan.side_off[4] = so;
+    for (int k = 0; k <= n; ++k) plan.off[k] = long(k) * so;
+    for (int k = 0; k < HALO_MAX_FIELDS; ++k) hf.p[k] = k < n ? list[k] : nullptr;
+    return true;
+}
+
+template <typename T>
+static int halo_t(bool unpack, const pxh_grid *g, const pxh_fields *s, int hw, uint32_t groups, int sides, void *buf,
+                  hipStream_t st) {
+    HaloPlan plan;
+    HaloFields<T> hf;
+    if (!halo_plan<T>(g, s, hw, groups, sides, plan, hf)) return PXH_E_ARG;
+    const long total = plan.off[plan.nf];
+    const unsigned nb = unsigned(total / 256 + 1 < 4096 ? total / 256 + 1 : 4096);
+    if (unpack)
+        hipLaunchKernelGGL((k_halo<T, true>), dim3(nb), dim3(256), 0, st, hf, plan, static_cast<T *>(buf), g->ny,
+                           g->nx, hw);
+    else
+        hipLaunchKernelGGL((k_halo<T, false>), dim3(nb), dim3(256), 0, st, hf, plan, static_cast<T *>(buf), g->ny,
+                           g->nx, hw);
+    return launch_status();
+}
+
 }  // namespace pxh
 
 using namespace pxh;
@@ -932,7 +999,7 @@ int pxh_roe_exner_bed_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_
 int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void *dt_out, void *stream) {
     if (!grid_ok(g) || !dt_partial || !dt_out) return PXH_E_ARG;
     const dim3 sg = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx);  // the partials of pxh_roe_exner_bed_sweep
-    const int nb = int(sg.x * sg.y);
+    const int nb = int(sg.x * sg.y) + seam_blocks(g->ny, g->nx);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (g->dtype == PXH_F64)
         hipLaunchKernelGGL(k_dt_final<double>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const double *>(dt_partial),
@@ -992,7 +1059,7 @@ int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const px
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
     const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx);
-    int64_t nb = int64_t(sa.x) * sa.y;
+    int64_t nb = int64_t(sa.x) * sa.y + seam_blocks(g->ny, g->nx);
     const int64_t nfused = int64_t(fg.x) * fg.y + (frame_cells(g->ny, g->nx) + 255) / 256;
     if (nfused > nb) nb = nfused;
     return (nb > DT_MAX_BLOCKS ? nb : DT_MAX_BLOCKS) * 8;
@@ -1055,6 +1122,32 @@ int pxh_bc_apply(const pxh_grid *g, const pxh_fields *s, const pxh_bc *bc, void 
     if (!grid_ok(g) || !s || !bc) return PXH_E_ARG;
     if (!bc_ok(bc)) return PXH_E_ARG;
     return DISPATCH(g, bc_apply_t, g, s, bc, static_cast<hipStream_t>(stream));
+}
+
+int64_t pxh_halo_bytes(const pxh_grid *g, const pxh_fields *s, int hw, uint32_t groups, int sides) {
+    if (!grid_ok(g) || !s) return PXH_E_ARG;
+    HaloPlan plan;
+    if (g->dtype == PXH_F64) {
+        HaloFields<double> hf;
+        if (!halo_plan<double>(g, s, hw, groups, sides, plan, hf)) return PXH_E_ARG;
+        return plan.off[plan.nf] * 8;
+    }
+    HaloFields<float> hf;
+    if (!halo_plan<float>(g, s, hw, groups, sides, plan, hf)) return PXH_E_ARG;
+    return plan.off[plan.nf] * 4;
+}
+
+int pxh_halo_pack(const pxh_grid *g, const pxh_fields *s, int hw, uint32_t groups, int sides, void *buf,
+                  void *stream) {
+    if (!grid_ok(g) || !s || !buf) return PXH_E_ARG;
+    return DISPATCH(g, halo_t, false, g, s, hw, groups, sides, buf, static_cast<hipStream_t>(stream));
+}
+
+int pxh_halo_unpack(const pxh_grid *g, const pxh_fields *s, int hw, uint32_t groups, int sides, const void *buf,
+                    void *stream) {
+    if (!grid_ok(g) || !s || !buf) return PXH_E_ARG;
+    return DISPATCH(g, halo_t, true, g, s, hw, groups, sides, const_cast<void *>(buf),
+                    static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -406,6 +406,36 @@ __device__ __forceinline__ DD<T> dd_at(const F<T> &s, const uint8_t *m0, long i)
     return dd_make<T, FAST>(s.h[i], s.hu[i], s.hv[i], s.G[i], m0[i]);
 }
 
+// The y edges (r-1 | r) at the first row r of every two-sweep bed strip but
+// the first, from the stored combined state: the strip above cannot take row
+// r's combined value from `old` while the strip below combines row r in place
+// (it may already hold the result), so those edges are reduced here, after the
+// combine.  nseam = (ny - 1) / SR_BED seams; one partial per block.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_rx_seam_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
+                                                     int nseam, T *__restrict__ partial) {
+    __shared__ T red[4];
+    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    T v = inf_v<T>();
+    if (t < long(nseam) * nx) {
+        const int k = int(t / nx), c = int(t - long(k) * nx);
+        const long i = long(k + 1) * SR_BED * nx + c;
+        const DD<T> n = dd_at<T, FAST>(s, m0, i - nx), me = dd_at<T, FAST>(s, m0, i);
+        if (dd_active(n, me)) v = dt_edge_d<1, T, FAST>(n, me, dx);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) v = pmin(v, red[k]);
+        partial[blockIdx.x] = v;
+    }
+}
+
+static inline int seam_count(int ny) { return (ny - 1) / SR_BED; }
+static inline int seam_blocks(int ny, int nx) { return int((long(seam_count(ny)) * nx + 255) / 256); }
+
 // CFL minimum over every edge with a frame cell on either side (an edge
 // between two frame cells may be evaluated twice; the minimum does not mind),
 // of the combined state s; one partial per block.

@@ -1,0 +1,216 @@
+"""A parNy x parNx tile decomposition of one Roe-Exner domain on one device.
+
+The reference decomposes a domain over MPI ranks (`parallel/mpi_handler.py`),
+gives every rank a tile with a one-cell ghost ring, and swaps the ring with
+`make_halo_exchange` (solvers/kernels/roe_exner.py:719-803) inside the step
+(solvers/roe_exner_solver.py:92-110).  Here each tile keeps its owned block
+plus two ghost rings on every side it shares with another tile (none on the
+sides of the physical boundary), the strips are packed and unpacked on the
+device with `pxh_halo_pack` / `pxh_halo_unpack`, and the sendrecv between
+tiles is a device copy (on a multi-GPU machine it would be the RCCL
+send/recv of the same buffers).
+
+Two ghost rings make the tiled run equal the single-domain run cell for cell:
+the momentum corrections of an owned cell next to the tile edge read the
+updated depth of a ghost cell, whose own update needs one more ring.  (With
+the reference's single ring that depth is computed from an incomplete
+stencil.)  The step therefore runs the two-sweep stage with an exchange after
+each sweep, and the dt from every tile's own minimum, reduced over tiles:
+
+    dt   = min over tiles of compute_dt(a)            (the MIN allreduce)
+    b    = hydro(a); fill(b: h, hu, hv); exchange(b: h, hu, hv)
+    b    = bed(b, a); fill(b: z_b, G); exchange(b: z_b, G)
+    c    = hydro(b); fill(c: h, hu, hv); exchange(c: h, hu, hv)
+    a    = 0.5 a + 0.5 bed(c, b); fill(a: z_b, G); exchange(a: z_b, G)
+"""
+import numpy as np
+import torch
+
+from . import _hip
+from .boundaries import ENTRY_DTYPE
+from .state import fields_struct
+
+HW = 2  # ghost rings on a shared side
+_SIDES = (_hip.HALO_W, _hip.HALO_N, _hip.HALO_E, _hip.HALO_S)
+_OPPOSITE = {_hip.HALO_W: _hip.HALO_E, _hip.HALO_E: _hip.HALO_W, _hip.HALO_N: _hip.HALO_S,
+             _hip.HALO_S: _hip.HALO_N}
+_STEP = (1 << _hip.G_H) | (1 << _hip.G_HUV) | (1 << _hip.G_ZB) | (1 << _hip.G_G)
+_HYDRO = (1 << _hip.G_H) | (1 << _hip.G_HUV)
+_BED = (1 << _hip.G_ZB) | (1 << _hip.G_G)
+
+
+def split(n, parts):
+    """Owned extents [start, stop) of `parts` near-equal bands of n cells."""
+    edges = np.linspace(0, n, parts + 1).round().astype(int)
+    return [(int(edges[k]), int(edges[k + 1])) for k in range(parts)]
+
+
+def _like(state, names):
+    kw = {n: torch.empty_like(getattr(state, n)) for n in names}
+    return state.replace(**kw)
+
+
+def _tile_program(entries, shape, owned, ext):
+    """The global ring-fill program restricted to the fills whose target this
+    tile owns, in the tile's local indices (sources must lie in the tile)."""
+    ny, nx = shape
+    r0, r1, c0, c1 = owned
+    e0, _, f0, _ = ext
+    tny, tnx = ext[1] - ext[0], ext[3] - ext[2]
+    t = entries["target"].astype(np.int64)
+    s = entries["src"].astype(np.int64)
+    tr, tc, sr, sc = t // nx, t % nx, s // nx, s % nx
+    keep = (tr >= r0) & (tr < r1) & (tc >= c0) & (tc < c1)
+    sub = entries[keep].copy()
+    srl, scl = sr[keep] - e0, sc[keep] - f0
+    if np.any((srl < 0) | (srl >= tny) | (scl < 0) | (scl >= tnx)):
+        raise NotImplementedError("a ring fill reads a cell outside its tile (periodic rings do not tile)")
+    sub["target"] = ((tr[keep] - e0) * tnx + (tc[keep] - f0)).astype(np.int32)
+    sub["src"] = (srl * tnx + scl).astype(np.int32)
+    grp = sub["group"].astype(np.int64)
+    start = [int(v) for v in np.searchsorted(grp, np.arange(_hip.N_GROUPS + 1))]
+    return sub, start
+
+
+class Tile:
+    """One tile: its extent in the global arrays, three buffer sets, grid,
+    mask, ring-fill program and dt workspace."""
+
+    def __init__(self, backend, state, mask, entries, handlers, pos, par, rows, cols, device):
+        ty, tx = pos
+        pny, pnx = par
+        r0, r1 = rows[ty]
+        c0, c1 = cols[tx]
+        self.pos = pos
+        self.ring = {_hip.HALO_N: HW if ty > 0 else 0, _hip.HALO_S: HW if ty < pny - 1 else 0,
+                     _hip.HALO_W: HW if tx > 0 else 0, _hip.HALO_E: HW if tx < pnx - 1 else 0}
+        self.owned = (r0, r1, c0, c1)
+        self.ext = (r0 - self.ring[_hip.HALO_N], r1 + self.ring[_hip.HALO_S],
+                    c0 - self.ring[_hip.HALO_W], c1 + self.ring[_hip.HALO_E])
+        e0, e1, f0, f1 = self.ext
+        if min(r1 - r0, c1 - c0) < 2 * HW + 1:
+            raise ValueError(f"tile {pos} owns {r1 - r0}x{c1 - c0} cells: too small for {HW} ghost rings")
+        names = [n for n in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G", "z_b_lo") if getattr(state, n) is not None]
+        self.a = state.replace(**{n: getattr(state, n)[e0:e1, f0:f1].contiguous() for n in names})
+        evolving = ("h", "hu", "hv", "z_b", "G") + (("z_b_lo",) if state.z_b_lo is not None else ())
+        self.b = _like(self.a, evolving)
+        self.c = _like(self.a, evolving)
+        self.fa, self.fb, self.fc = (fields_struct(x) for x in (self.a, self.b, self.c))
+        self.mask = mask[:, e0:e1, f0:f1].contiguous()
+        self.m0 = self.mask[0].data_ptr()
+        g = backend.grid
+        self.grid = _hip.Grid(e1 - e0, f1 - f0, g.dtype, g.flags, g.dx)
+        sub, start = _tile_program(entries, state.h.shape, self.owned, self.ext)
+        self.n_entries = len(sub)
+        self._entries = torch.from_numpy(sub.view(np.uint8).copy()).to(device) if len(sub) else None
+        self.programs = {}
+        for key, groups in (("hy", _HYDRO), ("bed1", (1 << _hip.G_ZB) | (0 if backend.sed.mpm else 1 << _hip.G_G)),
+                            ("bed", _BED)):
+            bc = _hip.BC()
+            bc.entries = self._entries.data_ptr() if self._entries is not None else None
+            bc.n_entries = self.n_entries
+            bc.n_handlers = len(handlers)
+            bc.group_mask = groups
+            bc.flags = _hip.BC_GROUPED
+            for k in range(_hip.N_GROUPS + 1):
+                bc.group_start[k] = start[k]
+            for k, (kind, nxn, nyn) in enumerate(handlers):
+                bc.handlers[k].kind, bc.handlers[k].nx, bc.handlers[k].ny = kind, nxn, nyn
+            self.programs[key] = bc
+        ws = _hip.load().pxh_dt_workspace_size(self.grid)
+        self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
+        self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
+        lib = _hip.load()
+        self.send = {}
+        for side in _SIDES:
+            if self.ring[side]:
+                nbytes = lib.pxh_halo_bytes(self.grid, self.fa, HW, _STEP, side)
+                if nbytes < 0:
+                    raise ValueError("invalid halo geometry")
+                self.send[side] = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+
+    def fields(self, which):
+        return {"a": self.fa, "b": self.fb, "c": self.fc}[which]
+
+
+class TiledExner:
+    """Roe-Exner SSPRK2 on a par = (parNy, parNx) decomposition of an
+    initialised single-domain state (init_fn applied), every tile on this device."""
+
+    def __init__(self, backend, state, mask, boundaries, par, cfl):
+        self.be = backend
+        self.cfl = float(cfl)
+        self.par = tuple(par)
+        pny, pnx = self.par
+        ny, nx = state.h.shape
+        self.rows, self.cols = split(ny, pny), split(nx, pnx)
+        entries = boundaries.entries if boundaries is not None else np.zeros(0, dtype=ENTRY_DTYPE)
+        handlers = [(hd.kind, float(hd.normal[0]), float(hd.normal[1])) for hd in
+                    (boundaries.boundary_handlers if boundaries is not None else [])]
+        device = state.h.device
+        self.tiles = {(ty, tx): Tile(backend, state, mask, entries, handlers, (ty, tx), self.par, self.rows,
+                                     self.cols, device) for ty in range(pny) for tx in range(pnx)}
+        self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
+
+    def _neighbour(self, pos, side):
+        ty, tx = pos
+        d = {_hip.HALO_W: (0, -1), _hip.HALO_E: (0, 1), _hip.HALO_N: (-1, 0), _hip.HALO_S: (1, 0)}[side]
+        return self.tiles[(ty + d[0], tx + d[1])]
+
+    def exchange(self, which, groups):
+        """Swap the ghost strips of buffer set `which` ("a", "b", "c") for the
+        field groups `groups`: west/east first, then north/south (corners)."""
+        st = torch.cuda.current_stream().cuda_stream
+        for phase in ((_hip.HALO_W, _hip.HALO_E), (_hip.HALO_N, _hip.HALO_S)):
+            for t in self.tiles.values():
+                for side in phase:
+                    if t.ring[side]:
+                        _hip.call("pxh_halo_pack", t.grid, t.fields(which), HW, groups, side,
+                                  t.send[side].data_ptr(), st)
+            for t in self.tiles.values():
+                for side in phase:
+                    if t.ring[side]:
+                        src = self._neighbour(t.pos, side).send[_OPPOSITE[side]]
+                        _hip.call("pxh_halo_unpack", t.grid, t.fields(which), HW, groups, side, src.data_ptr(), st)
+
+    def compute_dt(self):
+        st = torch.cuda.current_stream().cuda_stream
+        for t in self.tiles.values():
+            _hip.call("pxh_roe_exner_dt", t.grid, t.fa, t.m0, self.cfl, t.dt.data_ptr(), t.workspace.data_ptr(), st)
+        self.dt.copy_(torch.stack([t.dt for t in self.tiles.values()]).min())
+        return self.dt
+
+    def _sweeps(self, hyd_in, hyd_out, bed_in, old, fill_bed):
+        st = torch.cuda.current_stream().cuda_stream
+        dtp = self.dt.data_ptr()
+        for t in self.tiles.values():
+            _hip.call("pxh_roe_exner_hydro_sweep", t.grid, t.fields(hyd_in), t.fields(hyd_out), t.m0, dtp, st)
+            if t.n_entries:
+                _hip.call("pxh_bc_apply", t.grid, t.fields(hyd_out), t.programs["hy"], st)
+        self.exchange(hyd_out, _HYDRO)
+        out = old if old is not None else hyd_out
+        for t in self.tiles.values():
+            _hip.call("pxh_roe_exner_bed_sweep", t.grid, t.fields(hyd_out), t.fields(bed_in), t.fields(out),
+                      t.fields(old) if old is not None else None, t.m0, self.be.sed, dtp, None, st)
+            if t.n_entries:
+                _hip.call("pxh_bc_apply", t.grid, t.fields(out), t.programs[fill_bed], st)
+        self.exchange(out, _BED)
+
+    def ssprk2_step(self):
+        """One SSPRK2 step (integrators/SSPRK2.py:32-40) of every tile."""
+        self.compute_dt()
+        self._sweeps("a", "b", "a", None, "bed1")
+        self._sweeps("b", "c", "b", "a", "bed")
+
+    def gather(self, name):
+        """The owned cells of every tile, as one global array."""
+        parts = []
+        for ty in range(self.par[0]):
+            row = []
+            for tx in range(self.par[1]):
+                t = self.tiles[(ty, tx)]
+                r0, r1, c0, c1 = t.owned
+                e0, _, f0, _ = t.ext
+                row.append(getattr(t.a, name)[r0 - e0:r1 - e0, c0 - f0:c1 - f0])
+            parts.append(torch.cat(row, dim=1))
+        return torch.cat(parts, dim=0)

@@ -340,3 +340,22 @@ def test_stage_and_frame_sweeps_equal_fused_ssprk2(cuda, dtype, grass):
         for f in FIELDS_EXNER + (("z_b_lo",) if ref.state.z_b_lo is not None else ()):
             assert torch.equal(getattr(ref.state, f), getattr(seq.state, f)), (k, f)
     assert float(ref.dt) > 0
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_next_dt_of_ssprk2_matches_oracle_f64(cuda, fused):
+    """The dt an SSPRK2 step leaves for the next step equals compute_dt_fn of
+    the new state (oracle), on the radial dam break whose fastest edges cross
+    the sweeps' strip seams.  (The two-sweep combine once took the seam edges
+    from `old` rows the strip below had already combined in place.)"""
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = cases.radial_dambreak(nx=60, ny=66, dx=0.05, n_bumps=6, radius=0.9)
+    params = dict(params, backend={"fused_stage": fused})
+    so, prob = oracle_exner(mesh, arr, params)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    st = ExnerStepper(sc.backend, sg, mask, prob.cfl)
+    for k in range(8):
+        so, _ = osteps.ssprk2_step(prob, so)
+        st.ssprk2_step()
+        want = float(prob.dt(so))
+        assert abs(float(st.dt) - want) <= 1e-13 * want, (k, float(st.dt), want)

@@ -1,0 +1,138 @@
+"""Tile decomposition on one GPU: halo strips through the C-ABI and tiled runs
+against the single-domain run.
+
+Bars (written here, checked below): the strips are bit-exact copies; a tiled
+SSPRK2 run equals the single-domain two-sweep run bit for bit in every owned
+cell and in dt, in float64 and float32."""
+import numpy as np
+import pytest
+import torch
+
+from pyexner_amd import _hip, cases
+from pyexner_amd.state import RoeExnerState, dtype_code, fields_struct
+from pyexner_amd.stepper import ExnerStepper
+from pyexner_amd.tiles import TiledExner
+from tests.helpers import FIELDS_EXNER, gpu_case
+
+pytestmark = pytest.mark.gpu
+
+GROUPS = ((1 << _hip.G_H) | (1 << _hip.G_HUV) | (1 << _hip.G_ZB) | (1 << _hip.G_G))
+FIELD_ORDER = ("h", "hu", "hv", "z_b", "z_b_lo", "G")
+
+
+def _state(ny, nx, dtype, lo, seed):
+    g = torch.Generator().manual_seed(seed)
+    mk = lambda: torch.rand((ny, nx), generator=g, dtype=torch.float64).to(dtype).cuda()
+    kw = {n: mk() for n in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")}
+    kw["z_b_lo"] = mk() if lo else None
+    kw["seds"] = torch.zeros((7, 1), dtype=dtype)
+    return RoeExnerState(**kw)
+
+
+def _cell(side, d, i, ny, nx, hw, recv):
+    """Where element (side, depth d, position i) of a strip lives (the layout
+    the header documents), written out independently of the kernel."""
+    if side == _hip.HALO_W:
+        return i, (hw - 1 - d) if recv else (hw + d)
+    if side == _hip.HALO_E:
+        return i, (nx - hw + d) if recv else (nx - hw - 1 - d)
+    if side == _hip.HALO_N:
+        return ((hw - 1 - d) if recv else (hw + d)), i
+    return ((ny - hw + d) if recv else (ny - hw - 1 - d)), i
+
+
+@pytest.mark.parametrize("dtype,lo", [(torch.float32, True), (torch.float64, False)])
+@pytest.mark.parametrize("hw", [1, 2])
+def test_halo_strips_pack_and_unpack(cuda, dtype, lo, hw):
+    ny, nx = 23, 31
+    st = _state(ny, nx, dtype, lo, seed=5)
+    grid = _hip.Grid(ny, nx, dtype_code(dtype), 0, 0.1)
+    fs = fields_struct(st)
+    names = [n for n in FIELD_ORDER if getattr(st, n) is not None]
+    stream = torch.cuda.current_stream().cuda_stream
+    lib = _hip.load()
+    for sides in (_hip.HALO_W, _hip.HALO_N, _hip.HALO_E, _hip.HALO_S, _hip.HALO_W | _hip.HALO_E,
+                  _hip.HALO_N | _hip.HALO_S):
+        nbytes = lib.pxh_halo_bytes(grid, fs, hw, GROUPS, sides)
+        sel = [s for s in (_hip.HALO_W, _hip.HALO_N, _hip.HALO_E, _hip.HALO_S) if sides & s]
+        per = sum(hw * (ny if s in (_hip.HALO_W, _hip.HALO_E) else nx) for s in sel)
+        assert nbytes == per * len(names) * st.h.element_size()
+        buf = torch.empty(nbytes // st.h.element_size(), dtype=dtype, device="cuda")
+        _hip.call("pxh_halo_pack", grid, fs, hw, GROUPS, sides, buf.data_ptr(), stream)
+        want = []
+        host = {n: getattr(st, n).cpu().numpy() for n in names}
+        for n in names:
+            for s in sel:
+                L = ny if s in (_hip.HALO_W, _hip.HALO_E) else nx
+                for d in range(hw):
+                    want.extend(host[n][_cell(s, d, i, ny, nx, hw, False)] for i in range(L))
+        assert np.array_equal(buf.cpu().numpy(), np.array(want, dtype=buf.cpu().numpy().dtype)), sides
+        # unpack a distinct buffer: exactly the receive cells change
+        src = torch.arange(buf.numel(), dtype=dtype, device="cuda") + 1000.0
+        before = {n: getattr(st, n).clone() for n in names}
+        _hip.call("pxh_halo_unpack", grid, fs, hw, GROUPS, sides, src.data_ptr(), stream)
+        torch.cuda.synchronize()
+        k = 0
+        s_host = src.cpu().numpy()
+        for n in names:
+            exp = before[n].cpu().numpy().copy()
+            for s in sel:
+                L = ny if s in (_hip.HALO_W, _hip.HALO_E) else nx
+                for d in range(hw):
+                    for i in range(L):
+                        exp[_cell(s, d, i, ny, nx, hw, True)] = s_host[k]
+                        k += 1
+            assert np.array_equal(getattr(st, n).cpu().numpy(), exp), (n, sides)
+            getattr(st, n).copy_(before[n])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("which,par", [("channel", (2, 2)), ("radial", (3, 2)), ("wetdry", (2, 3))])
+def test_tiled_run_equals_single_domain(cuda, dtype, which, par):
+    """float64: every owned cell of a tiled SSPRK2 run, and every dt, equal the
+    single-domain two-sweep run bit for bit.  float32 (hardware math, fp
+    contraction per inlined copy of the edge, so an edge evaluated at the start
+    of a strip can round differently from the same edge inside one, and tiles
+    start strips at other rows): the float32 parity bar, h, hu, G within 1e-5
+    of max |field|, hv within 1e-5 of max |hu|, the compensated bed within 1e-4
+    of max |z_b - z_b(0)|, dt within 1e-6."""
+    if which == "channel":
+        mesh, arr, params = cases.channel(nx=70, ny=40, n_bumps=8, seed=21)
+    elif which == "radial":
+        mesh, arr, params = cases.radial_dambreak(nx=60, ny=66, dx=0.05, n_bumps=6, radius=0.9)
+    else:
+        mesh, arr, params = cases.wetdry_obstacle(nx=75, ny=44)
+    params = dict(params, backend={"fused_stage": False})
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
+    mono = ExnerStepper(sc.backend, sg, mask, 0.5)
+    tiled = TiledExner(sc.backend, sg, mask, sc.boundaries, par, 0.5)
+    exact = dtype == torch.float64
+    names = FIELDS_EXNER + (("z_b_lo",) if sg.z_b_lo is not None else ())
+    zb0 = sg.z_b.double() + (sg.z_b_lo.double() if sg.z_b_lo is not None else 0)
+    for k in range(60):
+        if k == 0:
+            mono.compute_dt()
+        dt_used = mono.dt.clone()
+        mono.ssprk2_step()
+        tiled.ssprk2_step()
+        torch.cuda.synchronize()
+        if exact:
+            assert torch.equal(tiled.dt, dt_used), (k, float(tiled.dt), float(dt_used))
+        else:
+            assert abs(float(tiled.dt) - float(dt_used)) <= 1e-6 * float(dt_used), k
+    if exact:
+        for f in names:
+            got, want = tiled.gather(f), getattr(mono.state, f)
+            assert torch.equal(got, want), (f, float((got - want).abs().max()))
+    else:
+        for f in ("h", "hu", "hv", "G"):
+            got, want = tiled.gather(f).double(), getattr(mono.state, f).double()
+            ref = mono.state.hu if f == "hv" else getattr(mono.state, f)  # hv is a small cross-flow
+            scale = max(float(ref.double().abs().max()), 1e-30)
+            assert float((got - want).abs().max()) <= 1e-5 * scale, (f, float((got - want).abs().max()) / scale)
+        bed = lambda z, lo: z.double() + (lo.double() if lo is not None else 0)  # noqa: E731
+        got = bed(tiled.gather("z_b"), tiled.gather("z_b_lo") if sg.z_b_lo is not None else None)
+        want = bed(mono.state.z_b, mono.state.z_b_lo)
+        dz = max(float((want - zb0).abs().max()), 1e-30)
+        assert float((got - want).abs().max()) <= 1e-4 * dz
+    assert float(mono.state.h.sum()) > 0

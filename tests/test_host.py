@@ -246,3 +246,52 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         assert int(val) == want, (cname, what, int(val), want)
         seen += 1
     assert seen == sum(1 + len(c._fields_) for c in mirrors.values())
+
+
+def test_tile_programs_partition_the_ring_fill():
+    """Every fill of the single-domain program lands in exactly one tile, with
+    target and source at the same global cells in the tile's local indices."""
+    from pyexner_amd.tiles import HW, _tile_program, split
+    mesh, arr, params = cases.channel(nx=47, ny=31, n_bumps=2)
+    bm = BoundaryManager(params, mesh.local_X, mesh.local_Y, device="cpu")
+    ny, nx = mesh.local_X.shape
+    par = (2, 3)
+    rows, cols = split(ny, par[0]), split(nx, par[1])
+    assert rows[0][0] == 0 and rows[-1][1] == ny and cols[-1][1] == nx
+    seen = []
+    for ty in range(par[0]):
+        for tx in range(par[1]):
+            r0, r1 = rows[ty]
+            c0, c1 = cols[tx]
+            ext = (r0 - (HW if ty else 0), r1 + (HW if ty < par[0] - 1 else 0),
+                   c0 - (HW if tx else 0), c1 + (HW if tx < par[1] - 1 else 0))
+            sub, start = _tile_program(bm.entries, (ny, nx), (r0, r1, c0, c1), ext)
+            tnx = ext[3] - ext[2]
+            assert start[0] == 0 and start[-1] == len(sub)
+            for e in sub:
+                tr, tc = divmod(int(e["target"]), tnx)
+                sr, sc = divmod(int(e["src"]), tnx)
+                seen.append(((tr + ext[0]) * nx + tc + ext[2], (sr + ext[0]) * nx + sc + ext[2], int(e["group"])))
+    glob = sorted((int(e["target"]), int(e["src"]), int(e["group"])) for e in bm.entries)
+    assert sorted(seen) == glob
+
+
+def test_halo_sizes_and_rejected_geometry():
+    """pxh_halo_bytes (host-only: it reads no device memory): strip sizes, and
+    the calls the strips cannot serve."""
+    # stand-in addresses: the size query only checks which pointers are set
+    fs = _hip.Fields(*[0x1000 * (k + 1) for k in range(8)], None)
+    grid = _hip.Grid(8, 9, _hip.PXH_F32, 0, 0.1)
+    lib = _hip.load()
+    groups = (1 << _hip.G_H) | (1 << _hip.G_HUV) | (1 << _hip.G_ZB) | (1 << _hip.G_G)  # h hu hv z_b G
+    assert lib.pxh_halo_bytes(grid, fs, 2, groups, _hip.HALO_N | _hip.HALO_S) == 2 * 2 * 9 * 5 * 4
+    assert lib.pxh_halo_bytes(grid, fs, 1, groups, _hip.HALO_W) == 1 * 8 * 5 * 4
+    assert lib.pxh_halo_bytes(grid, fs, 3, groups, _hip.HALO_W) < 0               # hw beyond 2
+    assert lib.pxh_halo_bytes(grid, fs, 2, groups, 0) < 0                          # no side
+    assert lib.pxh_halo_bytes(grid, fs, 2, groups, 15) < 0                         # both phases in one call
+    assert lib.pxh_halo_bytes(grid, fs, 2, groups, _hip.HALO_W | _hip.HALO_N) < 0
+    assert lib.pxh_halo_bytes(grid, fs, 2, 1 << 9, _hip.HALO_W) < 0                # no such group
+    small = _hip.Grid(4, 9, _hip.PXH_F32, 0, 0.1)
+    assert lib.pxh_halo_bytes(small, fs, 2, groups, _hip.HALO_N) < 0               # no owned row between rings
+    fs.z_b_lo = 0x9000                                                              # compensated bed travels with z_b
+    assert lib.pxh_halo_bytes(grid, fs, 1, groups, _hip.HALO_E) == 1 * 8 * 6 * 4
