@@ -74,6 +74,8 @@ def parse():
     p.add_argument("--cpu-size", type=int, default=1024)
     p.add_argument("--cpu-steps", type=int, default=30)
     p.add_argument("--graph", type=int, default=1, help="capture each step into a hipGraph (1) or not (0)")
+    p.add_argument("--tile-steps", type=int, default=50,
+                   help="steps of the same domain as a 4x4 decomposition (0: skip)")
     return p.parse_args()
 
 
@@ -103,6 +105,41 @@ def cpu_baseline(n, steps):
             "sample": f"float32 numpy oracle, radial dam break {n}x{n} (same physics scaled), "
                       f"{steps} SSPRK2 steps, {el:.1f} s, cpu={platform.processor() or platform.machine()}, "
                       f"host cores={os.cpu_count()}"}
+
+
+def tiled_run(a, n, params, torch):
+    """The same domain as a 4 x 4 decomposition of (n/4)^2 owned tiles, each
+    with two ghost rings per shared side, all on this GPU: halo strips packed
+    and unpacked on the device, exchanges as device copies, one hipGraph per
+    step.  Returns its throughput (not the headline metric)."""
+    from pyexner_amd.runtime.driver import setup_case
+    from pyexner_amd.tiles import TiledExner
+    log(f"tiled run: 4x4 tiles of {n // 4}^2")
+    mesh, arr, _ = case(n)
+    params = dict(params, backend={"fused_stage": False})
+    state, bm, solver, sc, integ, ic = setup_case(mesh, arr, params, dtype=torch.float32)
+    del arr
+    mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
+    state = solver.init_fn(state, mask, sc)
+    tl = TiledExner(sc.backend, state, mask, bm, (4, 4), params["cfl"])
+    del state
+    tl.capture()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.tile_steps):
+        tl.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.tile_steps
+    if not torch.isfinite(tl.gather("h")).all():
+        raise RuntimeError("non-finite depth in the tiled run")
+    hb = tl.halo_bytes_per_step()
+    log(f"tiled: {ms:.3f} ms/step, {n * n / (ms * 1e-3) / 1e9:.3f} G cell-steps/s, halo {hb / 1e6:.1f} MB/step")
+    t = tl.tiles[(1, 1)]
+    return {"ms_per_step": ms, "cell_steps_per_s": n * n / (ms * 1e-3), "steps": a.tile_steps,
+            "tile_owned": [n // 4, n // 4], "tile_array": [t.grid.ny, t.grid.nx], "ghost_rings": 2,
+            "halo_bytes_per_step": hb, "path": "two-sweep stage, exchange after each sweep, dt reduced over tiles"}
 
 
 def main():
@@ -231,6 +268,10 @@ def main():
                 "alg_bytes_per_cell": ALG_BYTES[dom], "cells_per_launch": n * n}
     step_bytes = sum(ALG_BYTES[k] * c for k, c in PER_STEP.items())
 
+    tile = None
+    if a.tile_steps > 0 and n % 4 == 0:
+        tile = tiled_run(a, n, params, torch)
+
     cpu = None
     if not a.no_cpu:
         log(f"cpu baseline: oracle float32 at {a.cpu_size}^2 x {a.cpu_steps} steps")
@@ -257,6 +298,7 @@ def main():
         "roofline": roofline,
         "kernels": kern,
         "cpu_baseline": cpu,
+        "tiles_4x4": tile,
     }
     print(json.dumps(out), flush=True)
 

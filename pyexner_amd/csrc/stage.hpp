@@ -38,12 +38,11 @@ struct StagePos {
     bool hval;  // updated depth complete in this lane
 };
 
-__device__ __forceinline__ StagePos stage_pos(int ny, int nx) {
+// The strip of block (bx, by) of a domain's stage grid.
+__device__ __forceinline__ StagePos stage_pos_at(int bx, int by, int ny, int nx) {
     StagePos p;
     p.lane = threadIdx.x & (SW - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SW);
-    int bx, by;
-    xcd_block(bx, by);
     p.c = bx * FO - FHW + p.lane;
     p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
     p.out = p.lane >= FHW && p.lane < FHW + FO && p.c < nx;
@@ -51,6 +50,12 @@ __device__ __forceinline__ StagePos stage_pos(int ny, int nx) {
     p.r0 = (by * FWPB + w) * SR_STAGE;
     p.r1 = p.r0 + SR_STAGE < ny ? p.r0 + SR_STAGE : ny;
     return p;
+}
+
+__device__ __forceinline__ StagePos stage_pos(int ny, int nx) {
+    int bx, by;
+    xcd_block(bx, by);
+    return stage_pos_at(bx, by, ny, nx);
 }
 
 static inline dim3 stage_grid(int ny, int nx) {
@@ -128,14 +133,15 @@ __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, co
 //   the interior owned cells; with DT, the CFL minimum over the edges between
 //   interior cells of the combined state into partial[block].
 // in: the stage input (hydro fields, z, n_b, z_b, z_b_lo, G).
+// The strip p of one domain; returns this lane's dt minimum (DT).
+// Everything by value: a reference to a kernel argument reaches the inlined
+// body as a generic pointer, and the fields read through it land in vector
+// registers.
 template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
-__global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
-                                                             const uint8_t *__restrict__ m0,
-                                                             const T *__restrict__ dtp, int ny, int nx, T dx,
-                                                             SedK sd, int write_G,
-                                                             T *__restrict__ partial) {
-    __shared__ T red[FWPB];
-    const StagePos p = stage_pos(ny, nx);
+__device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, const F<T> out, const F<T> cb,
+                                             const F<T> old, const uint8_t *__restrict__ m0,
+                                             const T *__restrict__ dtp, int ny, int nx, T dx, const SedK &sd,
+                                             int write_G) {
     const Frame fr = frame_of(ny, nx);
     const T zero = T(0), hf = T(0.5);
     const int c = p.c;
@@ -271,6 +277,18 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
             edp = edk;
         }
     }
+    return vmin;
+}
+
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
+__global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
+                                                             const uint8_t *__restrict__ m0,
+                                                             const T *__restrict__ dtp, int ny, int nx, T dx,
+                                                             SedK sd, int write_G,
+                                                             T *__restrict__ partial) {
+    __shared__ T red[FWPB];
+    const StagePos p = stage_pos(ny, nx);
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
         if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = vmin;

@@ -202,6 +202,33 @@ class TiledExner:
         self._sweeps("a", "b", "a", None, "bed1")
         self._sweeps("b", "c", "b", "a", "bed")
 
+    def halo_bytes_per_step(self):
+        """Bytes the tiles send each other in one step (four exchanges)."""
+        lib = _hip.load()
+        total = 0
+        for t in self.tiles.values():
+            for side in _SIDES:
+                if t.ring[side]:
+                    total += 2 * lib.pxh_halo_bytes(t.grid, t.fa, HW, _HYDRO, side)
+                    total += 2 * lib.pxh_halo_bytes(t.grid, t.fa, HW, _BED, side)
+        return int(total)
+
+    def capture(self):
+        """Record one step into a hipGraph (one warm-up step first)."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.ssprk2_step()
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.ssprk2_step()
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+
     def gather(self, name):
         """The owned cells of every tile, as one global array."""
         parts = []
