@@ -76,6 +76,8 @@ def parse():
     p.add_argument("--graph", type=int, default=1, help="capture each step into a hipGraph (1) or not (0)")
     p.add_argument("--tile-steps", type=int, default=50,
                    help="steps of the same domain as a 4x4 decomposition (0: skip)")
+    p.add_argument("--ens-members", type=int, default=256, help="members of the batched ensemble (0: skip)")
+    p.add_argument("--ens-steps", type=int, default=20, help="timed steps of the ensemble and of its baseline")
     return p.parse_args()
 
 
@@ -140,6 +142,58 @@ def tiled_run(a, n, params, torch):
     return {"ms_per_step": ms, "cell_steps_per_s": n * n / (ms * 1e-3), "steps": a.tile_steps,
             "tile_owned": [n // 4, n // 4], "tile_array": [t.grid.ny, t.grid.nx], "ghost_rings": 2,
             "halo_bytes_per_step": hb, "path": "two-sweep stage, exchange after each sweep, dt reduced over tiles"}
+
+
+def ensemble_run(a, torch):
+    """The batched ensemble of uneven channel scenarios (nx ~ U{256..2048},
+    ny ~ U{64..512}, seeded inflow, depth and sediment diameter): one hipGraph
+    step for all members, against looping the single-domain step over them
+    (eagerly, and with that whole loop captured in one hipGraph)."""
+    from pyexner_amd.ensemble import Ensemble, channel_members
+    from pyexner_amd.stepper import ExnerStepper
+    t0 = time.perf_counter()
+    members = channel_members(a.ens_members, seed=4)
+    ens = Ensemble(members, dtype=torch.float32, cfl=0.5, keep_members=True)
+    del members
+    alone = [ExnerStepper(be, s, m, 0.5) for be, s, m in ens.alone]
+    ens.alone = None
+    torch.cuda.synchronize()
+    log(f"ensemble: {ens.n} members, {ens.cells / 1e6:.1f} M cells, setup {time.perf_counter() - t0:.1f} s")
+    k = a.ens_steps
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    ens.capture()
+    ms_ens = timed(ens.replay)
+
+    def loop():
+        for s in alone:
+            s.ssprk2_step()
+
+    loop()
+    ms_loop = timed(loop)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        loop()
+    ms_loop_graph = timed(g.replay)
+    if not torch.isfinite(ens.a.h).all():
+        raise RuntimeError("non-finite depth in the ensemble")
+    cells = ens.cells
+    log(f"ensemble: {ms_ens:.3f} ms/step ({cells / (ms_ens * 1e-3) / 1e9:.2f} G cell-steps/s); "
+        f"looped single-domain steps {ms_loop:.3f} ms eager, {ms_loop_graph:.3f} ms in one hipGraph")
+    return {"members": ens.n, "cells": cells, "steps": k, "ms_per_step": ms_ens,
+            "cell_steps_per_s": cells / (ms_ens * 1e-3),
+            "loop_ms_per_step": ms_loop, "loop_graph_ms_per_step": ms_loop_graph,
+            "speedup_vs_loop": ms_loop / ms_ens, "speedup_vs_loop_graph": ms_loop_graph / ms_ens,
+            "dtype": "f32", "scenario": "channel members, seed 4, dx 0.02, Transmissive in/out, Reflective walls"}
 
 
 def main():
@@ -271,6 +325,11 @@ def main():
     tile = None
     if a.tile_steps > 0 and n % 4 == 0:
         tile = tiled_run(a, n, params, torch)
+    ens = None
+    if a.ens_members > 0 and a.ens_steps > 0:
+        del st
+        torch.cuda.empty_cache()
+        ens = ensemble_run(a, torch)
 
     cpu = None
     if not a.no_cpu:
@@ -299,6 +358,7 @@ def main():
         "kernels": kern,
         "cpu_baseline": cpu,
         "tiles_4x4": tile,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

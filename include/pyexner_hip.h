@@ -234,6 +234,40 @@ int64_t pxh_dt_workspace_size(const pxh_grid *grid);
 int pxh_ssprk2_combine(const pxh_grid *grid, const pxh_fields *a, const pxh_fields *b,
                        const pxh_fields *out, void *stream);
 
+/* ---- ensembles ------------------------------------------------------------ */
+/* Many independent Roe-Exner domains (members of uneven shape, each with its
+ * own sediment table and its own CFL dt) advanced together: each SSPRK2 step is
+ * one launch per sweep for all members, not one step per member.  No reference
+ * counterpart (the reference runs one domain per process); it is the batched
+ * form of pxh_roe_exner_ssprk2 on frame-local rings.
+ * Every field of every member lives in one arena per field (pxh_fields of
+ * arena pointers), member m's (ny, nx) arrays at element offset off; the mask
+ * arena likewise.  The ring-fill program addresses arena elements (targets and
+ * sources offset by their member's off) and must be frame-local for every
+ * member (PXH_BC_FRAME).  dt_dev holds one dt per member.
+ * pxh_ensemble_plan fills, on the host, the plan the step reads (member
+ * table and float32 sediment folds); called with plan == NULL or too small a
+ * plan_bytes it returns the size.  The caller copies the plan to device memory
+ * and passes both copies.  workspace: pxh_ensemble_workspace_size bytes. */
+typedef struct {
+    int64_t off;              /* first element of the member's arrays in the arenas */
+    int32_t ny, nx;
+    double dx;
+    pxh_sediments sed;
+} pxh_member;
+int64_t pxh_ensemble_plan(int32_t dtype, int32_t flags, const pxh_member *members, int32_t n, void *plan,
+                          int64_t plan_bytes);
+int64_t pxh_ensemble_workspace_size(const void *plan);
+/* compute_dt_fn of every member into dt_dev[m]. */
+int pxh_ensemble_dt(const void *plan, const void *plan_dev, const pxh_fields *a, const uint8_t *mask0, double cfl,
+                    void *dt_dev, void *workspace, void *stream);
+/* One SSPRK2 step of every member on three arena sets, as pxh_roe_exner_ssprk2:
+ * dt from a unless dt_ready, b = S(a), c = S(b), a = 0.5 a + 0.5 c, and dt_dev
+ * left holding each member's dt of the new a. */
+int pxh_ensemble_ssprk2(const void *plan, const void *plan_dev, const pxh_fields *a, const pxh_fields *b,
+                        const pxh_fields *c, const uint8_t *mask0, const pxh_bc *bc, double cfl, void *dt_dev,
+                        void *workspace, int dt_ready, void *stream);
+
 /* ---- plain Roe ----------------------------------------------------------- */
 int pxh_roe_step(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
                  const pxh_bc *bc, const void *dt_dev, void *stream);

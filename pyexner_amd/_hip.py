@@ -47,6 +47,11 @@ class Sediments(ctypes.Structure):
         (n, ctypes.c_double * MAX_SED) for n in ("frac", "A", "B", "C", "D", "theta_c", "AB", "CD")]
 
 
+class Member(ctypes.Structure):      # pxh_member
+    _fields_ = [("off", ctypes.c_int64), ("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("dx", ctypes.c_double),
+                ("sed", Sediments)]
+
+
 class BCEntry(ctypes.Structure):
     _fields_ = [("target", ctypes.c_int32), ("src", ctypes.c_int32), ("group", ctypes.c_uint8),
                 ("nchain", ctypes.c_uint8), ("chain", ctypes.c_uint8 * MAX_CHAIN)]
@@ -102,6 +107,14 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_halo_unpack": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_ensemble_plan": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, _P(Member), ctypes.c_int32,
+                                           ctypes.c_void_p, ctypes.c_int64]),
+    "pxh_ensemble_workspace_size": (ctypes.c_int64, [ctypes.c_void_p]),
+    "pxh_ensemble_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _P(Fields), ctypes.c_void_p, ctypes.c_double,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_ensemble_ssprk2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _P(Fields), _P(Fields), _P(Fields),
+                                           ctypes.c_void_p, _P(BC), ctypes.c_double, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "pxh_roe_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
     "pxh_momentum_corrections": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),

@@ -1,0 +1,139 @@
+// Batched Roe-Exner SSPRK2: many independent domains (an ensemble of
+// scenarios of uneven size, each with its own sediment table and dt) advanced
+// by one launch per sweep for all members.
+//
+// Every member's fields live in shared arenas (one per field), member m at
+// element offset `off`.  A member table on the device gives each member's
+// shape and the first block of its strips in each launch; a block finds its
+// member by a binary search over those starts (wave-uniform, scalar loads) and
+// then runs the same strip body as the single-domain kernels.  Launch for
+// launch the batch is the fused SSPRK2 step of stage.hpp:
+//   stage sweep (all members) -> ring fill -> frame sweep -> ring fill ->
+//   stage sweep + combine + dt minima -> ring fill -> frame sweep + frame dt
+//   -> ring fill -> one block per member reduces its minima into its dt.
+#pragma once
+#include "stage.hpp"
+
+namespace pxh {
+
+struct EnsMember {
+    long off;      // first element of the member's arrays in the arenas
+    long nband;    // cells of its two-cell frame
+    int ny, nx;
+    int sgx;       // stage-grid columns
+    int sblk0;     // first stage-sweep block
+    int fblk0;     // first frame block (256 cells each)
+    int dgx;       // standalone-dt grid columns
+    int dblk0;     // first standalone-dt block
+    int part0;     // first of its partial minima
+    int nstage;    // stage-sweep blocks
+    int nframe;    // frame blocks
+    int ndt;       // standalone-dt blocks
+    int pad;
+    double dx;
+};
+
+// The member whose block range [start, next start) holds blk; tab has n + 1
+// entries, the last a sentinel with the launch's block count.
+template <int EnsMember::*START>
+__device__ __forceinline__ int find_member(const EnsMember *__restrict__ tab, int n, int blk) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tab[mid].*START <= blk) lo = mid;
+        else hi = mid;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
+__global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> cb, F<T> old,
+                                                        const uint8_t *__restrict__ m0, const T *__restrict__ dts,
+                                                        const EnsMember *__restrict__ tab, int n,
+                                                        const SedK *__restrict__ seds, int write_G,
+                                                        T *__restrict__ partial) {
+    __shared__ T red[FWPB];
+    int lb, unused;
+    xcd_block(lb, unused);  // one-dimensional grid: lb is the XCD-ordered block
+    const int i = find_member<&EnsMember::sblk0>(tab, n, lb);
+    const EnsMember m = tab[i];
+    const int local = lb - m.sblk0;
+    const int by = local / m.sgx, bx = local - by * m.sgx;
+    const StagePos p = stage_pos_at(bx, by, m.ny, m.nx);
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, shifted(in, m.off), shifted(out, m.off),
+                                                         shifted(cb, m.off), shifted(old, m.off), m0 + m.off,
+                                                         dts + i, m.ny, m.nx, static_cast<T>(m.dx), seds[i],
+                                                         write_G);
+    if (MODE == 1 && DT) {
+        vmin = wave_block_min_n<FWPB>(vmin, red);
+        if (threadIdx.x == 0) partial[m.part0 + local] = vmin;
+    }
+}
+
+template <typename T, bool FAST, int MODE>
+__global__ __launch_bounds__(256) void k_ens_band_bed(F<T> hyd, F<T> bed, F<T> out, F<T> old,
+                                                      const uint8_t *__restrict__ m0, const T *__restrict__ dts,
+                                                      const EnsMember *__restrict__ tab, int n,
+                                                      const SedK *__restrict__ seds, int write_G) {
+    const int i = find_member<&EnsMember::fblk0>(tab, n, blockIdx.x);
+    const EnsMember m = tab[i];
+    const long t = long(blockIdx.x - m.fblk0) * 256 + threadIdx.x;
+    if (t < m.nband)
+        band_bed_cell<T, FAST, MODE>(t, shifted(hyd, m.off), shifted(bed, m.off), shifted(out, m.off),
+                                     shifted(old, m.off), m0 + m.off, dts + i, m.ny, m.nx, static_cast<T>(m.dx),
+                                     seds[i], write_G);
+}
+
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_ens_band_dt(F<T> s, const uint8_t *__restrict__ m0,
+                                                     const EnsMember *__restrict__ tab, int n,
+                                                     T *__restrict__ partial) {
+    const int i = find_member<&EnsMember::fblk0>(tab, n, blockIdx.x);
+    const EnsMember m = tab[i];
+    const int local = blockIdx.x - m.fblk0;
+    const long t = long(local) * 256 + threadIdx.x;
+    block256_min_to(band_dt_cell<T, FAST>(t, m.nband, shifted(s, m.off), m0 + m.off, m.ny, m.nx,
+                                          static_cast<T>(m.dx)),
+                    partial + m.part0 + m.nstage + local);
+}
+
+template <typename T, bool FAST>
+__global__ __launch_bounds__(SW *SWPB) void k_ens_dt_strip(F<T> s, const uint8_t *__restrict__ m0,
+                                                           const EnsMember *__restrict__ tab, int n,
+                                                           T *__restrict__ partial) {
+    __shared__ T red[SWPB];
+    int lb, unused;
+    xcd_block(lb, unused);
+    const int i = find_member<&EnsMember::dblk0>(tab, n, lb);
+    const EnsMember m = tab[i];
+    const int local = lb - m.dblk0;
+    const int by = local / m.dgx, bx = local - by * m.dgx;
+    const StripPos p = strip_pos_at<SO_WIDE, SR_BED>(bx, by, m.ny, m.nx);
+    T v = dt_strip_run<T, FAST>(p, shifted(s, m.off), m0 + m.off, m.ny, m.nx, static_cast<T>(m.dx));
+    v = wave_block_min(v, red);
+    if (threadIdx.x == 0) partial[m.part0 + local] = v;
+}
+
+// One block per member: dts[i] = max(cfl * min of its minima, 1e-6)
+// (compute_dt_roeexner, roe_exner_solver.py:118-122).  step: the minima of a
+// fused SSPRK2 step (stage + frame blocks); else of the standalone dt sweep.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ens_dt_final(const T *__restrict__ partial,
+                                                      const EnsMember *__restrict__ tab, T cfl, int step,
+                                                      T *__restrict__ dts) {
+    __shared__ T red[4];
+    const EnsMember m = tab[blockIdx.x];
+    const int cnt = step ? m.nstage + m.nframe : m.ndt;
+    T v = inf_v<T>();
+    for (int k = threadIdx.x; k < cnt; k += 256) v = pmin(v, partial[m.part0 + k]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) v = pmin(v, red[w]);
+        dts[blockIdx.x] = pmax(cfl * v, C<T>(1e-6));
+    }
+}
+
+}  // namespace pxh

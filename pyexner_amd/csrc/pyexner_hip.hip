@@ -14,6 +14,7 @@
 #include "strip.hpp"
 #include "stage.hpp"
 #include "halo.hpp"
+#include "ensemble.hpp"
 
 namespace pxh {
 
@@ -903,6 +904,187 @@ static int halo_t(bool unpack, const pxh_grid *g, const pxh_fields *s, int hw, u
     return launch_status();
 }
 
+// ---------------------------------------------------------------------------
+// ensembles (ensemble.hpp): the plan blob and the batched step
+// ---------------------------------------------------------------------------
+struct EnsPlanHeader {
+    uint32_t magic;
+    int32_t n, dtype, flags;
+    int32_t mpm;           // every member's sediment mode
+    int32_t sblocks;       // stage-sweep blocks over all members
+    int32_t fblocks;       // frame blocks
+    int32_t dblocks;       // standalone-dt blocks
+    int32_t nparts;        // partial minima
+    int32_t pad;
+    int64_t tab_off, sed_off, bytes;
+    int64_t cells;         // arena elements the members span
+};
+constexpr uint32_t ENS_MAGIC = 0x50584845u;
+
+static int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
+
+static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *mem, int32_t n, void *out,
+                              int64_t out_bytes) {
+    if (n <= 0 || !mem || (dtype != PXH_F32 && dtype != PXH_F64)) return PXH_E_ARG;
+    const int64_t tab_off = align16(sizeof(EnsPlanHeader));
+    const int64_t sed_off = align16(tab_off + int64_t(n + 1) * sizeof(EnsMember));
+    const int64_t bytes = align16(sed_off + int64_t(n) * sizeof(SedK));
+    if (!out || out_bytes < bytes) return bytes;
+    char *base = static_cast<char *>(out);
+    EnsPlanHeader h{};
+    h.magic = ENS_MAGIC;
+    h.n = n;
+    h.dtype = dtype;
+    h.flags = flags;
+    h.mpm = mem[0].sed.mpm;
+    EnsMember *tab = reinterpret_cast<EnsMember *>(base + tab_off);
+    SedK *seds = reinterpret_cast<SedK *>(base + sed_off);
+    int64_t sb = 0, fb = 0, db = 0, np = 0, cells = 0;
+    const long esz = dtype == PXH_F64 ? 8 : 4;
+    for (int i = 0; i < n; ++i) {
+        const pxh_member &q = mem[i];
+        if (q.ny < 2 || q.nx < 2 || q.off < 0 || !(q.dx > 0)) return PXH_E_ARG;
+        if (long(SR_STAGE + 4) * q.nx * esz >= (1L << 32)) return PXH_E_RANGE;
+        const pxh_sediments &sd = q.sed;
+        if (sd.n_sed < 0 || sd.n_sed > PXH_MAX_SED || (sd.mpm && sd.n_sed < 1) || sd.mpm != h.mpm) return PXH_E_RANGE;
+        EnsMember m{};
+        m.off = q.off;
+        m.ny = q.ny;
+        m.nx = q.nx;
+        m.dx = q.dx;
+        const dim3 fg = stage_grid(q.ny, q.nx), dg = strip_grid<SO_WIDE, SR_BED>(q.ny, q.nx);
+        m.sgx = int(fg.x);
+        m.nstage = int(fg.x * fg.y);
+        m.nband = frame_cells(q.ny, q.nx);
+        m.nframe = int((m.nband + 255) / 256);
+        m.dgx = int(dg.x);
+        m.ndt = int(dg.x * dg.y);
+        m.sblk0 = int(sb);
+        m.fblk0 = int(fb);
+        m.dblk0 = int(db);
+        m.part0 = int(np);
+        sb += m.nstage;
+        fb += m.nframe;
+        db += m.ndt;
+        np += (m.nstage + m.nframe > m.ndt ? m.nstage + m.nframe : m.ndt);
+        const int64_t end = q.off + int64_t(q.ny) * q.nx;
+        if (end > cells) cells = end;
+        tab[i] = m;
+        seds[i] = sed_k(sd);
+    }
+    if (sb >= (1L << 31) || np >= (1L << 31) || cells >= (1L << 31)) return PXH_E_RANGE;
+    EnsMember sentinel{};
+    sentinel.sblk0 = int(sb);
+    sentinel.fblk0 = int(fb);
+    sentinel.dblk0 = int(db);
+    sentinel.part0 = int(np);
+    tab[n] = sentinel;
+    h.sblocks = int(sb);
+    h.fblocks = int(fb);
+    h.dblocks = int(db);
+    h.nparts = int(np);
+    h.tab_off = tab_off;
+    h.sed_off = sed_off;
+    h.bytes = bytes;
+    h.cells = cells;
+    *reinterpret_cast<EnsPlanHeader *>(base) = h;
+    return bytes;
+}
+
+static const EnsPlanHeader *ens_header(const void *plan) {
+    const EnsPlanHeader *h = static_cast<const EnsPlanHeader *>(plan);
+    return (h && h->magic == ENS_MAGIC && h->n > 0) ? h : nullptr;
+}
+
+template <typename T, bool FAST, int MODE, bool DT>
+static void ens_stage(const EnsPlanHeader &h, const EnsMember *tab, const SedK *seds, hipStream_t st, const F<T> &in,
+                      const F<T> &out, const F<T> &cb, const F<T> &old, const uint8_t *m0, const T *dts,
+                      int write_G, T *partial) {
+    const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
+#define PXH_ENS(LO_, MPM_)                                                                                       \
+    hipLaunchKernelGGL((k_ens_stage<T, FAST, MODE, DT, LO_, MPM_>), dim3(h.sblocks), dim3(SW * FWPB), 0, st, in,  \
+                       out, cb, old, m0, dts, tab, h.n, seds, write_G, partial)
+    if constexpr (sizeof(T) == 4) {
+        if (lo) {
+            if (h.mpm) PXH_ENS(true, true);
+            else PXH_ENS(true, false);
+            return;
+        }
+    }
+    if (h.mpm) PXH_ENS(false, true);
+    else PXH_ENS(false, false);
+#undef PXH_ENS
+}
+
+template <typename T, bool FAST>
+static int ens_dt_t(const EnsPlanHeader &h, const char *pd, const F<T> &a, const uint8_t *m0, double cfl, T *dts,
+                    T *part, hipStream_t st) {
+    const EnsMember *tab = reinterpret_cast<const EnsMember *>(pd + h.tab_off);
+    hipLaunchKernelGGL((k_ens_dt_strip<T, FAST>), dim3(h.dblocks), dim3(SW * SWPB), 0, st, a, m0, tab, h.n, part);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_ens_dt_final<T>, dim3(h.n), dim3(256), 0, st, part, tab, static_cast<T>(cfl), 0, dts);
+    return launch_status();
+}
+
+template <typename T, bool FAST>
+static int ens_ssprk2_t(const EnsPlanHeader &h, const char *pd, const pxh_fields *a_, const pxh_fields *b_,
+                        const pxh_fields *c_, const uint8_t *m0, const pxh_bc *bc, double cfl, T *dts, T *part,
+                        int dt_ready, hipStream_t st) {
+    const F<T> a = cast_fields<T>(a_), b = cast_fields<T>(b_), c = cast_fields<T>(c_);
+    const EnsMember *tab = reinterpret_cast<const EnsMember *>(pd + h.tab_off);
+    const SedK *seds = reinterpret_cast<const SedK *>(pd + h.sed_off);
+    const F<T> none{};
+    const pxh_grid g{1, 1, h.dtype, h.flags, 1.0};  // the ring fills read no geometry
+    if (!dt_ready)
+        if (int e = ens_dt_t<T, FAST>(h, pd, a, m0, cfl, dts, part, st)) return e;
+    const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
+    const uint32_t BED = (1u << PXH_G_ZB) | (h.mpm ? 0u : (1u << PXH_G_G));
+    // stage 1: b = S(a)
+    ens_stage<T, FAST, 0, false>(h, tab, seds, st, a, b, none, none, m0, dts, h.mpm ? 0 : 1, nullptr);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(&g, b, bc, HY, st)) return e;
+    hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 0>), dim3(h.fblocks), dim3(256), 0, st, b, a, b, none, m0, dts, tab,
+                       h.n, seds, h.mpm ? 0 : 1);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(&g, b, bc, BED, st)) return e;
+    // stage 2: c = S(b), a = 0.5 a + 0.5 c, and each member's dt of the new a
+    ens_stage<T, FAST, 1, true>(h, tab, seds, st, b, none, c, a, m0, dts, 1, part);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(&g, c, bc, HY, st)) return e;
+    hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 1>), dim3(h.fblocks), dim3(256), 0, st, c, b, none, a, m0, dts, tab,
+                       h.n, seds, 1);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL((k_ens_band_dt<T, FAST>), dim3(h.fblocks), dim3(256), 0, st, a, m0, tab, h.n, part);
+    if (int e = launch_status()) return e;
+    if (int e = bc_launch<T>(&g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
+    hipLaunchKernelGGL(k_ens_dt_final<T>, dim3(h.n), dim3(256), 0, st, part, tab, static_cast<T>(cfl), 1, dts);
+    return launch_status();
+}
+
+template <typename T>
+static int ens_ssprk2(const EnsPlanHeader &h, const char *pd, const pxh_fields *a, const pxh_fields *b,
+                      const pxh_fields *c, const uint8_t *m0, const pxh_bc *bc, double cfl, void *dts, void *ws,
+                      int dt_ready, hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (!(h.flags & PXH_EXACT_F32))
+            return ens_ssprk2_t<T, true>(h, pd, a, b, c, m0, bc, cfl, static_cast<T *>(dts), static_cast<T *>(ws),
+                                         dt_ready, st);
+    }
+    return ens_ssprk2_t<T, false>(h, pd, a, b, c, m0, bc, cfl, static_cast<T *>(dts), static_cast<T *>(ws),
+                                  dt_ready, st);
+}
+
+template <typename T>
+static int ens_dt(const EnsPlanHeader &h, const char *pd, const pxh_fields *a, const uint8_t *m0, double cfl,
+                  void *dts, void *ws, hipStream_t st) {
+    const F<T> f = cast_fields<T>(a);
+    if constexpr (sizeof(T) == 4) {
+        if (!(h.flags & PXH_EXACT_F32))
+            return ens_dt_t<T, true>(h, pd, f, m0, cfl, static_cast<T *>(dts), static_cast<T *>(ws), st);
+    }
+    return ens_dt_t<T, false>(h, pd, f, m0, cfl, static_cast<T *>(dts), static_cast<T *>(ws), st);
+}
+
 }  // namespace pxh
 
 using namespace pxh;
@@ -1148,6 +1330,49 @@ int pxh_halo_unpack(const pxh_grid *g, const pxh_fields *s, int hw, uint32_t gro
     if (!grid_ok(g) || !s || !buf) return PXH_E_ARG;
     return DISPATCH(g, halo_t, true, g, s, hw, groups, sides, const_cast<void *>(buf),
                     static_cast<hipStream_t>(stream));
+}
+
+int64_t pxh_ensemble_plan(int32_t dtype, int32_t flags, const pxh_member *members, int32_t n, void *plan,
+                          int64_t plan_bytes) {
+    return ens_plan_build(dtype, flags, members, n, plan, plan_bytes);
+}
+
+int64_t pxh_ensemble_workspace_size(const void *plan) {
+    const EnsPlanHeader *h = ens_header(plan);
+    if (!h) return PXH_E_ARG;
+    return int64_t(h->nparts) * 8;
+}
+
+int pxh_ensemble_dt(const void *plan, const void *plan_dev, const pxh_fields *a, const uint8_t *m0, double cfl,
+                    void *dt_dev, void *workspace, void *stream) {
+    const EnsPlanHeader *h = ens_header(plan);
+    if (!h || !plan_dev || !fields_ok(a, true) || !m0 || !dt_dev || !workspace) return PXH_E_ARG;
+    const char *pd = static_cast<const char *>(plan_dev);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return h->dtype == PXH_F64 ? ens_dt<double>(*h, pd, a, m0, cfl, dt_dev, workspace, st)
+                               : ens_dt<float>(*h, pd, a, m0, cfl, dt_dev, workspace, st);
+}
+
+int pxh_ensemble_ssprk2(const void *plan, const void *plan_dev, const pxh_fields *a, const pxh_fields *b,
+                        const pxh_fields *c, const uint8_t *m0, const pxh_bc *bc, double cfl, void *dt_dev,
+                        void *workspace, int dt_ready, void *stream) {
+    const EnsPlanHeader *h = ens_header(plan);
+    if (!h || !plan_dev || !fields_ok(a, true) || !fields_ok(b, true) || !fields_ok(c, true) || !m0 || !dt_dev ||
+        !workspace)
+        return PXH_E_ARG;
+    if (!bc_ok(bc)) return PXH_E_ARG;
+    if (bc && bc->n_entries > 0 && !(bc->flags & PXH_BC_FRAME)) return PXH_E_UNSUPPORTED;
+    const pxh_grid g{1, 1, h->dtype, h->flags, 1.0};
+    if (!lo_ok(&g, {a, b, c})) return PXH_E_ARG;
+    const void *pa[] = {a->h, a->hu, a->hv, a->z_b, a->G}, *pb[] = {b->h, b->hu, b->hv, b->z_b, b->G},
+               *pc[] = {c->h, c->hu, c->hv, c->z_b, c->G};
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j)
+            if (pa[i] == pb[j] || pa[i] == pc[j] || pb[i] == pc[j]) return PXH_E_ARG;
+    const char *pd = static_cast<const char *>(plan_dev);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return h->dtype == PXH_F64 ? ens_ssprk2<double>(*h, pd, a, b, c, m0, bc, cfl, dt_dev, workspace, dt_ready, st)
+                               : ens_ssprk2<float>(*h, pd, a, b, c, m0, bc, cfl, dt_dev, workspace, dt_ready, st);
 }
 
 }  // extern "C"

@@ -351,12 +351,9 @@ __device__ __forceinline__ ED<T> ed_at(const F<T> &hyd, const F<T> &bed, const u
 // stage input bed; MODE 0: out.{z_b, z_b_lo (, G)}; MODE 1: old = 0.5 old +
 // 0.5 result over h, hu, hv, G, z_b, z_b_lo.
 template <typename T, bool FAST, int MODE>
-__global__ __launch_bounds__(256) void k_rx_band_bed(F<T> hyd, F<T> bed, F<T> out, F<T> old,
-                                                      const uint8_t *__restrict__ m0, const T *__restrict__ dtp,
-                                                      int ny, int nx, T dx, SedK sd, int write_G,
-                                                      long nband) {
-    const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    if (t >= nband) return;
+__device__ __forceinline__ void band_bed_cell(long t, const F<T> hyd, const F<T> bed, const F<T> out, const F<T> old,
+                                              const uint8_t *__restrict__ m0, const T *__restrict__ dtp, int ny,
+                                              int nx, T dx, const SedK &sd, int write_G) {
     int r, c;
     frame_cell(t, ny, nx, r, c);
     const T zero = T(0), hf = T(0.5);
@@ -419,6 +416,15 @@ __global__ __launch_bounds__(256) void k_rx_band_bed(F<T> hyd, F<T> bed, F<T> ou
     }
 }
 
+template <typename T, bool FAST, int MODE>
+__global__ __launch_bounds__(256) void k_rx_band_bed(F<T> hyd, F<T> bed, F<T> out, F<T> old,
+                                                      const uint8_t *__restrict__ m0, const T *__restrict__ dtp,
+                                                      int ny, int nx, T dx, SedK sd, int write_G,
+                                                      long nband) {
+    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    if (t < nband) band_bed_cell<T, FAST, MODE>(t, hyd, bed, out, old, m0, dtp, ny, nx, dx, sd, write_G);
+}
+
 template <typename T, bool FAST>
 __device__ __forceinline__ DD<T> dd_at(const F<T> &s, const uint8_t *m0, long i) {
     return dd_make<T, FAST>(s.h[i], s.hu[i], s.hv[i], s.G[i], m0[i]);
@@ -458,10 +464,8 @@ static inline int seam_blocks(int ny, int nx) { return int((long(seam_count(ny))
 // between two frame cells may be evaluated twice; the minimum does not mind),
 // of the combined state s; one partial per block.
 template <typename T, bool FAST>
-__global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
-                                                     long nband, T *__restrict__ partial) {
-    __shared__ T red[4];
-    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+__device__ __forceinline__ T band_dt_cell(long t, long nband, const F<T> s, const uint8_t *__restrict__ m0, int ny,
+                                          int nx, T dx) {
     T v = inf_v<T>();
     if (t < nband) {
         int r, c;
@@ -485,14 +489,28 @@ __global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__res
             if (dd_active(n, me)) v = pmin(v, dt_edge_d<1, T, FAST>(n, me, dx));
         }
     }
+    return v;
+}
+
+// Minimum over a 256-thread block, written by thread 0 to *slot.
+template <typename T>
+__device__ __forceinline__ void block256_min_to(T v, T *slot) {
+    __shared__ T red[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 1; k < 4; ++k) v = pmin(v, red[k]);
-        partial[blockIdx.x] = v;
+        *slot = v;
     }
+}
+
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
+                                                     long nband, T *__restrict__ partial) {
+    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    block256_min_to(band_dt_cell<T, FAST>(t, nband, s, m0, ny, nx, dx), partial + blockIdx.x);
 }
 
 }  // namespace pxh

@@ -56,18 +56,23 @@ __device__ __forceinline__ void xcd_block(int &bx, int &by) {
 }
 
 template <int SO, int SROWS>
-__device__ __forceinline__ StripPos strip_pos(int ny, int nx) {
+__device__ __forceinline__ StripPos strip_pos_at(int bx, int by, int ny, int nx) {
     StripPos p;
     p.lane = threadIdx.x & (SW - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SW);  // uniform: rows are scalar
-    int bx, by;
-    xcd_block(bx, by);
     p.c = bx * SO - 1 + p.lane;
     p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
     p.out = p.lane >= 1 && p.lane <= SO && p.c < nx;
     p.r0 = (by * SWPB + w) * SROWS;
     p.r1 = p.r0 + SROWS < ny ? p.r0 + SROWS : ny;
     return p;
+}
+
+template <int SO, int SROWS>
+__device__ __forceinline__ StripPos strip_pos(int ny, int nx) {
+    int bx, by;
+    xcd_block(bx, by);
+    return strip_pos_at<SO, SROWS>(bx, by, ny, nx);
 }
 
 template <int SO, int SROWS>
@@ -1103,10 +1108,8 @@ __device__ __forceinline__ DD<T> dd_from(const DR<T> &q) {
 }
 
 template <typename T, bool FAST>
-__global__ __launch_bounds__(SW *SWPB) void k_rx_dt_strip(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx,
-                                                          T dx, T *__restrict__ partial) {
-    __shared__ T red[SWPB];
-    const StripPos p = strip_pos<SO_WIDE, SR_BED>(ny, nx);
+__device__ __forceinline__ T dt_strip_run(const StripPos p, const F<T> s, const uint8_t *__restrict__ m0, int ny,
+                                          int nx, T dx) {
     T v = inf_v<T>();
     if (p.r0 < ny) {
         const unsigned ucc = unsigned(p.cc);
@@ -1130,6 +1133,15 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_dt_strip(F<T> s, const uint8_t 
             cur = nxt;
         }
     }
+    return v;
+}
+
+template <typename T, bool FAST>
+__global__ __launch_bounds__(SW *SWPB) void k_rx_dt_strip(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx,
+                                                          T dx, T *__restrict__ partial) {
+    __shared__ T red[SWPB];
+    const StripPos p = strip_pos<SO_WIDE, SR_BED>(ny, nx);
+    T v = dt_strip_run<T, FAST>(p, s, m0, ny, nx, dx);
     v = wave_block_min(v, red);
     if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
 }

@@ -295,3 +295,31 @@ def test_halo_sizes_and_rejected_geometry():
     assert lib.pxh_halo_bytes(small, fs, 2, groups, _hip.HALO_N) < 0               # no owned row between rings
     fs.z_b_lo = 0x9000                                                              # compensated bed travels with z_b
     assert lib.pxh_halo_bytes(grid, fs, 1, groups, _hip.HALO_E) == 1 * 8 * 6 * 4
+
+
+def test_ensemble_plan_sizes_and_rejections():
+    """pxh_ensemble_plan (host-only): sizing call, filled plan and workspace,
+    and the members it refuses."""
+    import ctypes
+    mem = (_hip.Member * 3)()
+    shapes = [(40, 70), (33, 129), (65, 64)]
+    off = 0
+    for k, (ny, nx) in enumerate(shapes):
+        mem[k].off, mem[k].ny, mem[k].nx, mem[k].dx = off, ny, nx, 0.02
+        mem[k].sed.n_sed, mem[k].sed.mpm, mem[k].sed.one_minus_por = 1, 1, 0.58
+        mem[k].sed.frac[0], mem[k].sed.AB[0], mem[k].sed.CD[0], mem[k].sed.theta_c[0] = 1.0, 0.1, 2.0, 0.047
+        off += ny * nx
+    lib = _hip.load()
+    size = lib.pxh_ensemble_plan(_hip.PXH_F32, 0, mem, 3, None, 0)
+    assert size > 0
+    buf = (ctypes.c_uint8 * size)()
+    assert lib.pxh_ensemble_plan(_hip.PXH_F32, 0, mem, 3, buf, size) == size
+    ws = lib.pxh_ensemble_workspace_size(buf)
+    assert ws > 0 and ws % 8 == 0
+    assert lib.pxh_ensemble_workspace_size(None) < 0
+    assert lib.pxh_ensemble_plan(_hip.PXH_F32, 0, mem, 0, None, 0) < 0           # no member
+    mem[1].sed.mpm = 0                                                             # sediment modes differ
+    assert lib.pxh_ensemble_plan(_hip.PXH_F32, 0, mem, 3, buf, size) < 0
+    mem[1].sed.mpm = 1
+    mem[2].ny = 1                                                                  # not a grid
+    assert lib.pxh_ensemble_plan(_hip.PXH_F32, 0, mem, 3, buf, size) < 0
