@@ -78,6 +78,9 @@ def parse():
                    help="steps of the same domain as a 4x4 decomposition (0: skip)")
     p.add_argument("--ens-members", type=int, default=256, help="members of the batched ensemble (0: skip)")
     p.add_argument("--ens-steps", type=int, default=20, help="timed steps of the ensemble and of its baseline")
+    p.add_argument("--adj-steps", type=int, default=200,
+                   help="steps of the gradient run (bed calibration, 1024x256 float64; 0: skip)")
+    p.add_argument("--adj-every", type=int, default=20, help="checkpoint interval of the gradient run")
     return p.parse_args()
 
 
@@ -194,6 +197,63 @@ def ensemble_run(a, torch):
             "loop_ms_per_step": ms_loop, "loop_graph_ms_per_step": ms_loop_graph,
             "speedup_vs_loop": ms_loop / ms_ens, "speedup_vs_loop_graph": ms_loop_graph / ms_ens,
             "dtype": "f32", "scenario": "channel members, seed 4, dx 0.02, Transmissive in/out, Reflective walls"}
+
+
+def adjoint_run(a, torch):
+    """Bed calibration: the gradient of 0.5 |z_b(T) - target|^2 with respect to
+    z_b0, d50 and porosity over a.adj_steps SSPRK2 steps of the 1024x256
+    calibration channel in float64, checkpoint every a.adj_every steps.  Times
+    the forward pass (fused steps + checkpoints), the backward pass (segment
+    replays + adjoint steps + init adjoint), and the same number of plain
+    float64 forward steps."""
+    from pyexner_amd import cases
+    from pyexner_amd.adjoint import ExnerGradientRun, seeded_target
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = cases.calibration_channel(nx=1024, ny=256)
+    run = ExnerGradientRun(mesh, arr, params, n_steps=a.adj_steps, checkpoint_every=a.adj_every)
+    zb0 = run.raw.z_b.clone()
+    target = seeded_target(zb0)
+    d50_0, por_0 = float(run.seds[1, 0]), float(run.seds[6, 0])
+
+    def once():
+        zb = zb0.clone().requires_grad_(True)
+        d50 = torch.tensor(d50_0, dtype=torch.float64, requires_grad=True)
+        por = torch.tensor(por_0, dtype=torch.float64, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = run(zb, d50, por)
+        loss = 0.5 * ((out[3] - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, float(loss.detach()), zb.grad, float(d50.grad), float(por.grad)
+
+    once()  # warm-up
+    tf, tb, loss, g, gd, gp = once()
+    # plain float64 forward steps of the same case (fused SSPRK2, no checkpoints)
+    stp = ExnerStepper(run.be, run.s0, run.mask, run.cfl)
+    stp.compute_dt()
+    stp.ssprk2_step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.adj_steps):
+        stp.ssprk2_step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fwd = e0.elapsed_time(e1) / a.adj_steps
+    ms_bwd = tb * 1e3 / a.adj_steps
+    if not torch.isfinite(g).all():
+        raise RuntimeError("non-finite bed gradient")
+    log(f"adjoint: {a.adj_steps} steps 1024x256 f64: forward {tf * 1e3:.1f} ms, backward {tb * 1e3:.1f} ms; "
+        f"plain forward step {ms_fwd:.3f} ms, backward per step {ms_bwd:.3f} ms ({ms_bwd / ms_fwd:.2f}x)")
+    return {"case": "calibration_channel 1024x256, dx 0.02, seed 5; target z_b0 + seeded perturbation (seed 6)",
+            "dtype": "f64", "steps": a.adj_steps, "checkpoint_every": a.adj_every,
+            "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "plain_forward_ms_per_step": ms_fwd,
+            "backward_ms_per_step": ms_bwd, "adjoint_step_vs_forward_step": ms_bwd / ms_fwd,
+            "loss": loss, "grad_zb0_norm": float(g.norm()), "grad_d50": gd, "grad_porosity": gp}
 
 
 def main():
@@ -331,6 +391,11 @@ def main():
         torch.cuda.empty_cache()
         ens = ensemble_run(a, torch)
 
+    adj = None
+    if a.adj_steps > 0:
+        torch.cuda.empty_cache()
+        adj = adjoint_run(a, torch)
+
     cpu = None
     if not a.no_cpu:
         log(f"cpu baseline: oracle float32 at {a.cpu_size}^2 x {a.cpu_steps} steps")
@@ -359,6 +424,7 @@ def main():
         "cpu_baseline": cpu,
         "tiles_4x4": tile,
         "ensemble": ens,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

@@ -20,6 +20,7 @@
  *   BoundaryManager.apply  domain/boundary_registry.py:183-187 -> pxh_bc_apply
  *   SSPRK2 combine          integrators/SSPRK2.py:40            -> pxh_ssprk2_combine
  *   make_halo_exchange      solvers/kernels/roe_exner.py:719-803 -> pxh_halo_pack / pxh_halo_unpack
+ *   (no counterpart: reverse mode of step_fn / compute_dt_fn / init_fn) -> pxh_adj_roe_exner_*
  */
 #ifndef PYEXNER_HIP_H
 #define PYEXNER_HIP_H
@@ -116,6 +117,12 @@ typedef struct {
  * then visits only the entries of the groups in group_mask.  Without it every
  * call visits every entry. */
 #define PXH_BC_GROUPED 2
+/* pxh_bc.flags: PXH_BC_SRC_SORTED promises that within each group the entries
+ * are ordered by src, so that the entries filled from one source are adjacent.
+ * The adjoint fill (pxh_adj_*) then sums their adjoints in a fixed order and
+ * its results do not depend on scheduling; without the flag it accumulates
+ * with atomics (the same sums up to round-off). */
+#define PXH_BC_SRC_SORTED 4
 #define PXH_N_GROUPS 7
 
 typedef struct {
@@ -267,6 +274,47 @@ int pxh_ensemble_dt(const void *plan, const void *plan_dev, const pxh_fields *a,
 int pxh_ensemble_ssprk2(const void *plan, const void *plan_dev, const pxh_fields *a, const pxh_fields *b,
                         const pxh_fields *c, const uint8_t *mask0, const pxh_bc *bc, double cfl, void *dt_dev,
                         void *workspace, int dt_ready, void *stream);
+
+/* ---- adjoint (double only) ------------------------------------------------ */
+/* Reverse-mode derivatives of the Roe-Exner step: each entry is the transpose
+ * of the forward entry it names, evaluated at the forward's inputs, for
+ * gradients of a scalar loss of the final state.  The reference has no
+ * gradient code; these are what jax.grad through its step_fn / compute_dt_fn /
+ * init_fn (solvers/roe_exner_solver.py:58-122) would give, with the
+ * derivative of every untaken branch taken as zero.  Adjoint fields ("lam") are (ny, nx) double arrays in
+ * pxh_fields slots of the same names; MPM G only (PXH_E_UNSUPPORTED otherwise).
+ * lam_dt: one device double; lam_par: five device doubles, the derivatives
+ * with respect to the first sediment class's A, B, C (pxh_sediments),
+ * one_minus_por and nb_sed, accumulated (may be NULL).  The ring-fill program
+ * must not read a cell it writes.  workspace: pxh_adj_workspace_size bytes,
+ * sized with the program of most entries that the calls will pass (may be
+ * NULL when they pass none). */
+int64_t pxh_adj_workspace_size(const pxh_grid *grid, const pxh_bc *largest_bc);
+/* One stage (pxh_roe_exner_step in -> out at dt): lam_out (h, hu, hv, z_b, G)
+ * is the adjoint of out and is consumed; the adjoint of in accumulates into
+ * lam_in (h, hu, hv, z_b, and n_b when set); d/d dt into *lam_dt. */
+int pxh_adj_roe_exner_stage(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
+                            const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc, const void *dt_dev,
+                            const pxh_fields *lam_out, const pxh_fields *lam_in, void *lam_dt, void *lam_par,
+                            void *workspace, void *stream);
+/* compute_dt_fn (pxh_roe_exner_dt): lam_s (h, hu, hv, G) += lam_dt * d dt / d s. */
+int pxh_adj_roe_exner_dt(const pxh_grid *grid, const pxh_fields *s, const uint8_t *mask0, double cfl,
+                         const void *lam_dt, const pxh_fields *lam_s, void *workspace, void *stream);
+/* One SSPRK2 step (pxh_roe_exner_ssprk2 from a, with dt = compute_dt_fn(a)):
+ * a the step's input state, b the stage-1 output (h, hu, hv, z_b), c the
+ * stage-2 output's hydro (h, hu, hv).  lam holds the adjoint of the step's
+ * output (h, hu, hv, z_b, G) on entry and of a on return; lam->n_b (may be
+ * NULL) accumulates the adjoint of the bed Manning coefficient n_b. */
+int pxh_adj_roe_exner_ssprk2(const pxh_grid *grid, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
+                             const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc, double cfl,
+                             const void *dt_dev, const pxh_fields *lam, void *lam_par, void *workspace,
+                             void *stream);
+/* init_fn (pxh_roe_exner_init + pxh_bc_apply of the full program bc): s the
+ * initialised state; lam the adjoint of s on entry (h, hu, hv, z_b, G, n_b)
+ * and of the state before initialisation on return (h, hu, hv, z_b). */
+int pxh_adj_roe_exner_init(const pxh_grid *grid, const pxh_fields *s, const uint8_t *mask0,
+                           const pxh_sediments *sed, const pxh_bc *bc, const pxh_fields *lam, void *lam_par,
+                           void *workspace, void *stream);
 
 /* ---- plain Roe ----------------------------------------------------------- */
 int pxh_roe_step(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,

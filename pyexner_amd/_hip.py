@@ -18,6 +18,7 @@ MAX_CHAIN = 6
 BC_COPY, BC_REFLECT_RX, BC_REFLECT_ROE = 0, 1, 2
 BC_FRAME = 1  # pxh_bc.flags: every target on the outer ring, every source within two cells of the edge
 BC_GROUPED = 2  # pxh_bc.flags: entries ordered by group, group_start[] valid
+BC_SRC_SORTED = 4  # pxh_bc.flags: within a group, entries ordered by src (adjoint programs)
 N_GROUPS = 7
 HALO_W, HALO_N, HALO_E, HALO_S = 1, 2, 4, 8  # pxh_halo_* sides
 SWEEP_SKIP_G = 1  # flags of the stage/frame sweeps: leave out.G unwritten in MPM mode
@@ -122,6 +123,17 @@ _SIGS = {
                                      ctypes.c_void_p]),
     "pxh_exner_solve_2D": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p]),
+    "pxh_adj_workspace_size": (ctypes.c_int64, [_P(Grid), _P(BC)]),
+    "pxh_adj_roe_exner_stage": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p, _P(Sediments),
+                                               _P(BC), ctypes.c_void_p, _P(Fields), _P(Fields), ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_adj_roe_exner_dt": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                            _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_adj_roe_exner_ssprk2": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
+                                                _P(Sediments), _P(BC), ctypes.c_double, ctypes.c_void_p, _P(Fields),
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_adj_roe_exner_init": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_void_p, _P(Sediments), _P(BC),
+                                              _P(Fields), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 _lib = None

@@ -283,6 +283,22 @@ class BoundaryManager:
             bc.handlers[k].ny = float(hd.normal[1])
         return bc
 
+    def adjoint_program(self, groups=None):
+        """The program ordered for the adjoint fill (pxh_adj_*): entries
+        sorted by (group, src), stably, so that the targets filled from one
+        source are adjacent and their adjoints are summed in a fixed order
+        (PXH_BC_SRC_SORTED)."""
+        if not hasattr(self, "_adj_entries_dev"):
+            ent = self.entries
+            order = np.lexsort((ent["src"], ent["group"])) if len(ent) else np.zeros(0, dtype=np.int64)
+            self._adj_entries = ent[order]
+            self._adj_entries_dev = (torch.from_numpy(self._adj_entries.view(np.uint8).copy()).to(self.device)
+                                     if len(ent) else None)
+        bc = self.program(groups)
+        bc.entries = self._adj_entries_dev.data_ptr() if self._adj_entries_dev is not None else None
+        bc.flags = _hip.BC_GROUPED | _hip.BC_SRC_SORTED
+        return bc
+
     def apply_inplace(self, state, groups=None):
         from .state import fields_struct, dtype_code
         if self.n_entries == 0:

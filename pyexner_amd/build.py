@@ -1,34 +1,51 @@
 """Build libpyexner_hip.so in-tree with hipcc for gfx950.
 
-The library is compiled with -ffp-contract=off so the double build reproduces
-the float64 restatement's operation order, and with packed fp32 instructions
-disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+Two translation units (the forward path and the adjoint) are compiled in
+parallel and linked into one shared library.  The library is compiled with
+-ffp-contract=off so the double build reproduces the float64 restatement's
+operation order, and with packed fp32 instructions disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
 import os
 import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "pyexner_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", n) for n in ("pyexner_hip.hip", "adjoint.hip")]
 OUT = os.path.join(HERE, "lib", "libpyexner_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PYEXNER_ARCH", "gfx950")
 
-FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _check(r, cmd):
+    if r.returncode != 0:
+        sys.stderr.write((r.stdout or "") + (r.stderr or ""))
+        raise RuntimeError(f"hipcc failed ({r.returncode}): {' '.join(cmd)}")
 
 
 def build(verbose=False):
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    objs, procs = [], []
+    for src in SRCS:
+        obj = os.path.join(os.path.dirname(OUT), os.path.splitext(os.path.basename(src))[0] + ".o")
+        cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, src]
+        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), cmd))
+        objs.append(obj)
+    for p, cmd in procs:
+        out, err = p.communicate()
+        r = subprocess.CompletedProcess(cmd, p.returncode, out, err)
+        _check(r, cmd)
+        if verbose:
+            sys.stderr.write(err)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, SRC]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode}): {' '.join(cmd)}")
-    if verbose:
-        sys.stderr.write(r.stderr)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    _check(subprocess.run(cmd, capture_output=True, text=True), cmd)
     os.replace(tmp, OUT)
+    for o in objs:
+        os.remove(o)
     return OUT
 
 

@@ -11,27 +11,13 @@
 #include "common.hpp"
 #include "flux_roe.hpp"
 #include "flux_roe_exner.hpp"
+#include "cell.hpp"
 #include "strip.hpp"
 #include "stage.hpp"
 #include "halo.hpp"
 #include "ensemble.hpp"
 
 namespace pxh {
-
-template <typename T>
-static F<T> cast_fields(const pxh_fields *f) {
-    F<T> r;
-    r.h = static_cast<T *>(f->h);
-    r.hu = static_cast<T *>(f->hu);
-    r.hv = static_cast<T *>(f->hv);
-    r.z = static_cast<T *>(f->z);
-    r.z_b = static_cast<T *>(f->z_b);
-    r.n = static_cast<T *>(f->n);
-    r.n_b = static_cast<T *>(f->n_b);
-    r.G = static_cast<T *>(f->G);
-    r.z_b_lo = sizeof(T) == 4 ? static_cast<T *>(f->z_b_lo) : nullptr;
-    return r;
-}
 
 constexpr int BX = 64, BY = 4;
 
@@ -40,17 +26,8 @@ static inline dim3 grid2d(int ny, int nx) {
 }
 
 // ---------------------------------------------------------------------------
-// Roe-Exner hydro: roe_solve_2D (roe_exner.py:535-612), one thread per cell,
-// summing F = M_x[r,c] + P_x[r,c-1] + M_y[r,c] + P_y[r-1,c] in that order.
+// Roe-Exner hydro: roe_solve_2D (roe_exner.py:535-612), one thread per cell.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ HydroCell<T> load_hc(const F<T> &s, int i) {
-    HydroCell<T> c;
-    c.h = s.h[i]; c.hu = s.hu[i]; c.hv = s.hv[i];
-    c.zb = s.z_b[i]; c.z = s.z[i]; c.n = s.n_b[i];
-    return c;
-}
-
 template <typename T>
 __global__ __launch_bounds__(BX *BY) void k_rx_hydro(F<T> in, F<T> out, const uint8_t *__restrict__ m0,
                                                       const T *__restrict__ dtp, int ny, int nx, T dx) {
@@ -58,47 +35,11 @@ __global__ __launch_bounds__(BX *BY) void k_rx_hydro(F<T> in, F<T> out, const ui
     const int r = blockIdx.y * BY + threadIdx.y;
     if (r >= ny || c >= nx) return;
     const int i = r * nx + c;
-    const T T_ = C<T>(DRY_TOL);
-    const HydroCell<T> me = load_hc(in, i);
-    const bool mme = m0[i] != 0;
-    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    T P[3], M[3];
-    if (c + 1 < nx) {  // east edge: this cell is side i -> upwM
-        const int j = i + 1;
-        const HydroCell<T> o = load_hc(in, j);
-        if (((me.h > T_) || (o.h > T_)) && !(mme || m0[j])) {
-            roe_exner_edge<0>(me, o, dx, P, M);
-            acc0 = acc0 + M[0]; acc1 = acc1 + M[1]; acc2 = acc2 + M[2];
-        }
-    }
-    if (c > 0) {  // west edge: this cell is side j -> upwP
-        const int j = i - 1;
-        const HydroCell<T> o = load_hc(in, j);
-        if (((o.h > T_) || (me.h > T_)) && !(m0[j] || mme)) {
-            roe_exner_edge<0>(o, me, dx, P, M);
-            acc0 = acc0 + P[0]; acc1 = acc1 + P[1]; acc2 = acc2 + P[2];
-        }
-    }
-    if (r + 1 < ny) {  // south edge (row r+1): this cell is side i -> upwM
-        const int j = i + nx;
-        const HydroCell<T> o = load_hc(in, j);
-        if (((me.h > T_) || (o.h > T_)) && !(mme || m0[j])) {
-            roe_exner_edge<1>(me, o, dx, P, M);
-            acc0 = acc0 + M[0]; acc1 = acc1 + M[1]; acc2 = acc2 + M[2];
-        }
-    }
-    if (r > 0) {  // north edge: this cell is side j -> upwP
-        const int j = i - nx;
-        const HydroCell<T> o = load_hc(in, j);
-        if (((o.h > T_) || (me.h > T_)) && !(m0[j] || mme)) {
-            roe_exner_edge<1>(o, me, dx, P, M);
-            acc0 = acc0 + P[0]; acc1 = acc1 + P[1]; acc2 = acc2 + P[2];
-        }
-    }
-    const T dt = *dtp;
-    out.h[i] = me.h - dt * acc0 / dx;
-    out.hu[i] = me.hu - dt * acc1 / dx;
-    out.hv[i] = me.hv - dt * acc2 / dx;
+    T h1, hu1, hv1;
+    rx_hydro_cell<T>(in, m0, *dtp, ny, nx, dx, r, c, h1, hu1, hv1);
+    out.h[i] = h1;
+    out.hu[i] = hu1;
+    out.hv[i] = hv1;
 }
 
 // momentum_corrections (roe_exner.py:14-62) on interior cells, in place on the
@@ -110,19 +51,10 @@ __global__ __launch_bounds__(BX *BY) void k_rx_momcorr(F<T> in, F<T> out, const 
     const int r = blockIdx.y * BY + threadIdx.y;
     if (r < 1 || c < 1 || r >= ny - 1 || c >= nx - 1) return;
     const int i = r * nx + c;
-    const T T_ = C<T>(DRY_TOL);
-    const int iW = i - 1, iE = i + 1, iN = i - nx, iS = i + nx;
-    const T hc = out.h[i];
-    const T eta = hc + (in.z[i] + in.z_b[i]);
-    const T zW = in.z[iW] + in.z_b[iW], zE = in.z[iE] + in.z_b[iE];
-    const T zN = in.z[iN] + in.z_b[iN], zS = in.z[iS] + in.z_b[iS];
-    const bool dryW = out.h[iW] < T_, dryE = out.h[iE] < T_;
-    const bool dryN = out.h[iN] < T_, dryS = out.h[iS] < T_;
-    const bool stop_u = ((eta < zE) && dryE) || ((eta < zW) && dryW) || m0[iE] || m0[iW];
-    const bool stop_v = ((eta < zN) && dryN) || ((eta < zS) && dryS) || m0[iN] || m0[iS];
-    const bool active = (hc > T_) && !m0[i];
-    if (!(active && !stop_u)) out.hu[i] = T(0);
-    if (!(active && !stop_v)) out.hv[i] = T(0);
+    bool keep_u, keep_v;
+    momcorr_keep<T>(out.h, in.z, in.z_b, m0, nx, i, keep_u, keep_v);
+    if (!keep_u) out.hu[i] = T(0);
+    if (!keep_v) out.hv[i] = T(0);
 }
 
 // G for every cell from the post-fill hydro state (roe_exner_solver.py:99).
@@ -137,21 +69,6 @@ __global__ __launch_bounds__(256) void k_rx_G(F<T> in, F<T> out, const uint8_t *
 
 // exner_solve_2D (roe_exner.py:665-717): reads the updated h, hu, hv, G and
 // the bed of the step start; div = ((0 - Fx[c-1]) + Fx[c]) - Fy[r-1] + Fy[r].
-template <typename T>
-__device__ __forceinline__ ExnerCell<T> load_ec(const F<T> &s, const T *zb, int i) {
-    ExnerCell<T> c;
-    c.h = s.h[i]; c.hu = s.hu[i]; c.hv = s.hv[i]; c.zb = zb[i]; c.G = s.G[i];
-    return c;
-}
-
-template <typename T>
-__device__ __forceinline__ bool exner_active(const ExnerCell<T> &a, const ExnerCell<T> &b, bool ma, bool mb) {
-    const T T_ = C<T>(DRY_TOL), S_ = C<T>(SED_TOL);
-    const bool stop = (ma || mb) || ((a.zb < S_) && (b.zb < S_));
-    const bool depth = (a.h > T_) || (b.h > T_);
-    return depth && !stop;
-}
-
 template <typename T>
 __global__ __launch_bounds__(BX *BY) void k_rx_exner(F<T> in, F<T> out, const uint8_t *__restrict__ m0,
                                                       const T *__restrict__ dtp, int ny, int nx, T dx) {
@@ -379,23 +296,6 @@ __global__ __launch_bounds__(256) void k_rx_init(F<T> s, const uint8_t *__restri
 // momentum entries apply the chain of handler transforms in order
 // (roeexner_reflective.py:31-59, roe_reflective.py:34-39).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void bc_transform(const pxh_bc_handler &hd, T &hu, T &hv) {
-    const T nx = static_cast<T>(hd.nx), ny = static_cast<T>(hd.ny);
-    if (hd.kind == PXH_BC_REFLECT_RX) {
-        const T mn = hu * nx + hv * ny;
-        const T mt = -hu * ny + hv * nx;
-        const T mr = -mn;
-        const T u2 = mr * nx - mt * ny;
-        const T v2 = mr * ny + mt * nx;
-        hu = u2;
-        hv = v2;
-    } else if (hd.kind == PXH_BC_REFLECT_ROE) {
-        hu = hu * (T(1) - T(2) * nx * nx);
-        hv = hv * (T(1) - T(2) * ny * ny);
-    }
-}
-
 // The entries one ring fill visits: range r is entries [start[r], start[r] +
 // off[r+1] - off[r]), taken by threads [off[r], off[r+1]).
 struct BcSel {
@@ -436,25 +336,6 @@ __global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restri
 // ---------------------------------------------------------------------------
 // host-side launch helpers
 // ---------------------------------------------------------------------------
-static inline bool grid_ok(const pxh_grid *g) {
-    return g && g->ny >= 2 && g->nx >= 2 && (g->dtype == PXH_F32 || g->dtype == PXH_F64) &&
-           (long(g->ny) * g->nx) < (1L << 31);
-}
-
-// A boundary program (or none) the ring-fill launch can run: with
-// PXH_BC_GROUPED, group_start must be non-decreasing from 0 to n_entries.
-static inline bool bc_ok(const pxh_bc *bc) {
-    if (!bc) return true;
-    if (bc->n_handlers < 0 || bc->n_handlers > PXH_MAX_HANDLERS || bc->n_entries < 0) return false;
-    if (bc->n_entries > 0 && !bc->entries) return false;
-    if (bc->flags & PXH_BC_GROUPED) {
-        if (bc->group_start[0] != 0 || bc->group_start[PXH_N_GROUPS] != bc->n_entries) return false;
-        for (int q = 0; q < PXH_N_GROUPS; ++q)
-            if (bc->group_start[q + 1] < bc->group_start[q]) return false;
-    }
-    return true;
-}
-
 static inline int dt_blocks(const pxh_grid *g) {
     long n = (long(g->ny) * g->nx + DT_THREADS - 1) / DT_THREADS;
     return int(n < DT_MAX_BLOCKS ? n : DT_MAX_BLOCKS);
@@ -823,12 +704,6 @@ static bool lo_ok(const pxh_grid *g, std::initializer_list<const pxh_fields *> f
         if (f->z_b_lo) ++n;
     }
     return n == 0 || (n == k && g->dtype == PXH_F32);
-}
-
-static bool fields_ok(const pxh_fields *f, bool exner) {
-    if (!f || !f->h || !f->hu || !f->hv || !f->z || !f->n) return false;
-    if (exner && (!f->z_b || !f->n_b || !f->G)) return false;
-    return true;
 }
 
 template <typename T>
