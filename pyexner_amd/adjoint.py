@@ -27,6 +27,7 @@ import numpy as np
 import torch
 
 from . import _hip
+from .constants import g
 from .runtime.driver import setup_case
 from .solvers.roe_exner_solver import sediment_table
 from .state import fields_struct
@@ -161,28 +162,31 @@ class ExnerGradientRun:
                   fl, lam_par.data_ptr(), ws, _stream())
         self.lam_init = L
         self.lam_par = lam_par[:5].cpu().numpy()
-        d_d50, d_por = self._chain_params(self.lam_par)
+        d_d50, d_por = chain_sediment_gradients(self.cur_seds, self.lam_par)
         return L.z_b.clone(), d_d50, d_por
-
-    def _chain_params(self, lp):
-        """dL/dd50 and dL/dporosity from the derivatives with respect to
-        A = np^2, B = (s-1) d50, C = np^3 sqrt(g), 1 - porosity and
-        nb = (d50 frac)^(1/6) / 21.1 (np = d50^(1/6) / 21.1; roe_exner.py:64-116)."""
-        frac, d50, density = self.cur_seds[0, 0], self.cur_seds[1, 0], self.cur_seds[2, 0]
-        sd = self.be.sed
-        s = density / 1000.0
-        A, C = sd.A[0], sd.C[0]
-        d50m = d50 * frac
-        nb = d50m ** (1.0 / 6.0) / 21.1
-        d_d50 = (lp[0] * A / (3.0 * d50) + lp[1] * (s - 1.0) + lp[2] * C / (2.0 * d50)
-                 + lp[4] * nb * frac / (6.0 * d50m))
-        d_por = -lp[3]
-        return float(d_d50), float(d_por)
 
     def __call__(self, z_b0, d50, porosity):
         d50 = torch.as_tensor(d50, dtype=torch.float64)
         porosity = torch.as_tensor(porosity, dtype=torch.float64)
         return _ExnerRunFn.apply(z_b0, d50, porosity, self)
+
+
+def chain_sediment_gradients(seds, lam_par):
+    """dL/dd50 and dL/dporosity of a one-class sediment table seds (7, 1) from
+    lam_par, the derivatives with respect to the table entries A = np^2,
+    B = (s-1) d50, C = np^3 sqrt(g), one_minus_por = 1 - porosity and
+    nb_sed = (d50 frac)^(1/6) / 21.1, where np = d50^(1/6) / 21.1 and
+    s = density / 1000 (roe_exner.py:64-116)."""
+    frac, d50, density = float(seds[0, 0]), float(seds[1, 0]), float(seds[2, 0])
+    lp = np.asarray(lam_par, dtype=np.float64)
+    s = density / 1000.0
+    np_ = d50 ** (1.0 / 6.0) / 21.1
+    A, C = np_ * np_, np_ ** 3 * np.sqrt(g)
+    d50m = d50 * frac
+    nb = d50m ** (1.0 / 6.0) / 21.1
+    d_d50 = (lp[0] * A / (3.0 * d50) + lp[1] * (s - 1.0) + lp[2] * C / (2.0 * d50)
+             + lp[4] * nb * frac / (6.0 * d50m))
+    return float(d_d50), float(-lp[3])
 
 
 class _ExnerRunFn(torch.autograd.Function):
@@ -221,4 +225,4 @@ def seeded_target(z_b0, seed=6, amp=1e-3):
     return z_b0 + torch.as_tensor(p, dtype=z_b0.dtype, device=z_b0.device)
 
 
-__all__ = ["ExnerGradientRun", "seeded_target"]
+__all__ = ["ExnerGradientRun", "chain_sediment_gradients", "seeded_target"]

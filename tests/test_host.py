@@ -323,3 +323,50 @@ def test_ensemble_plan_sizes_and_rejections():
     mem[1].sed.mpm = 1
     mem[2].ny = 1                                                                  # not a grid
     assert lib.pxh_ensemble_plan(_hip.PXH_F32, 0, mem, 3, buf, size) < 0
+
+
+def test_adjoint_programs_and_workspace():
+    """The adjoint's ring-fill program holds the forward entries sorted by
+    (group, src), flagged PXH_BC_SRC_SORTED; pxh_adj_workspace_size (host-only)
+    adds two doubles per entry of the program it is given and refuses float32
+    grids (the adjoint is double only)."""
+    for mk in (lambda: cases.calibration_channel(nx=40, ny=20), lambda: cases.periodic_box(nx=24, ny=20)):
+        mesh, arr, params = mk()
+        bm = BoundaryManager(params, mesh.local_X, mesh.local_Y, device="cpu")
+        bc = bm.adjoint_program()
+        ent = bm._adj_entries
+        assert bc.flags == _hip.BC_GROUPED | _hip.BC_SRC_SORTED
+        key = ent["group"].astype(np.int64) * (1 << 32) + ent["src"].astype(np.int64)
+        assert np.all(np.diff(key) >= 0)
+        as_rows = lambda e: sorted(map(bytes, e.view(np.uint8).reshape(len(e), -1)))  # noqa: E731
+        assert as_rows(ent) == as_rows(bm.entries)
+        assert list(bc.group_start) == bm.group_start
+        lib = _hip.load()
+        g64 = _hip.Grid(20, 40, _hip.PXH_F64, 0, 0.02)
+        base = lib.pxh_adj_workspace_size(g64, None)
+        assert base >= 11 * 8 * 20 * 40
+        assert lib.pxh_adj_workspace_size(g64, bc) == base + 16 * bm.n_entries
+        assert lib.pxh_adj_workspace_size(_hip.Grid(20, 40, _hip.PXH_F32, 0, 0.02), None) == -1
+
+
+def test_sediment_gradient_chain_rule_against_table_differences():
+    """chain_sediment_gradients maps derivatives with respect to the table
+    entries (A, B, C, 1 - porosity, nb_sed) to d50 and porosity: checked
+    against central differences of sediment_table itself."""
+    from pyexner_amd.adjoint import chain_sediment_gradients
+    seds = seds_from_params(cases.calibration_channel(nx=8, ny=8)[2]).astype(np.float64)
+    rng = np.random.default_rng(3)
+    lam = rng.normal(size=5)
+
+    def functional(s):
+        t = sediment_table(s, False, np.float64)
+        return lam[0] * t.A[0] + lam[1] * t.B[0] + lam[2] * t.C[0] + lam[3] * t.one_minus_por + lam[4] * t.nb_sed
+
+    d_d50, d_por = chain_sediment_gradients(seds, lam)
+    for row, want in ((1, d_d50), (6, d_por)):
+        eps = 1e-6 * abs(seds[row, 0])
+        sp, sm = seds.copy(), seds.copy()
+        sp[row, 0] += eps
+        sm[row, 0] -= eps
+        fd = (functional(sp) - functional(sm)) / (2 * eps)
+        assert abs(fd - want) <= 1e-7 * abs(fd), (row, fd, want)
