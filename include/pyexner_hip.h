@@ -96,9 +96,32 @@ typedef struct {
 typedef struct {
     int32_t target, src;            /* linear cell indices (row*nx + col) */
     uint8_t group;                  /* PXH_G_* */
-    uint8_t nchain;                 /* handlers applied to src, in order (momentum only) */
+    uint8_t nchain;                 /* handlers applied to src, in order (momentum only); with
+                                       PXH_BC_VALUE set, a value entry: chain[0] indexes pxh_bc.values */
     uint8_t chain[PXH_MAX_CHAIN];
 } pxh_bc_entry;                     /* 16 bytes */
+
+/* Value entries: the target of a value boundary (domain/boundaries/roeexner_*.py
+ * beyond the copy handlers) takes a value computed from the spec rather than
+ * a copy of its source.  Kinds (a, b, c as in pxh_bc_value; t = pxh_bc.time):
+ *   PXH_BV_CONST    field = a; for PXH_G_HUV hu = a, hv = b (a NaN component
+ *                   keeps the cell's value: ZeroMomentum with a NaN value)
+ *   PXH_BV_AFFINE_T field = b * t + a (Berthon / NormalFlowDepth bed, zb0 - alpha t)
+ *   PXH_BV_OFFSET   field = field[src] + a (SteepFall: z_b of the interior - 1)
+ *   PXH_G_H only, PXH_BV_OUTFLUX: h from the outgoing Riemann invariant of the
+ *                   source cell at the prescribed discharge q = a through the
+ *                   normal (b, c), five Newton steps (roeexner_constantoutflux.py:24-56) */
+#define PXH_BC_VALUE 0x80
+#define PXH_BV_CONST 1
+#define PXH_BV_AFFINE_T 2
+#define PXH_BV_OFFSET 3
+#define PXH_BV_OUTFLUX 4
+#define PXH_MAX_VALUES 32
+typedef struct {
+    int32_t kind;                   /* PXH_BV_* */
+    int32_t reserved;
+    double a, b, c;
+} pxh_bc_value;
 
 typedef struct {
     int32_t kind;                   /* PXH_BC_* */
@@ -133,6 +156,10 @@ typedef struct {
     int32_t flags;                  /* PXH_BC_* flags */
     int32_t group_start[PXH_N_GROUPS + 1];  /* with PXH_BC_GROUPED; group_start[7] = n_entries */
     pxh_bc_handler handlers[PXH_MAX_HANDLERS];
+    double time;                    /* simulation time of the fill (PXH_BV_AFFINE_T values) */
+    int32_t n_values;
+    int32_t reserved;
+    pxh_bc_value values[PXH_MAX_VALUES];
 } pxh_bc;
 
 /* ---- library ------------------------------------------------------------ */
@@ -282,7 +309,8 @@ int pxh_ensemble_ssprk2(const void *plan, const void *plan_dev, const pxh_fields
  * gradient code; these are what jax.grad through its step_fn / compute_dt_fn /
  * init_fn (solvers/roe_exner_solver.py:58-122) would give, with the
  * derivative of every untaken branch taken as zero.  Adjoint fields ("lam") are (ny, nx) double arrays in
- * pxh_fields slots of the same names; MPM G only (PXH_E_UNSUPPORTED otherwise).
+ * pxh_fields slots of the same names; MPM G and ring fills without
+ * PXH_BV_OUTFLUX values only (PXH_E_UNSUPPORTED otherwise).
  * lam_dt: one device double; lam_par: five device doubles, the derivatives
  * with respect to the first sediment class's A, B, C (pxh_sediments),
  * one_minus_por and nb_sed, accumulated (may be NULL).  The ring-fill program

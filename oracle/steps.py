@@ -35,14 +35,14 @@ def init_roeexner(state, mask, bcs, constant):
     return bcs.apply(state)
 
 
-def step_roeexner(state, dt, mask, bcs, dx, constant):
+def step_roeexner(state, dt, mask, bcs, dx, constant, time=0.0):
     state = rx.roe_solve_2D(state, dt, dx, mask)
     state = rx.momentum_corrections(state, mask)
-    state = bcs.apply(state)
+    state = bcs.apply(state, time)
     G = rx.compute_G_mode(state.h, state.hu, state.hv, state.seds, mask, state.G, constant)
     state = state.replace(G=G)
     state = rx.exner_solve_2D(state, dt, dx, mask)
-    state = bcs.apply(state)
+    state = bcs.apply(state, time)
     # n_b recomputed then discarded (roe_exner_solver.py:112-114)
     return state
 
@@ -76,8 +76,8 @@ class ExnerProblem:
     def dt(self, s):
         return dt_roeexner(s, self.cfl, self.mask, self.dx, self.cubic_dtype)
 
-    def step(self, s, dt):
-        return step_roeexner(s, dt, self.mask, self.bcs, self.dx, self.constant)
+    def step(self, s, dt, time=0.0):
+        return step_roeexner(s, dt, self.mask, self.bcs, self.dx, self.constant, time)
 
 
 class RoeProblem:
@@ -87,29 +87,32 @@ class RoeProblem:
     def dt(self, s):
         return dt_roe(s, self.cfl, self.dx)
 
-    def step(self, s, dt):
+    def step(self, s, dt, time=0.0):
         return step_roe(s, dt, self.bcs, self.dx)
 
 
-def fe_step(prob, s):
+def fe_step(prob, s, time=0.0):
     dt = prob.dt(s)
-    return prob.step(s, dt), dt
+    return prob.step(s, dt, time), dt
 
 
-def ssprk2_step(prob, s):
+def ssprk2_step(prob, s, time=0.0):
+    """Both stages see the step's start time (SSPRK2.py:34-35)."""
     dt = prob.dt(s)
-    s1 = prob.step(s, dt)
-    s2 = prob.step(s1, dt)
+    s1 = prob.step(s, dt, time)
+    s2 = prob.step(s1, dt, time)
     return combine_half(s, s2), dt
 
 
 def run_steps(prob, s, nsteps, integrator="Forward Euler"):
+    """nsteps steps from time 0; the boundaries see each step's start time,
+    accumulated as time + dt in the state's dtype (forwardeuler.py:43)."""
     f = fe_step if integrator == "Forward Euler" else ssprk2_step
-    t = 0.0
+    t = s.h.dtype.type(0.0)
     for _ in range(nsteps):
-        s, dt = f(prob, s)
-        t += float(dt)
-    return s, t
+        s, dt = f(prob, s, t)
+        t = s.h.dtype.type(t + dt)
+    return s, float(t)
 
 
 def run_forward_euler(prob, s, end_time, out_freq, on_output=None):
@@ -130,17 +133,17 @@ def run_forward_euler(prob, s, end_time, out_freq, on_output=None):
             if not (time + dt < target - TIMESTEP_TOL):
                 break
             dt = prob.dt(s)
-            s = prob.step(s, dt)
+            s = prob.step(s, dt, time)
             time = dtype(time + dt)
             iters += 1
         dt = prob.dt(s)
         nxt = (int(time / out_freq) + 1) * out_freq
         target = min(nxt, end_time)
         if time + dt >= target - TIMESTEP_TOL:
-            probe = prob.step(s, dtype(target - time))
+            probe = prob.step(s, dtype(target - time), time)
             if on_output is not None:
                 on_output(target, probe)
-        s = prob.step(s, dt)
+        s = prob.step(s, dt, time)
         time = dtype(time + dt)
         iters += 1
     return s, float(time), iters

@@ -15,10 +15,14 @@ MAX_SED = 8
 MAX_HANDLERS = 16
 MAX_CHAIN = 6
 
+PXH_OK, PXH_E_ARG, PXH_E_RANGE, PXH_E_UNSUPPORTED = 0, -1, -2, -3
 BC_COPY, BC_REFLECT_RX, BC_REFLECT_ROE = 0, 1, 2
 BC_FRAME = 1  # pxh_bc.flags: every target on the outer ring, every source within two cells of the edge
 BC_GROUPED = 2  # pxh_bc.flags: entries ordered by group, group_start[] valid
 BC_SRC_SORTED = 4  # pxh_bc.flags: within a group, entries ordered by src (adjoint programs)
+BC_VALUE = 0x80  # pxh_bc_entry.nchain flag: a value entry, chain[0] indexes pxh_bc.values
+BV_CONST, BV_AFFINE_T, BV_OFFSET, BV_OUTFLUX = 1, 2, 3, 4
+MAX_VALUES = 32
 N_GROUPS = 7
 HALO_W, HALO_N, HALO_E, HALO_S = 1, 2, 4, 8  # pxh_halo_* sides
 SWEEP_SKIP_G = 1  # flags of the stage/frame sweeps: leave out.G unwritten in MPM mode
@@ -63,11 +67,18 @@ class BCHandler(ctypes.Structure):
                 ("ny", ctypes.c_double)]
 
 
+class BCValue(ctypes.Structure):     # pxh_bc_value
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("a", ctypes.c_double),
+                ("b", ctypes.c_double), ("c", ctypes.c_double)]
+
+
 class BC(ctypes.Structure):
     _fields_ = [("entries", ctypes.c_void_p), ("n_entries", ctypes.c_int32), ("n_handlers", ctypes.c_int32),
                 ("group_mask", ctypes.c_uint32), ("flags", ctypes.c_int32),
                 ("group_start", ctypes.c_int32 * (N_GROUPS + 1)),
-                ("handlers", BCHandler * MAX_HANDLERS)]
+                ("handlers", BCHandler * MAX_HANDLERS),
+                ("time", ctypes.c_double), ("n_values", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("values", BCValue * MAX_VALUES)]
 
 
 assert ctypes.sizeof(BCEntry) == 16

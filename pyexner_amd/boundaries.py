@@ -114,6 +114,30 @@ class Handler:
             fn = compute_periodic_indices if self.periodic else compute_reflective_indices
             self.boundary_indices, self.interior_indices = fn(self.mask, self.normal)
 
+    def value_spec(self, grp):
+        """(kind, a, b, c) of the value this handler writes into group grp
+        (pxh_bc_value), or None when it copies the group or leaves it."""
+        return None
+
+
+def _f32(v):
+    """A YAML boundary value as the reference holds it: a float32 array
+    element (boundary_registry.py:103-107)."""
+    return np.float32(v)
+
+
+def _berthon_profile(vals):
+    """u, h = q / u and zb0 of the Berthon et al. steady state at the boundary's
+    x (roeexner_berthon.py:35-49, roeexner_normalflowdepth.py:31-40), in
+    float32 like the reference's float32 values: values = [q, A, alpha, beta,
+    gamma, x]."""
+    q, A, alpha, beta, gamma, x = (_f32(v) for v in vals[:6])
+    u = np.power(np.float32((alpha * x + beta) / A), np.float32(1.0 / 3.0))
+    h = np.float32(q / u)
+    two_g = np.float32(2.0 * 9.81)
+    zb0 = np.float32(-(u * (u * u) + two_g * q) / (two_g * np.maximum(u, np.float32(1e-12))) + gamma)
+    return u, h, zb0, alpha
+
 
 @register_boundary("Roe Exner Reflective")
 class RoeExner_ReflectiveBoundary(Handler):
@@ -138,6 +162,74 @@ class RoeExner_TransmissiveBedBoundary(Handler):
 class RoeExner_PeriodicBoundary(Handler):
     groups = (G.G_H, G.G_HUV, G.G_Z, G.G_ZB, G.G_N, G.G_NB, G.G_G)
     periodic = True
+
+
+@register_boundary("Roe Exner ZeroMomentum")
+class RoeExner_ZeroMomentumBoundary(Handler):
+    """roeexner_zeromomentum.py:17-35: hu = values[1] * normal[0], hv =
+    values[1] * normal[1] on every boundary cell (a NaN keeps the cell's)."""
+    def value_spec(self, grp):
+        if grp != G.G_HUV:
+            return None
+        v1 = float(_f32(self.values[1]))
+        return (_hip.BV_CONST, v1 * float(self.normal[0]), v1 * float(self.normal[1]), 0.0)
+
+
+@register_boundary("Roe Exner ConstantInflux")
+class RoeExner_ConstantInfluxBoundary(Handler):
+    """roeexner_constantinflux.py:17-47: (hu, hv) = q (values[1], values[2])
+    with q = values[0]."""
+    def value_spec(self, grp):
+        if grp != G.G_HUV:
+            return None
+        q, nx, ny = (_f32(v) for v in self.values[:3])
+        return (_hip.BV_CONST, float(np.float32(q * nx)), float(np.float32(q * ny)), 0.0)
+
+
+@register_boundary("Roe Exner ConstantOutflux")
+class RoeExner_ConstantOutfluxBoundary(Handler):
+    """roeexner_constantoutflux.py:20-71: h from the interior cell's outgoing
+    Riemann invariant at discharge q = values[0] through (values[1],
+    values[2]); (hu, hv) = q (values[1], values[2])."""
+    def value_spec(self, grp):
+        q, nx, ny = (_f32(v) for v in self.values[:3])
+        if grp == G.G_H:
+            return (_hip.BV_OUTFLUX, float(q), float(nx), float(ny))
+        if grp == G.G_HUV:
+            return (_hip.BV_CONST, float(np.float32(q * nx)), float(np.float32(q * ny)), 0.0)
+        return None
+
+
+@register_boundary("Roe Exner NormalFlowDepth")
+class RoeExner_NormalFlowDepthBoundary(Handler):
+    """roeexner_normalflowdepth.py:20-47: h = q / u of the Berthon profile,
+    (hu, hv) copied from the interior cell, z_b = zb0 - alpha t."""
+    groups = (G.G_HUV,)
+
+    def value_spec(self, grp):
+        u, h, zb0, alpha = _berthon_profile(self.values)
+        if grp == G.G_H:
+            return (_hip.BV_CONST, float(h), 0.0, 0.0)
+        if grp == G.G_ZB:
+            return (_hip.BV_AFFINE_T, float(zb0), -float(alpha), 0.0)
+        return None
+
+
+@register_boundary("Roe Exner Berthon")
+class RoeExner_BerthonBoundary(Handler):
+    """roeexner_berthon.py:19-53: z_b = zb0 - alpha t of the Berthon profile."""
+    def value_spec(self, grp):
+        if grp != G.G_ZB:
+            return None
+        u, h, zb0, alpha = _berthon_profile(self.values)
+        return (_hip.BV_AFFINE_T, float(zb0), -float(alpha), 0.0)
+
+
+@register_boundary("Roe Exner SteepFall")
+class RoeExner_SteepFall(Handler):
+    """roeexner_steepfall.py:17-26: z_b = z_b of the interior cell - 1."""
+    def value_spec(self, grp):
+        return (_hip.BV_OFFSET, -1.0, 0.0, 0.0) if grp == G.G_ZB else None
 
 
 @register_boundary("Roe Reflective")
@@ -170,32 +262,96 @@ ENTRY_DTYPE = np.dtype([("target", "<i4"), ("src", "<i4"), ("group", "u1"), ("nc
                         ("chain", "u1", (_hip.MAX_CHAIN,))])
 
 
-def compose(handlers, shape):
-    """Resolve the sequential handlers into per-target (src, chain) entries.
+def _reflect(hd, hu, hv):
+    """The momentum transform of handler hd (as bc_transform applies it) on
+    host floats."""
+    nx, ny = float(hd.normal[0]), float(hd.normal[1])
+    if hd.kind == _hip.BC_REFLECT_RX:
+        mn = hu * nx + hv * ny
+        mt = -hu * ny + hv * nx
+        mr = -mn
+        return mr * nx - mt * ny, mr * ny + mt * nx
+    if hd.kind == _hip.BC_REFLECT_ROE:
+        return hu * (1.0 - 2.0 * nx * nx), hv * (1.0 - 2.0 * ny * ny)
+    return hu, hv
 
-    Returns (entries: structured numpy array of pxh_bc_entry, idempotent: bool).
-    A program is idempotent when no entry's source is a target of the same
-    group: then one launch reads only values the launch never writes, and
-    applying it twice equals applying it once."""
+
+def _spec_key(spec):
+    """Identity of a value spec bit for bit (-0.0 differs from 0.0, NaN equals NaN)."""
+    return (spec[0],) + tuple(float(v).hex() for v in spec[1:])
+
+
+def compose(handlers, shape):
+    """Resolve the sequential handlers into per-target entries.
+
+    Returns (entries: structured numpy array of pxh_bc_entry, idempotent: bool,
+    values: list of (kind, a, b, c) pxh_bc_value specs the value entries
+    index).  Each (group, target) ends up as one entry: a copy of a source
+    cell through a chain of momentum transforms, or a value (a constant, an
+    affine function of time, an offset of a source cell, or the outflux depth
+    of a source cell).  A program is idempotent when no entry reads a cell
+    the program writes in a group the entry reads: then one launch reads only
+    values the launch never writes, and applying it twice equals applying it
+    once."""
     ny, nx = shape
     rows = []
     idempotent = True
+    values, value_keys = [], []
+    # cells written by handlers before handler k, in any group (a value
+    # handler reading such a cell is not supported)
+    written_before = []
+    acc = set()
+    for hd in handlers:
+        written_before.append(set(acc))
+        acc |= set((hd.boundary_indices[:, 0] * nx + hd.boundary_indices[:, 1]).tolist())
+    targets_of = {}
+    reads = []  # (src, groups read) of the entries
     for grp in range(7):
-        src_map, chain_map = {}, {}
+        src_map, chain_map, val_map = {}, {}, {}
         for k, hd in enumerate(handlers):
-            if grp not in hd.groups:
+            spec = hd.value_spec(grp)
+            copies = grp in hd.groups
+            if spec is None and not copies:
                 continue
             b = hd.boundary_indices[:, 0] * nx + hd.boundary_indices[:, 1]
             i = hd.interior_indices[:, 0] * nx + hd.interior_indices[:, 1]
             transform = grp == G.G_HUV and hd.kind != _hip.BC_COPY
             new = []
             for bb, ii in zip(b.tolist(), i.tolist()):
-                ch = chain_map.get(ii, ())
-                new.append((bb, src_map.get(ii, ii), ch + (k,) if transform else ch))
-            for bb, s, ch in new:
-                src_map[bb] = s
-                chain_map[bb] = ch
-        targets = set(src_map)
+                if spec is not None:
+                    if spec[0] == _hip.BV_OFFSET:
+                        # an offset of a cell an earlier handler copied into reads that copy's source
+                        if ii in val_map or chain_map.get(ii):
+                            raise NotImplementedError("an offset boundary reading another boundary's value")
+                        new.append((bb, ("val", spec, src_map.get(ii, ii))))
+                    elif spec[0] == _hip.BV_OUTFLUX:
+                        if ii in written_before[k]:
+                            raise NotImplementedError("an outflux boundary reading a cell an earlier handler writes")
+                        new.append((bb, ("val", spec, ii)))
+                    else:
+                        new.append((bb, ("val", spec, bb)))
+                elif ii in val_map:
+                    vs, vsrc = val_map[ii]
+                    if transform:
+                        # a reflected constant momentum is a constant
+                        if vs[0] != _hip.BV_CONST or vs[1] != vs[1] or vs[2] != vs[2]:
+                            raise NotImplementedError("a reflection of a value boundary's momentum")
+                        vs = (_hip.BV_CONST,) + _reflect(hd, vs[1], vs[2]) + (0.0,)
+                    new.append((bb, ("val", vs, vsrc if vs[0] in (_hip.BV_OFFSET, _hip.BV_OUTFLUX) else bb)))
+                else:
+                    ch = chain_map.get(ii, ())
+                    new.append((bb, ("copy", src_map.get(ii, ii), ch + (k,) if transform else ch)))
+            for bb, x in new:
+                if x[0] == "val":
+                    val_map[bb] = (x[1], x[2])
+                    src_map.pop(bb, None)
+                    chain_map.pop(bb, None)
+                else:
+                    src_map[bb] = x[1]
+                    chain_map[bb] = x[2]
+                    val_map.pop(bb, None)
+        targets = set(src_map) | set(val_map)
+        targets_of[grp] = targets
         for t, s in src_map.items():
             ch = chain_map[t]
             if s == t and not ch:
@@ -204,13 +360,33 @@ def compose(handlers, shape):
                 idempotent = False
             if len(ch) > _hip.MAX_CHAIN:
                 raise NotImplementedError(f"boundary chain of {len(ch)} reflections exceeds {_hip.MAX_CHAIN}")
-            rows.append((t, s, grp, ch))
+            rows.append((t, s, grp, ch, None))
+        for t, (spec, src) in val_map.items():
+            key = _spec_key(spec)
+            if key not in value_keys:
+                value_keys.append(key)
+                values.append(spec)
+            if spec[0] == _hip.BV_OFFSET:
+                reads.append((src, (grp,)))
+            elif spec[0] == _hip.BV_OUTFLUX:
+                reads.append((src, (G.G_H, G.G_HUV)))
+            rows.append((t, src, grp, (), value_keys.index(key)))
+    for src, grps in reads:
+        if any(src in targets_of.get(g_, ()) for g_ in grps):
+            idempotent = False
+    if len(values) > _hip.MAX_VALUES:
+        raise NotImplementedError(f"more than {_hip.MAX_VALUES} boundary values")
     ent = np.zeros(len(rows), dtype=ENTRY_DTYPE)
-    for q, (t, s, grp, ch) in enumerate(rows):
-        ent[q]["target"], ent[q]["src"], ent[q]["group"], ent[q]["nchain"] = t, s, grp, len(ch)
-        ent[q]["chain"][:len(ch)] = ch
+    for q, (t, s, grp, ch, vi) in enumerate(rows):
+        ent[q]["target"], ent[q]["src"], ent[q]["group"] = t, s, grp
+        if vi is None:
+            ent[q]["nchain"] = len(ch)
+            ent[q]["chain"][:len(ch)] = ch
+        else:
+            ent[q]["nchain"] = _hip.BC_VALUE
+            ent[q]["chain"][0] = vi
     assert ent.itemsize == 16
-    return ent, idempotent
+    return ent, idempotent, values
 
 
 def frame_local(entries, shape):
@@ -228,6 +404,14 @@ def frame_local(entries, shape):
     ring = (tr == 0) | (tr == ny - 1) | (tc == 0) | (tc == nx - 1)
     frame = (sr <= 1) | (sr >= ny - 2) | (sc <= 1) | (sc >= nx - 2)
     return bool(np.all(ring) and np.all(frame))
+
+
+def set_values(bc, values, time=0.0):
+    """Fill pxh_bc's value table from (kind, a, b, c) specs."""
+    bc.n_values = len(values)
+    bc.time = float(time)
+    for k, (kind, a, b, c) in enumerate(values):
+        bc.values[k].kind, bc.values[k].a, bc.values[k].b, bc.values[k].c = kind, a, b, c
 
 
 class BoundaryManager:
@@ -248,12 +432,14 @@ class BoundaryManager:
                 raise KeyError(f"Unknown boundary '{btype}'. Available options: {list(BOUNDARY_REGISTRY.keys())}")
             m = compute_label_mask(self.X, self.Y, spec["polygon"])
             masks.append(m)
-            vals = np.array([np.nan if str(v).lower() == "nan" else float(v) for v in spec.get("values", [])])
+            vals = np.array([np.nan if str(v).lower() == "nan" else float(v) for v in spec.get("values", [])],
+                            dtype=np.float32).astype(np.float64)
             self.boundary_handlers.append(BOUNDARY_REGISTRY[btype](mask=m, normal=spec["normal"], values=vals))
         self.boundary_mask = np.any(np.stack(masks), axis=0) if masks else np.zeros(shape, dtype=bool)
         if len(self.boundary_handlers) > _hip.MAX_HANDLERS:
             raise NotImplementedError(f"more than {_hip.MAX_HANDLERS} boundary handlers")
-        entries, self.idempotent = compose(self.boundary_handlers, shape)
+        entries, self.idempotent, self.values = compose(self.boundary_handlers, shape)
+        self.time_dependent = any(v[0] == _hip.BV_AFFINE_T for v in self.values)
         self.entries = entries
         self.n_entries = len(entries)
         self.frame_local = frame_local(entries, shape)
@@ -281,6 +467,7 @@ class BoundaryManager:
             bc.handlers[k].kind = hd.kind
             bc.handlers[k].nx = float(hd.normal[0])
             bc.handlers[k].ny = float(hd.normal[1])
+        set_values(bc, self.values)
         return bc
 
     def adjoint_program(self, groups=None):
@@ -299,17 +486,18 @@ class BoundaryManager:
         bc.flags = _hip.BC_GROUPED | _hip.BC_SRC_SORTED
         return bc
 
-    def apply_inplace(self, state, groups=None):
+    def apply_inplace(self, state, groups=None, time=0.0):
         from .state import fields_struct, dtype_code
         if self.n_entries == 0:
             return state
         grid = _hip.Grid(state.h.shape[0], state.h.shape[1], dtype_code(state.h.dtype), 0, 0.0)
         fs = fields_struct(state)
         bc = self.program(groups)
+        bc.time = float(time)
         stream = torch.cuda.current_stream().cuda_stream
         _hip.call("pxh_bc_apply", grid, fs, bc, stream)
         return state
 
     def apply(self, state, time=0.0):
         """Functional ring fill: returns a new state (boundary_registry.py:183-187)."""
-        return self.apply_inplace(state.clone())
+        return self.apply_inplace(state.clone(), time=time)

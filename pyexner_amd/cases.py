@@ -196,3 +196,60 @@ def calibration_channel(nx=1024, ny=256, dx=0.02, n_bumps=8, seed=5):
                                     ("inlet_bed", "TransmissiveBed", "W"), ("outlet_bed", "TransmissiveBed", "E"),
                                     ("south", "Reflective", "S"), ("north", "Reflective", "N")])
     return mesh, arrays, _exner_params(mesh, bounds)
+
+
+def _with_values(bounds, name, values):
+    bounds[name] = dict(bounds[name], values=[float(v) for v in values])
+    return bounds
+
+
+def berthon_channel(nx=140, ny=22, dx=0.05, gamma=1.2, grass=0.005, integrator="Forward Euler"):
+    """The Berthon et al. steady state of tests/runtime/2D_const_flux
+    (2D_build_num_sol.py:8-56: q = 1, A = alpha = beta = 0.005, constant G)
+    on one-cell ring polygons: Berthon bed and ConstantInflux at the inlet,
+    NormalFlowDepth at the outlet, Reflective walls.  gamma lifts the bed so
+    that it carries sediment (the reference's gamma = 1 leaves it below
+    SED_TOL, where the Exner update is off)."""
+    mesh = uniform_mesh(ny, nx, dx)
+    X = mesh.local_X
+    A = alpha = beta = 0.005
+    q, g = 1.0, 9.81
+    xf = X - dx / 2  # file x, where the reference evaluates the profile
+    u = ((alpha * xf + beta) / A) ** (1.0 / 3.0)
+    h = q / u
+    zb = -(u ** 3 + 2 * g * q) / (2 * g * u) + gamma
+    arrays = {"h": h, "hu": h * u, "hv": np.zeros_like(h), "z": np.zeros_like(h), "z_b": zb,
+              "n": np.zeros_like(h)}
+    bounds = ring_boundaries(mesh, [("bottom", "Reflective", "S"), ("inlet_bed", "Berthon", "W"),
+                                    ("inlet_flux", "ConstantInflux", "W"), ("outlet_bed", "NormalFlowDepth", "E"),
+                                    ("top", "Reflective", "N")])
+    prof = [q, A, alpha, beta, gamma]
+    _with_values(bounds, "inlet_bed", prof + [float(xf[0, 0])])
+    _with_values(bounds, "inlet_flux", [q, 1.0, 0.0])
+    _with_values(bounds, "outlet_bed", prof + [float(xf[0, -1])])
+    params = {"flux_scheme": "Roe Exner", "integrator": integrator, "cfl": 0.5, "end_time": 1.0, "out_freq": 1.0,
+              "parNx": 1, "parNy": 1, "erosion": {"grass_factor": grass}, "boundaries": bounds}
+    return mesh, arrays, params
+
+
+def outflux_channel(nx=120, ny=30, dx=0.02, seed=12, q=0.05, steep_fall=False):
+    """An MPM channel with a prescribed discharge: ConstantInflux and
+    TransmissiveBed at the inlet, ConstantOutflux and TransmissiveBed at the
+    outlet (with steep_fall, SteepFall instead: the bed one metre below the
+    interior's, which drains this shallow subcritical channel within a few
+    hundred steps), a Reflective north wall, and a south wall that reflects and
+    then zeroes the momentum (ZeroMomentum)."""
+    mesh = uniform_mesh(ny, nx, dx)
+    rng = np.random.default_rng(seed)
+    zb = 0.02 + gaussian_bumps(mesh.local_X, mesh.local_Y, rng, 6, amp=(0.0, 0.01))
+    h = np.full(zb.shape, 0.1)
+    arrays = {"h": h, "hu": h * 0.5, "hv": np.zeros_like(h), "z": np.zeros_like(h), "z_b": zb,
+              "n": np.full_like(h, 0.02), "G": np.zeros_like(h)}
+    bounds = ring_boundaries(mesh, [("inlet", "ConstantInflux", "W"), ("inlet_bed", "TransmissiveBed", "W"),
+                                    ("outlet", "ConstantOutflux", "E"),
+                                    ("outlet_bed", "SteepFall" if steep_fall else "TransmissiveBed", "E"),
+                                    ("north", "Reflective", "N"), ("south", "Reflective", "S"),
+                                    ("south_wall", "ZeroMomentum", "S")])
+    _with_values(bounds, "inlet", [q, 1.0, 0.0])
+    _with_values(bounds, "outlet", [q, 1.0, 0.0])
+    return mesh, arrays, _exner_params(mesh, bounds)

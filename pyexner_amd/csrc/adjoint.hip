@@ -50,6 +50,14 @@ inline AdjWs ws_carve(int ny, int nx, void *ws) {
 
 inline bool adj_grid_ok(const pxh_grid *g) { return grid_ok(g) && g->dtype == PXH_F64; }
 
+// The outflux depth (a Newton solve from the source cell) has no adjoint here.
+inline bool bc_adj_ok(const pxh_bc *bc) {
+    if (!bc) return true;
+    for (int k = 0; k < bc->n_values; ++k)
+        if (bc->values[k].kind == PXH_BV_OUTFLUX) return false;
+    return true;
+}
+
 inline bool lam_ok(const pxh_fields *l, bool zb, bool G) {
     return l && l->h && l->hu && l->hv && (!zb || l->z_b) && (!G || l->G);
 }
@@ -145,7 +153,7 @@ int pxh_adj_roe_exner_stage(const pxh_grid *g, const pxh_fields *in, const pxh_f
     if (!adj_grid_ok(g) || !fields_ok(in, true) || !lam_ok(out, false, false) || !m0 || !sd || !dt_dev ||
         !lam_ok(lam_out, true, true) || !lam_ok(lam_in, true, false) || !lam_dt || !ws || !bc_ok(bc))
         return PXH_E_ARG;
-    if (!sd->mpm) return PXH_E_UNSUPPORTED;
+    if (!sd->mpm || !bc_adj_ok(bc)) return PXH_E_UNSUPPORTED;
     const AdjWs w = ws_carve(g->ny, g->nx, ws);
     return stage_t(g, cast_fields<double>(in), cast_fields<double>(out), m0, sd, bc,
                    static_cast<const double *>(dt_dev), cast_fields<double>(lam_out), cast_fields<double>(lam_in),
@@ -167,7 +175,7 @@ int pxh_adj_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a_, const pxh_
     if (!adj_grid_ok(g) || !fields_ok(a_, true) || !lam_ok(b_, true, false) || !lam_ok(c_, false, false) || !m0 ||
         !sd || !dt_dev || !lam_ok(lam, true, true) || !ws || !bc_ok(bc))
         return PXH_E_ARG;
-    if (!sd->mpm) return PXH_E_UNSUPPORTED;
+    if (!sd->mpm || !bc_adj_ok(bc)) return PXH_E_UNSUPPORTED;
     hipStream_t st = hipStream_t(stream);
     const AdjWs w = ws_carve(g->ny, g->nx, ws);
     const F<double> a = cast_fields<double>(a_), la = cast_fields<double>(lam);
@@ -193,7 +201,7 @@ int pxh_adj_roe_exner_init(const pxh_grid *g, const pxh_fields *s, const uint8_t
                            const pxh_bc *bc, const pxh_fields *lam, void *lam_par, void *ws, void *stream) {
     if (!adj_grid_ok(g) || !fields_ok(s, true) || !m0 || !sd || !lam_ok(lam, true, true) || !ws || !bc_ok(bc))
         return PXH_E_ARG;
-    if (!sd->mpm) return PXH_E_UNSUPPORTED;
+    if (!sd->mpm || !bc_adj_ok(bc)) return PXH_E_UNSUPPORTED;
     if (lam_par && !lam->n_b) return PXH_E_ARG;
     hipStream_t st = hipStream_t(stream);
     const AdjWs w = ws_carve(g->ny, g->nx, ws);

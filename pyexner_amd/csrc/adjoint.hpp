@@ -108,7 +108,21 @@ __global__ __launch_bounds__(256) void k_adj_bc_take(F<double> lam, const pxh_bc
     if (k >= n) return;
     const pxh_bc_entry e = ent[k];
     double a = 0.0, b = 0.0;
-    if ((groups >> e.group) & 1u) {
+    if (((groups >> e.group) & 1u) && (e.nchain & PXH_BC_VALUE)) {
+        // a set value carries no derivative from the state (the target's
+        // adjoint stops there); an offset passes it on like a copy; a NaN
+        // momentum component leaves the cell as it was (identity)
+        const pxh_bc_value v = bc.values[e.chain[0]];
+        double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
+        const int t = e.target;
+        const bool keep0 = v.kind == PXH_BV_CONST && e.group == PXH_G_HUV && !(v.a == v.a);
+        const bool keep1 = v.kind == PXH_BV_CONST && e.group == PXH_G_HUV && !(v.b == v.b);
+        if (f0 && !keep0) {
+            if (v.kind == PXH_BV_OFFSET) a = f0[t];
+            f0[t] = 0.0;
+        }
+        if (f1 && !keep1) f1[t] = 0.0;
+    } else if ((groups >> e.group) & 1u) {
         double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
         const int t = e.target;
         if (f0) {

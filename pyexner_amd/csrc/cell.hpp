@@ -137,6 +137,78 @@ __device__ __forceinline__ void bc_transform(const pxh_bc_handler &hd, T &hu, T 
     }
 }
 
+// The field of a ring-fill group (PXH_G_HUV: hu; its hv is q = 1).
+template <typename T>
+__device__ __forceinline__ T *group_field(const F<T> &s, int group, int q = 0) {
+    switch (group) {
+        case PXH_G_H: return s.h;
+        case PXH_G_HUV: return q ? s.hv : s.hu;
+        case PXH_G_Z: return s.z;
+        case PXH_G_ZB: return s.z_b;
+        case PXH_G_N: return s.n;
+        case PXH_G_NB: return s.n_b;
+        case PXH_G_G: return s.G;
+        default: return nullptr;
+    }
+}
+
+// The depth a ConstantOutflux boundary sets (roeexner_constantoutflux.py:24-56):
+// five Newton steps on q/h + 2 sqrt(g h) = un_i + 2 sqrt(g h_i), the outgoing
+// Riemann invariant of the interior cell, from h = h_i.
+template <typename T>
+__device__ __forceinline__ T outflux_depth(T h_src, T hu_src, T hv_src, T q, T nx, T ny) {
+    const T g = C<T>(GRAV), fl = C<T>(1e-8);
+    const T h_i = pmax(h_src, fl);
+    const T u_i = hu_src / h_i, v_i = hv_src / h_i;
+    const T un_i = u_i * nx + v_i * ny;
+    const T c_i = d_sqrt(g * h_i);
+    const T Rp = un_i + T(2) * c_i;
+    T h = h_i;
+    for (int k = 0; k < 5; ++k) {
+        h = pmax(h, fl);
+        const T c = d_sqrt(g * h);
+        const T f = q / h + T(2) * c - Rp;
+        const T df = -q / (h * h) + g / pmax(c, fl);
+        h = pmax(h - f / df, fl);
+    }
+    return h;
+}
+
+// A value entry of the ring fill (pxh_bc_value); the compensated-bed low part
+// of a set bed is zero, of an offset bed the source's.
+template <typename T>
+__device__ __forceinline__ void bc_value_apply(const F<T> &s, const pxh_bc_entry &e, const pxh_bc &bc) {
+    const pxh_bc_value v = bc.values[e.chain[0]];
+    const int t = e.target, src = e.src;
+    T *f = group_field(s, e.group);
+    if (!f) return;
+    switch (v.kind) {
+        case PXH_BV_CONST:
+            if (e.group == PXH_G_HUV) {
+                if (v.a == v.a) s.hu[t] = static_cast<T>(v.a);
+                if (v.b == v.b) s.hv[t] = static_cast<T>(v.b);
+            } else {
+                f[t] = static_cast<T>(v.a);
+                if (e.group == PXH_G_ZB && s.z_b_lo) s.z_b_lo[t] = T(0);
+            }
+            break;
+        case PXH_BV_AFFINE_T:
+            f[t] = static_cast<T>(v.b) * static_cast<T>(bc.time) + static_cast<T>(v.a);
+            if (e.group == PXH_G_ZB && s.z_b_lo) s.z_b_lo[t] = T(0);
+            break;
+        case PXH_BV_OFFSET:
+            f[t] = f[src] + static_cast<T>(v.a);
+            if (e.group == PXH_G_ZB && s.z_b_lo) s.z_b_lo[t] = s.z_b_lo[src];
+            break;
+        case PXH_BV_OUTFLUX:
+            if (e.group == PXH_G_H)
+                s.h[t] = outflux_depth<T>(s.h[src], s.hu[src], s.hv[src], static_cast<T>(v.a), static_cast<T>(v.b),
+                                          static_cast<T>(v.c));
+            break;
+        default: break;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host-side argument checks
 // ---------------------------------------------------------------------------
@@ -150,6 +222,7 @@ static inline bool grid_ok(const pxh_grid *g) {
 static inline bool bc_ok(const pxh_bc *bc) {
     if (!bc) return true;
     if (bc->n_handlers < 0 || bc->n_handlers > PXH_MAX_HANDLERS || bc->n_entries < 0) return false;
+    if (bc->n_values < 0 || bc->n_values > PXH_MAX_VALUES) return false;
     if (bc->n_entries > 0 && !bc->entries) return false;
     if (bc->flags & PXH_BC_GROUPED) {
         if (bc->group_start[0] != 0 || bc->group_start[PXH_N_GROUPS] != bc->n_entries) return false;

@@ -312,6 +312,10 @@ __global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restri
     while (r + 1 < sel.n && k >= sel.off[r + 1]) ++r;
     const pxh_bc_entry e = ent[sel.start[r] + (k - sel.off[r])];
     if (!((bc.group_mask >> e.group) & 1u)) return;
+    if (e.nchain & PXH_BC_VALUE) {
+        bc_value_apply<T>(s, e, bc);
+        return;
+    }
     const int t = e.target, src = e.src;
     switch (e.group) {
         case PXH_G_H: s.h[t] = s.h[src]; break;

@@ -41,6 +41,8 @@ class Ensemble:
             state = solver.init_fn(state, mask, sc)
             if bm.n_entries and not bm.frame_local:
                 raise NotImplementedError("ensemble members need frame-local rings (PXH_BC_FRAME)")
+            if bm.values:
+                raise NotImplementedError("ensemble members take copy and reflection boundaries only")
             hd = [(h.kind, float(h.normal[0]), float(h.normal[1])) for h in bm.boundary_handlers]
             if handlers is None:
                 handlers = hd

@@ -2,7 +2,8 @@
 solver bundle -> integrator bundle -> run.
 
 ``run_driver`` takes the path of a YAML file (or the parsed dict) whose
-``input_file`` is a NetCDF-3 initial condition.  ``setup_case`` / ``run_case``
+``input_file`` is a NetCDF-3 initial condition; with an ``output_file`` it
+writes the output windows' snapshots there (pyexner_amd.output.NetCDFWriter).  ``setup_case`` / ``run_case``
 take in-memory arrays instead (the form the synthetic workloads use, like
 tests/runtime/2D_dambreak/run2D_dambreak.py:139-162).
 """
@@ -61,4 +62,15 @@ def run_driver(config_path, dtype=torch.float32, device="cuda", io=None):
     if not os.path.isabs(path):
         path = os.path.join(base, path)
     mesh, arrays = mesh_and_arrays_from_netcdf(path, params)
-    return run_case(mesh, arrays, params, dtype, device, io)
+    out = params.get("output_file")
+    if io is not None or not out:
+        return run_case(mesh, arrays, params, dtype, device, io)
+    # the snapshot file of the YAML's output_file (driver.py:44-45), resolved
+    # like input_file
+    from ..inputs import read_netcdf
+    from ..output import NetCDFWriter
+    if not os.path.isabs(out):
+        out = os.path.join(base, out)
+    x, y, _ = read_netcdf(path, names=())
+    with NetCDFWriter(out, mesh, x=x, y=y) as writer:
+        return run_case(mesh, arrays, params, dtype, device, writer)

@@ -93,7 +93,11 @@ class ExnerBackend:
         if self.bc_all is not None and self.bc_all.n_entries:
             _hip.call("pxh_bc_apply", self.grid, fs, self.bc_all, stream)
 
-    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream):
+    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream, time=0.0):
+        """One stage; time: the simulation time the ring fill sees (value
+        boundaries whose bed moves with time)."""
+        if self.bc_step is not None:
+            self.bc_step.time = float(time)
         _hip.call("pxh_roe_exner_step", self.grid, fin, fout, mask0_ptr, self.sed, self.bc_step, dt_ptr, stream)
 
     def dt_into(self, fs, mask0_ptr, cfl, dt_ptr, stream):
@@ -103,9 +107,12 @@ class ExnerBackend:
     def combine_into(self, fa, fb, fo, stream):
         _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
 
-    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False):
+    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False, time=0.0):
         """dt (unless dt_ready), two stages and the combine; a is advanced in
-        place and dt_ptr receives the dt of the new a."""
+        place and dt_ptr receives the dt of the new a.  Both stages' ring
+        fills see `time`, the step's start time (SSPRK2.py:34-35)."""
+        if self.bc_step is not None:
+            self.bc_step.time = float(time)
         _hip.call("pxh_roe_exner_ssprk2", self.grid, fa, fb, fc, mask0_ptr, self.sed, self.bc_step, float(cfl),
                   dt_ptr, self.workspace.data_ptr(), int(bool(dt_ready)), stream)
 
@@ -181,7 +188,7 @@ def step_fn_roeexner(state, time, dt, mask, config):
     be = config.backend
     out = _new_evolving(state)
     dtt = _dt_tensor(dt, state.h.dtype, state.h.device)
-    be.step_into(fields_struct(state), fields_struct(out), _mask0_ptr(mask), dtt.data_ptr(), _stream())
+    be.step_into(fields_struct(state), fields_struct(out), _mask0_ptr(mask), dtt.data_ptr(), _stream(), time=time)
     return out
 
 

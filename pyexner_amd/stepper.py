@@ -35,6 +35,7 @@ class ExnerStepper:
         self.fa, self.fb, self.fc = (fields_struct(s) for s in (self.a, self.b, self.c))
         self.dt = torch.zeros((), dtype=state.h.dtype, device=state.h.device)
         self.dt_ready = False  # self.dt holds compute_dt_fn(self.a)
+        self.time = 0.0
         self.graph = None
 
     @property
@@ -48,17 +49,20 @@ class ExnerStepper:
         self.be.dt_into(self.fa, self.m0, self.cfl, self.dt.data_ptr(), self._stream())
         self.dt_ready = True
 
-    def ssprk2_step(self):
-        """One SSPRK2 step; afterwards self.dt holds the dt of the new state."""
+    def ssprk2_step(self, time=None):
+        """One SSPRK2 step; afterwards self.dt holds the dt of the new state.
+        time: the step's start time for time-dependent value boundaries
+        (default self.time; the stepper does not read dt back, so a caller
+        with such boundaries advances self.time itself)."""
         self.be.ssprk2_into(self.fa, self.fb, self.fc, self.m0, self.cfl, self.dt.data_ptr(), self._stream(),
-                            dt_ready=self.dt_ready)
+                            dt_ready=self.dt_ready, time=self.time if time is None else time)
         self.dt_ready = True
 
     def fe_step(self):
         st = self._stream()
         dtp = self.dt.data_ptr()
         self.be.dt_into(self.fa, self.m0, self.cfl, dtp, st)
-        self.be.step_into(self.fa, self.fb, self.m0, dtp, st)
+        self.be.step_into(self.fa, self.fb, self.m0, dtp, st, time=self.time)
         self.a, self.b = self.b, self.a
         self.fa, self.fb = self.fb, self.fa
         self.dt_ready = False

@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import _hip
-from .boundaries import ENTRY_DTYPE
+from .boundaries import ENTRY_DTYPE, set_values
 from .state import fields_struct
 
 HW = 2  # ghost rings on a shared side
@@ -76,7 +76,7 @@ class Tile:
     """One tile: its extent in the global arrays, three buffer sets, grid,
     mask, ring-fill program and dt workspace."""
 
-    def __init__(self, backend, state, mask, entries, handlers, pos, par, rows, cols, device):
+    def __init__(self, backend, state, mask, entries, handlers, pos, par, rows, cols, device, values=()):
         ty, tx = pos
         pny, pnx = par
         r0, r1 = rows[ty]
@@ -116,6 +116,7 @@ class Tile:
                 bc.group_start[k] = start[k]
             for k, (kind, nxn, nyn) in enumerate(handlers):
                 bc.handlers[k].kind, bc.handlers[k].nx, bc.handlers[k].ny = kind, nxn, nyn
+            set_values(bc, values)
             self.programs[key] = bc
         ws = _hip.load().pxh_dt_workspace_size(self.grid)
         self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
@@ -145,11 +146,14 @@ class TiledExner:
         ny, nx = state.h.shape
         self.rows, self.cols = split(ny, pny), split(nx, pnx)
         entries = boundaries.entries if boundaries is not None else np.zeros(0, dtype=ENTRY_DTYPE)
+        values = boundaries.values if boundaries is not None else []
+        if boundaries is not None and boundaries.time_dependent:
+            raise NotImplementedError("tiles run at time 0: boundaries whose values move with time are not tiled")
         handlers = [(hd.kind, float(hd.normal[0]), float(hd.normal[1])) for hd in
                     (boundaries.boundary_handlers if boundaries is not None else [])]
         device = state.h.device
         self.tiles = {(ty, tx): Tile(backend, state, mask, entries, handlers, (ty, tx), self.par, self.rows,
-                                     self.cols, device) for ty in range(pny) for tx in range(pnx)}
+                                     self.cols, device, values) for ty in range(pny) for tx in range(pnx)}
         self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
 
     def _neighbour(self, pos, side):

@@ -35,15 +35,15 @@ def gpu_case(mesh, arrays, params, dtype):
     return state, mask, solver, sc
 
 
-def gpu_fe(state, mask, solver, sc, cfl):
+def gpu_fe(state, mask, solver, sc, cfl, time=0.0):
     dt = solver.compute_dt_fn(state, cfl, mask, sc)
-    return solver.step_fn(state, 0.0, dt, mask, sc), dt
+    return solver.step_fn(state, time, dt, mask, sc), dt
 
 
-def gpu_ssprk2(state, mask, solver, sc, cfl):
+def gpu_ssprk2(state, mask, solver, sc, cfl, time=0.0):
     from pyexner_amd.integrators.ssprk2 import ssprk2_step
     dt = solver.compute_dt_fn(state, cfl, mask, sc)
-    return ssprk2_step(solver, sc, state, 0.0, dt, mask), dt
+    return ssprk2_step(solver, sc, state, time, dt, mask), dt
 
 
 FIELDS_EXNER = ("h", "hu", "hv", "z_b", "G")

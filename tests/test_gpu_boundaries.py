@@ -1,0 +1,100 @@
+"""Value boundaries (ZeroMomentum, ConstantInflux, ConstantOutflux,
+NormalFlowDepth, Berthon, SteepFall) against the oracle's restatement of the
+reference handlers.
+
+Bar (written here, checked below): in float64 the GPU ring fill equals the
+handlers applied one after another bit for bit (at a nonzero time, for the
+beds that move with time), and runs with these boundaries stay within 1e-12
+relative L-inf of the float64 oracle per field, the boundaries seeing each
+step's start time.  Float32 runs stay within the float32 bar of the float64
+oracle (1e-5 relative on h, hu; hv against max|hu|; 1e-4 of max|dz_b| on the
+bed).  The adjoint refuses an outflux program (PXH_E_UNSUPPORTED).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import steps as osteps
+from pyexner_amd import _hip, cases
+from pyexner_amd.state import fields_struct
+from tests.helpers import FIELDS_EXNER, compare, gpu_case, gpu_fe, gpu_ssprk2, oracle_exner
+
+pytestmark = pytest.mark.gpu
+
+BUILDERS = {"berthon": lambda: cases.berthon_channel(nx=90, ny=14),
+            "outflux": lambda: cases.outflux_channel(nx=70, ny=22),
+            "steepfall": lambda: cases.outflux_channel(nx=70, ny=22, steep_fall=True)}
+
+
+@pytest.mark.parametrize("name", sorted(BUILDERS))
+def test_value_fill_matches_sequential_handlers_f64(cuda, name):
+    mesh, arr, params = BUILDERS[name]()
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    so, prob = oracle_exner(mesh, arr, params)
+    rng = np.random.default_rng(3)
+    host = sg.to_host()
+    noise = {k: rng.uniform(-0.05, 0.05, host.h.shape) for k in FIELDS_EXNER}
+    so = so.replace(**{k: getattr(so, k) + noise[k] for k in FIELDS_EXNER})
+    sg = sg.replace(**{k: torch.as_tensor(getattr(so, k), device="cuda") for k in FIELDS_EXNER})
+    t = 0.37
+    got = sc.boundaries.apply(sg, t).to_host()
+    want = prob.bcs.apply(so, t)
+    for k in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G"):
+        np.testing.assert_array_equal(getattr(got, k), getattr(want, k), err_msg=k)
+
+
+@pytest.mark.parametrize("name,integrator", [("berthon", "Forward Euler"), ("outflux", "SSPRK2"),
+                                             ("steepfall", "SSPRK2")])
+def test_value_boundary_runs_f64(cuda, name, integrator):
+    mesh, arr, params = BUILDERS[name]()
+    so, prob = oracle_exner(mesh, arr, params)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    step = gpu_fe if integrator == "Forward Euler" else gpu_ssprk2
+    ostep = osteps.fe_step if integrator == "Forward Euler" else osteps.ssprk2_step
+    t_o, t_g = 0.0, 0.0
+    for _ in range(60):
+        so, dto = ostep(prob, so, t_o)
+        sg, dtg = step(sg, mask, solver, sc, prob.cfl, time=t_g)
+        t_o = float(np.float64(t_o) + dto)
+        t_g = t_g + float(dtg)
+    errs = compare(sg, so, FIELDS_EXNER)
+    print(name, integrator, "t", t_g, {k: float(f"{v:.3e}") for k, v in errs.items()})
+    assert abs(t_g - t_o) <= 1e-12 * t_o
+    assert max(errs.values()) <= 1e-12, errs
+    assert np.isfinite(sg.to_host().h).all()
+
+
+@pytest.mark.parametrize("name", ["outflux", "berthon"])
+def test_value_boundary_f32_within_tolerance(cuda, name):
+    mesh, arr, params = BUILDERS[name]()
+    so, prob = oracle_exner(mesh, arr, params)
+    z0 = so.z_b.copy()
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+    t_o, t_g = 0.0, 0.0
+    for _ in range(100):
+        so, dto = osteps.ssprk2_step(prob, so, t_o)
+        sg, dtg = gpu_ssprk2(sg, mask, solver, sc, prob.cfl, time=t_g)
+        t_o += float(dto)
+        t_g += float(dtg)
+    g = sg.to_host()
+    hu_scale = np.max(np.abs(so.hu))
+    assert np.max(np.abs(g.h - so.h)) <= 1e-5 * np.max(np.abs(so.h))
+    assert np.max(np.abs(g.hu - so.hu)) <= 1e-5 * hu_scale
+    assert np.max(np.abs(g.hv - so.hv)) <= 1e-5 * hu_scale
+    dz = np.max(np.abs(so.z_b - z0))
+    assert np.max(np.abs(g.z_b - so.z_b)) <= 1e-4 * dz + 1e-7
+    print(name, "max|dz_b|", dz)
+
+
+def test_adjoint_refuses_an_outflux_program(cuda):
+    mesh, arr, params = BUILDERS["outflux"]()
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    be = sc.backend
+    bc = sc.boundaries.adjoint_program(be.STEP_GROUPS)
+    ws = torch.empty(int(_hip.load().pxh_adj_workspace_size(be.grid, bc)), dtype=torch.uint8, device="cuda")
+    lam = sg.clone()
+    one = torch.ones(1, dtype=torch.float64, device="cuda")
+    st = _hip.load().pxh_adj_roe_exner_ssprk2(be.grid, fields_struct(sg), fields_struct(sg), fields_struct(sg),
+                                              mask[0].data_ptr(), be.sed, bc, 0.5, one.data_ptr(),
+                                              fields_struct(lam), None, ws.data_ptr(), None)
+    assert st == _hip.PXH_E_UNSUPPORTED

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     assert lib.pxh_error_string(-1) == b"invalid argument"
 
 
-def _emulate_program(entries, handlers, state, groups=None):
+def _emulate_program(entries, handlers, state, groups=None, values=(), time=0.0):
     """Apply the composed program the way k_bc does (every read before any write)."""
     out = state.copy()
     ny, nx = state.h.shape
@@ -42,6 +42,31 @@ def _emulate_program(entries, handlers, state, groups=None):
         if groups is not None and g_ not in groups:
             continue
         t, s = divmod(int(e["target"]), nx), divmod(int(e["src"]), nx)
+        if int(e["nchain"]) & _hip.BC_VALUE:
+            kind, a, b, c = values[int(e["chain"][0])]
+            if kind == _hip.BV_CONST and g_ == _hip.G_HUV:
+                if a == a:
+                    out.hu[t] = a
+                if b == b:
+                    out.hv[t] = b
+            elif kind == _hip.BV_CONST:
+                getattr(out, names[g_])[t] = a
+            elif kind == _hip.BV_AFFINE_T:
+                getattr(out, names[g_])[t] = b * time + a
+            elif kind == _hip.BV_OFFSET:
+                getattr(out, names[g_])[t] = getattr(state, names[g_])[s] + a
+            elif kind == _hip.BV_OUTFLUX:
+                fl, gg = 1e-8, 9.81
+                h_i = max(state.h[s], fl)
+                un = state.hu[s] / h_i * b + state.hv[s] / h_i * c
+                Rp = un + 2.0 * np.sqrt(gg * h_i)
+                hb = h_i
+                for _ in range(5):
+                    hb = max(hb, fl)
+                    cc = np.sqrt(gg * hb)
+                    hb = max(hb - (a / hb + 2.0 * cc - Rp) / (-a / (hb * hb) + gg / max(cc, fl)), fl)
+                out.h[t] = hb
+            continue
         if g_ == _hip.G_HUV:
             hu, hv = state.hu[s], state.hv[s]
             for q in range(int(e["nchain"])):
@@ -68,13 +93,20 @@ def _random_exner(shape, rng):
 @pytest.mark.parametrize("builder,order", [
     ("channel", None), ("radial", None), ("periodic", None),
     ("radial", [3, 0, 2, 1]), ("channel", [5, 4, 0, 3, 1, 2]),
+    ("berthon", None), ("berthon", [4, 2, 3, 1, 0]), ("outflux", None), ("outflux", [2, 0, 6, 4, 5, 1, 3]),
 ])
 def test_composed_program_equals_sequential_handlers(builder, order):
+    """The composed program (copies, reflections and value entries) against
+    the handlers applied one after another, at t = 0.37 (time-dependent beds)."""
     rng = np.random.default_rng(7)
     if builder == "channel":
         mesh, arr, params = cases.channel(nx=14, ny=9, n_bumps=1)
     elif builder == "radial":
         mesh, arr, params = cases.radial_dambreak(nx=11, ny=13, n_bumps=1, radius=0.2)
+    elif builder == "berthon":
+        mesh, arr, params = cases.berthon_channel(nx=15, ny=8)
+    elif builder == "outflux":
+        mesh, arr, params = cases.outflux_channel(nx=13, ny=9, steep_fall=True)
     else:
         mesh, arr, params = cases.periodic_box(nx=12, ny=10)
     specs = params["boundaries"]
@@ -83,17 +115,20 @@ def test_composed_program_equals_sequential_handlers(builder, order):
         specs = {keys[i]: specs[keys[i]] for i in order}
     bm = BoundaryManager({"flux_scheme": "Roe Exner", "boundaries": specs}, mesh.local_X, mesh.local_Y,
                          device="cpu")
-    entries, idem = compose(bm.boundary_handlers, mesh.local_shape)
-    assert idem
+    entries, idem, values = compose(bm.boundary_handlers, mesh.local_shape)
+    assert idem and (values == []) == (builder in ("channel", "radial", "periodic"))
     oc = ob.Boundaries("Roe Exner", specs, mesh.local_X, mesh.local_Y)
     np.testing.assert_array_equal(bm.boundary_mask, oc.boundary_mask)
     st = _random_exner(mesh.local_shape, rng)
-    got = _emulate_program(entries, bm.boundary_handlers, st)
-    exp = oc.apply(st)
+    st = st.replace(h=np.abs(st.h) + 0.1)  # the outflux depth needs water
+    got = _emulate_program(entries, bm.boundary_handlers, st, values=values, time=0.37)
+    exp = oc.apply(st, 0.37)
     for f in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G"):
         np.testing.assert_array_equal(getattr(got, f), getattr(exp, f), err_msg=f)
+        # -0.0 and 0.0 are told apart too
+        np.testing.assert_array_equal(np.signbit(getattr(got, f)), np.signbit(getattr(exp, f)), err_msg=f)
     # idempotent: applying twice equals once
-    again = _emulate_program(entries, bm.boundary_handlers, got)
+    again = _emulate_program(entries, bm.boundary_handlers, got, values=values, time=0.37)
     for f in ("h", "hu", "hv", "z_b", "G"):
         np.testing.assert_array_equal(getattr(again, f), getattr(got, f))
 
@@ -104,7 +139,7 @@ def test_composed_program_roe_reflective_and_oblique_normal():
     specs = dict(params["boundaries"])
     specs["north"] = dict(specs["north"], normal=[0.6, 0.8])  # oblique normal: rounding pinned too
     bm = BoundaryManager({"flux_scheme": "Roe", "boundaries": specs}, mesh.local_X, mesh.local_Y, device="cpu")
-    entries, idem = compose(bm.boundary_handlers, mesh.local_shape)
+    entries, idem, values = compose(bm.boundary_handlers, mesh.local_shape)
     oc = ob.Boundaries("Roe", specs, mesh.local_X, mesh.local_Y)
     st = RoeState(**{k: rng.uniform(-1, 1, mesh.local_shape) for k in ("h", "hu", "hv", "z", "n")})
     ex = ExnerState(h=st.h, hu=st.hu, hv=st.hv, z=st.z, z_b=st.z * 0, n=st.n, n_b=st.n * 0, G=st.n * 0,
@@ -370,3 +405,33 @@ def test_sediment_gradient_chain_rule_against_table_differences():
         sm[row, 0] -= eps
         fd = (functional(sp) - functional(sm)) / (2 * eps)
         assert abs(fd - want) <= 1e-7 * abs(fd), (row, fd, want)
+
+
+def test_netcdf_writer_round_trip(tmp_path):
+    """NetCDFWriter (pnetcdf_reader.py:178-256 write side): (t, y, x) float32
+    snapshots with NaN under the nodata mask, file coordinates recovered from
+    the cell centres, read back with scipy."""
+    from oracle.state import ExnerState
+    from pyexner_amd.mesh import uniform_mesh
+    from pyexner_amd.output import NetCDFWriter, read_snapshots
+    ny, nx = 6, 9
+    mesh = uniform_mesh(ny, nx, 0.1)
+    rng = np.random.default_rng(5)
+    f = lambda: rng.normal(size=(ny, nx))  # noqa: E731
+    mask = np.zeros((ny, nx), dtype=bool)
+    mask[0, :] = mask[:, -1] = True
+    states = [ExnerState(h=f(), hu=f(), hv=f(), z=f(), z_b=f(), n=f(), n_b=f(), G=f(), seds=np.zeros((7, 1)))
+              for _ in range(2)]
+    path = str(tmp_path / "out.nc")
+    with NetCDFWriter(path, mesh) as w:
+        for s in states:
+            w.write_state(s, mesh, mask)
+    x, y, out = read_snapshots(path)
+    np.testing.assert_allclose(x + 0.05, mesh.local_X[0], atol=1e-12)
+    np.testing.assert_allclose(y - 0.05, mesh.local_Y[:, 0], atol=1e-12)
+    assert set(out) == {"h", "hu", "hv", "z", "z_b", "n", "n_b", "G"}
+    for name, arr in out.items():
+        assert arr.shape == (2, ny, nx) and arr.dtype == np.float32
+        for k, s in enumerate(states):
+            want = np.where(mask, np.nan, getattr(s, name)).astype(np.float32)
+            np.testing.assert_array_equal(arr[k], want)
