@@ -253,3 +253,16 @@ def outflux_channel(nx=120, ny=30, dx=0.02, seed=12, q=0.05, steep_fall=False):
     _with_values(bounds, "inlet", [q, 1.0, 0.0])
     _with_values(bounds, "outlet", [q, 1.0, 0.0])
     return mesh, arrays, _exner_params(mesh, bounds)
+
+
+def nodata_island(nx=96, ny=40, dx=0.02, seed=13):
+    """The calibration channel's flow around a nodata island: z = NaN on an
+    interior block (isnan(z) is nodata, roe_exner_solver.py:21-39), as the
+    reference's erodible channel marks the cells outside its flume
+    (erodible_channel.py:124-129)."""
+    mesh, arrays, params = calibration_channel(nx=nx, ny=ny, dx=dx, seed=seed)
+    z = np.array(arrays["z"], dtype=np.float64)
+    r0, c0 = ny // 3, nx // 3
+    z[r0:r0 + ny // 4, c0:c0 + nx // 6] = np.nan
+    arrays = dict(arrays, z=z)
+    return mesh, arrays, params
