@@ -81,6 +81,8 @@ def parse():
     p.add_argument("--adj-steps", type=int, default=200,
                    help="steps of the gradient run (bed calibration, 1024x256 float64; 0: skip)")
     p.add_argument("--adj-every", type=int, default=20, help="checkpoint interval of the gradient run")
+    p.add_argument("--autotune", type=int, default=1,
+                   help="choose the fused stage's strip height by timing the candidates (1) or keep 32 (0)")
     return p.parse_args()
 
 
@@ -174,6 +176,23 @@ def ensemble_run(a, torch):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
+    rows, tune_ms = 32, {}
+    if a.autotune:
+        # the strip height of the members' fused stage sweeps, timed on the
+        # ensemble itself and then restored to its initial state
+        keep = {f: getattr(ens.a, f).clone() for f in ("h", "hu", "hv", "z_b", "z_b_lo", "G")
+                if getattr(ens.a, f) is not None}
+        for r in (16, 24, 32, 48):
+            ens.set_strip_rows(r)
+            ens.ssprk2_step()
+            tune_ms[r] = timed(ens.ssprk2_step)
+        rows = min(tune_ms, key=tune_ms.get)
+        ens.set_strip_rows(rows)
+        for f, t in keep.items():
+            getattr(ens.a, f).copy_(t)
+        ens.dt_ready = False
+        del keep
+        log("ensemble strip rows: " + ", ".join(f"{r}: {t:.3f} ms" for r, t in tune_ms.items()) + f" -> {rows}")
     ens.capture()
     ms_ens = timed(ens.replay)
 
@@ -196,7 +215,8 @@ def ensemble_run(a, torch):
             "cell_steps_per_s": cells / (ms_ens * 1e-3),
             "loop_ms_per_step": ms_loop, "loop_graph_ms_per_step": ms_loop_graph,
             "speedup_vs_loop": ms_loop / ms_ens, "speedup_vs_loop_graph": ms_loop_graph / ms_ens,
-            "dtype": "f32", "scenario": "channel members, seed 4, dx 0.02, Transmissive in/out, Reflective walls"}
+            "dtype": "f32", "scenario": "channel members, seed 4, dx 0.02, Transmissive in/out, Reflective walls",
+            "strip_rows": rows, "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}}
 
 
 def adjoint_run(a, torch):
@@ -277,6 +297,12 @@ def main():
     del arr
     mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
     state = solver.init_fn(state, mask, sc)
+    rows, tune_ms = 32, {}
+    if a.autotune:
+        from pyexner_amd.autotune import tune_strip_rows
+        rows, tune_ms = tune_strip_rows(sc.backend, state, mask, params["cfl"], steps=3, use_cache=False)
+        torch.cuda.empty_cache()
+        log("strip rows: " + ", ".join(f"{r}: {t:.3f} ms" for r, t in tune_ms.items()) + f" -> {rows}")
     st = ExnerStepper(sc.backend, state, mask, params["cfl"])
     init = {f: getattr(state, f).clone() for f in ("h", "hu", "hv", "z_b", "z_b_lo", "G")}
     del state
@@ -417,7 +443,8 @@ def main():
         "config": {"workload": "roe_exner_ssprk2_radial_dambreak", "nx": n, "ny": n, "dx": 0.05,
                    "integrator": "SSPRK2", "cfl": 0.5, "bumps": 256 if n == 8192 else None,
                    "steps_from_initial_state": a.steps, "hipgraph": use_graph,
-                   "stage": "fused (frame-local ring)", "bed": "float32 z_b + z_b_lo (compensated)", "math": "float32 hardware rcp/rsq/log2/exp2",
+                   "stage": "fused (frame-local ring)", "strip_rows": rows,
+                   "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}, "bed": "float32 z_b + z_b_lo (compensated)", "math": "float32 hardware rcp/rsq/log2/exp2",
                    "step_alg_bytes_per_cell": step_bytes, "step_alg_GBs": round(step_bytes * value / 1e9, 1)},
         "roofline": roofline,
         "kernels": kern,

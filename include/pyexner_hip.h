@@ -45,6 +45,10 @@ extern "C" {
 
 /* Grid geometry. */
 #define PXH_EXACT_F32 1  /* flags bit: float32 with IEEE division/sqrt/libm (default: hardware rcp/rsq/log2/exp2) */
+/* flags bits 8-15: owned rows per strip of the fused stage sweep (0: 32;
+ * otherwise 8..128), the launch shape pyexner_amd.autotune chooses per grid. */
+#define PXH_STRIP_ROWS_SHIFT 8
+#define PXH_STRIP_ROWS(flags) (((flags) >> PXH_STRIP_ROWS_SHIFT) & 0xff)
 
 typedef struct {
     int32_t ny, nx;   /* array shape, ghost ring included */

@@ -1,0 +1,97 @@
+"""Launch-shape autotuning for the fused Roe-Exner step.
+
+The fused stage sweep gives each wavefront a strip of 60 owned columns and
+`rows` owned rows (PXH_STRIP_ROWS of pxh_grid.flags).  Taller strips recompute
+fewer halo rows; shorter strips give more wavefronts to the 256 CUs on small
+grids and shorter register lifetimes.  The best height depends on the grid
+and the GPU, so it is measured: `tune_strip_rows` times a few SSPRK2 steps of
+the actual case for each candidate height on a copy of the state and keeps
+the fastest.  Results are cached per (shape, dtype, math, sediment mode) for
+the process, and in a JSON file when PYEXNER_TUNE_CACHE names one.
+
+The strip height changes which wavefront computes a cell, never what it
+computes: every candidate gives the same result bit for bit.
+"""
+import json
+import os
+
+import torch
+
+from . import _hip
+
+CANDIDATES = (16, 24, 32, 48, 64)
+_CACHE = {}
+
+
+def _key(be):
+    return f"{be.shape[0]}x{be.shape[1]}:{'f64' if be.dtype == torch.float64 else 'f32'}:" \
+           f"{'exact' if be.exact_f32 else 'fast'}:{'mpm' if be.sed.mpm else 'const'}:" \
+           f"{'lo' if be.compensated_bed else 'plain'}"
+
+
+def _file():
+    return os.environ.get("PYEXNER_TUNE_CACHE")
+
+
+def _load_file():
+    path = _file()
+    if not path or not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def _store_file(key, rows):
+    path = _file()
+    if not path:
+        return
+    data = _load_file()
+    data[key] = rows
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def time_steps(be, state, mask, cfl, steps):
+    """Milliseconds per fused SSPRK2 step of `state` under the backend's
+    current launch shape (one warm-up step; the state is not modified)."""
+    from .stepper import ExnerStepper
+    st = ExnerStepper(be, state, mask, cfl)
+    st.compute_dt()
+    st.ssprk2_step()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        st.ssprk2_step()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def tune_strip_rows(be, state, mask, cfl=0.5, candidates=CANDIDATES, steps=4, use_cache=True):
+    """Choose the fused stage's strip height for backend `be` (an
+    ExnerBackend) on `state` (initialised) and apply it.  Returns (rows,
+    {rows: ms per step}); the timings are empty when the choice came from a
+    cache."""
+    key = _key(be)
+    if use_cache:
+        rows = _CACHE.get(key) or _load_file().get(key)
+        if rows:
+            be.set_strip_rows(rows)
+            _CACHE[key] = rows
+            return rows, {}
+    times = {}
+    for rows in candidates:
+        be.set_strip_rows(rows)
+        times[rows] = time_steps(be, state, mask, cfl, steps)
+    best = min(times, key=times.get)
+    be.set_strip_rows(best)
+    _CACHE[key] = best
+    _store_file(key, best)
+    return best, times
+
+
+def strip_rows_of(flags):
+    """The strip height a grid's flags select (PXH_STRIP_ROWS; 0 = 32)."""
+    r = (int(flags) >> _hip.STRIP_ROWS_SHIFT) & 0xFF
+    return r or 32

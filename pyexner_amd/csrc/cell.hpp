@@ -213,8 +213,10 @@ __device__ __forceinline__ void bc_value_apply(const F<T> &s, const pxh_bc_entry
 // host-side argument checks
 // ---------------------------------------------------------------------------
 static inline bool grid_ok(const pxh_grid *g) {
-    return g && g->ny >= 2 && g->nx >= 2 && (g->dtype == PXH_F32 || g->dtype == PXH_F64) &&
-           (long(g->ny) * g->nx) < (1L << 31);
+    if (!g) return false;
+    const int rows = PXH_STRIP_ROWS(g->flags);
+    return g->ny >= 2 && g->nx >= 2 && (g->dtype == PXH_F32 || g->dtype == PXH_F64) &&
+           (long(g->ny) * g->nx) < (1L << 31) && (rows == 0 || (rows >= 8 && rows <= 128));
 }
 
 // A boundary program (or none) the ring-fill launch can run: with

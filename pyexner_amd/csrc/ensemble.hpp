@@ -49,7 +49,7 @@ __device__ __forceinline__ int find_member(const EnsMember *__restrict__ tab, in
 template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
 __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                         const uint8_t *__restrict__ m0, const T *__restrict__ dts,
-                                                        const EnsMember *__restrict__ tab, int n,
+                                                        const EnsMember *__restrict__ tab, int n, int sr,
                                                         const SedK *__restrict__ seds, int write_G,
                                                         T *__restrict__ partial) {
     __shared__ T red[FWPB];
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> 
     const EnsMember m = tab[i];
     const int local = lb - m.sblk0;
     const int by = local / m.sgx, bx = local - by * m.sgx;
-    const StagePos p = stage_pos_at(bx, by, m.ny, m.nx);
+    const StagePos p = stage_pos_at(bx, by, m.ny, m.nx, sr);
     T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, shifted(in, m.off), shifted(out, m.off),
                                                          shifted(cb, m.off), shifted(old, m.off), m0 + m.off,
                                                          dts + i, m.ny, m.nx, static_cast<T>(m.dx), seds[i],

@@ -388,14 +388,15 @@ static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, con
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     const F<T> none{};
-    const dim3 fg = stage_grid(ny, nx);
+    const int sr = strip_rows(g->flags);
+    const dim3 fg = stage_grid(ny, nx, sr);
     if (!old)
-        launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, dx, sed_k(*sd), write_G,
+        launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, sr, dx, sed_k(*sd), write_G,
                                         static_cast<T *>(nullptr));
     else if (part)
-        launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, sed_k(*sd), 1, part);
+        launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1, part);
     else
-        launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, dx, sed_k(*sd), 1,
+        launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1,
                                         static_cast<T *>(nullptr));
     return launch_status();
 }
@@ -418,7 +419,7 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
                        dx, sed_k(*sd), 1, nband);
     if (int e = launch_status()) return e;
     if (!part) return PXH_OK;
-    const dim3 fg = stage_grid(ny, nx);
+    const dim3 fg = stage_grid(ny, nx, strip_rows(g->flags));
     const int nf = int(fg.x * fg.y);
     hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbb), dim3(256), 0, st, *old, m0, ny, nx, dx, nband, part + nf);
     if (int e = launch_status()) return e;
@@ -439,10 +440,10 @@ static int rx_stage0_fused(const pxh_grid *g, const F<T> &in, const F<T> &out, c
 }
 
 // The fused stage addresses a strip's rows with 32-bit byte offsets from the
-// strip's first row: SR_STAGE + 3 rows of nx elements must fit.
+// strip's first row: sr + 3 rows of nx elements must fit.
 static bool stage_fits(const pxh_grid *g) {
     const long esz = g->dtype == PXH_F64 ? 8 : 4;
-    return long(SR_STAGE + 4) * g->nx * esz < (1L << 32);
+    return long(strip_rows(g->flags) + 4) * g->nx * esz < (1L << 32);
 }
 
 static bool frame_fused(const pxh_grid *g, const pxh_bc *bc) {
@@ -805,6 +806,8 @@ static int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
 static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *mem, int32_t n, void *out,
                               int64_t out_bytes) {
     if (n <= 0 || !mem || (dtype != PXH_F32 && dtype != PXH_F64)) return PXH_E_ARG;
+    const int rows = PXH_STRIP_ROWS(flags);
+    if (rows != 0 && (rows < 8 || rows > 128)) return PXH_E_ARG;
     const int64_t tab_off = align16(sizeof(EnsPlanHeader));
     const int64_t sed_off = align16(tab_off + int64_t(n + 1) * sizeof(EnsMember));
     const int64_t bytes = align16(sed_off + int64_t(n) * sizeof(SedK));
@@ -823,7 +826,7 @@ static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *me
     for (int i = 0; i < n; ++i) {
         const pxh_member &q = mem[i];
         if (q.ny < 2 || q.nx < 2 || q.off < 0 || !(q.dx > 0)) return PXH_E_ARG;
-        if (long(SR_STAGE + 4) * q.nx * esz >= (1L << 32)) return PXH_E_RANGE;
+        if (long(strip_rows(flags) + 4) * q.nx * esz >= (1L << 32)) return PXH_E_RANGE;
         const pxh_sediments &sd = q.sed;
         if (sd.n_sed < 0 || sd.n_sed > PXH_MAX_SED || (sd.mpm && sd.n_sed < 1) || sd.mpm != h.mpm) return PXH_E_RANGE;
         EnsMember m{};
@@ -831,7 +834,7 @@ static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *me
         m.ny = q.ny;
         m.nx = q.nx;
         m.dx = q.dx;
-        const dim3 fg = stage_grid(q.ny, q.nx), dg = strip_grid<SO_WIDE, SR_BED>(q.ny, q.nx);
+        const dim3 fg = stage_grid(q.ny, q.nx, strip_rows(flags)), dg = strip_grid<SO_WIDE, SR_BED>(q.ny, q.nx);
         m.sgx = int(fg.x);
         m.nstage = int(fg.x * fg.y);
         m.nband = frame_cells(q.ny, q.nx);
@@ -882,7 +885,7 @@ static void ens_stage(const EnsPlanHeader &h, const EnsMember *tab, const SedK *
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
 #define PXH_ENS(LO_, MPM_)                                                                                       \
     hipLaunchKernelGGL((k_ens_stage<T, FAST, MODE, DT, LO_, MPM_>), dim3(h.sblocks), dim3(SW * FWPB), 0, st, in,  \
-                       out, cb, old, m0, dts, tab, h.n, seds, write_G, partial)
+                       out, cb, old, m0, dts, tab, h.n, strip_rows(h.flags), seds, write_G, partial)
     if constexpr (sizeof(T) == 4) {
         if (lo) {
             if (h.mpm) PXH_ENS(true, true);
@@ -1119,7 +1122,7 @@ int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const px
 
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
-    const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx);
+    const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx, strip_rows(g->flags));
     int64_t nb = int64_t(sa.x) * sa.y + seam_blocks(g->ny, g->nx);
     const int64_t nfused = int64_t(fg.x) * fg.y + (frame_cells(g->ny, g->nx) + 255) / 256;
     if (nfused > nb) nb = nfused;

@@ -29,7 +29,7 @@ namespace pxh {
 #endif
 constexpr int FO = 60;          // owned columns per fused strip
 constexpr int FHW = 2;          // halo lanes on each side
-constexpr int SR_STAGE = PXH_SR_STAGE;  // owned rows per fused strip
+constexpr int SR_STAGE = PXH_SR_STAGE;  // owned rows per fused strip unless the grid's flags choose
 constexpr int FWPB = PXH_FWPB;  // fused strips (wavefronts) per block
 
 struct StagePos {
@@ -38,8 +38,14 @@ struct StagePos {
     bool hval;  // updated depth complete in this lane
 };
 
-// The strip of block (bx, by) of a domain's stage grid.
-__device__ __forceinline__ StagePos stage_pos_at(int bx, int by, int ny, int nx) {
+// Owned rows per fused strip of a grid (PXH_STRIP_ROWS of its flags).
+__host__ __device__ inline int strip_rows(int flags) {
+    const int r = PXH_STRIP_ROWS(flags);
+    return r ? r : SR_STAGE;
+}
+
+// The strip of block (bx, by) of a domain's stage grid of sr-row strips.
+__device__ __forceinline__ StagePos stage_pos_at(int bx, int by, int ny, int nx, int sr) {
     StagePos p;
     p.lane = threadIdx.x & (SW - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SW);
@@ -47,20 +53,20 @@ __device__ __forceinline__ StagePos stage_pos_at(int bx, int by, int ny, int nx)
     p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
     p.out = p.lane >= FHW && p.lane < FHW + FO && p.c < nx;
     p.hval = p.lane >= 1 && p.lane <= SW - 2 && p.c >= 0 && p.c < nx;
-    p.r0 = (by * FWPB + w) * SR_STAGE;
-    p.r1 = p.r0 + SR_STAGE < ny ? p.r0 + SR_STAGE : ny;
+    p.r0 = (by * FWPB + w) * sr;
+    p.r1 = p.r0 + sr < ny ? p.r0 + sr : ny;
     return p;
 }
 
-__device__ __forceinline__ StagePos stage_pos(int ny, int nx) {
+__device__ __forceinline__ StagePos stage_pos(int ny, int nx, int sr) {
     int bx, by;
     xcd_block(bx, by);
-    return stage_pos_at(bx, by, ny, nx);
+    return stage_pos_at(bx, by, ny, nx, sr);
 }
 
-static inline dim3 stage_grid(int ny, int nx) {
+static inline dim3 stage_grid(int ny, int nx, int sr = SR_STAGE) {
     const int sx = (nx + FO - 1) / FO;
-    const int sy = (ny + SR_STAGE - 1) / SR_STAGE;
+    const int sy = (ny + sr - 1) / sr;
     return dim3(sx, (sy + FWPB - 1) / FWPB);
 }
 
@@ -283,11 +289,11 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
 template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
 __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                              const uint8_t *__restrict__ m0,
-                                                             const T *__restrict__ dtp, int ny, int nx, T dx,
-                                                             SedK sd, int write_G,
+                                                             const T *__restrict__ dtp, int ny, int nx, int sr,
+                                                             T dx, SedK sd, int write_G,
                                                              T *__restrict__ partial) {
     __shared__ T red[FWPB];
-    const StagePos p = stage_pos(ny, nx);
+    const StagePos p = stage_pos(ny, nx, sr);
     T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
@@ -299,12 +305,12 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
 // sediment mode.  In a compensated run every set (in, out, old) has z_b_lo.
 template <typename T, bool FAST, int MODE, bool DT>
 static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
-                         const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, T dx,
+                         const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, int sr, T dx,
                          const SedK &sd, int write_G, T *partial) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
 #define PXH_STAGE(LO_, MPM_)                                                                                   \
     hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_>), grid, dim3(SW * FWPB), 0, st, in, out, cb, old, \
-                       m0, dtp, ny, nx, dx, sd, write_G, partial)
+                       m0, dtp, ny, nx, sr, dx, sd, write_G, partial)
     if constexpr (sizeof(T) == 4) {
         if (lo) {
             if (sd.mpm) PXH_STAGE(true, true);

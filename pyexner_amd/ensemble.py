@@ -94,24 +94,39 @@ class Ensemble:
             bc.handlers[k].kind, bc.handlers[k].nx, bc.handlers[k].ny = kind, nxn, nyn
         self.bc = bc
         # the plan: member table and sediment folds, host and device copies
-        mem = (_hip.Member * self.n)()
+        self._mem = (_hip.Member * self.n)()
         for k, (shape, sed, dx, _) in enumerate(meta):
-            mem[k].off, mem[k].ny, mem[k].nx, mem[k].dx, mem[k].sed = int(self.offsets[k]), shape[0], shape[1], dx, sed
+            self._mem[k].off, self._mem[k].ny, self._mem[k].nx = int(self.offsets[k]), shape[0], shape[1]
+            self._mem[k].dx, self._mem[k].sed = dx, sed
+        self._code = _hip.PXH_F32 if dtype == torch.float32 else _hip.PXH_F64
+        self._flags = meta[0][3]
+        self.device = device
+        self.workspace = None
+        self._build_plan()
+        self.dt = torch.zeros(self.n, dtype=dtype, device=device)
+        self.dt_ready = False
+        self.graph = None
+
+    def _build_plan(self):
         lib = _hip.load()
-        code = _hip.PXH_F32 if dtype == torch.float32 else _hip.PXH_F64
-        flags = meta[0][3]
-        size = lib.pxh_ensemble_plan(code, flags, mem, self.n, None, 0)
+        size = lib.pxh_ensemble_plan(self._code, self._flags, self._mem, self.n, None, 0)
         if size < 0:
             raise ValueError(f"pxh_ensemble_plan: {lib.pxh_error_string(int(size)).decode()}")
         self._plan_host = (ctypes.c_uint8 * size)()
-        got = lib.pxh_ensemble_plan(code, flags, mem, self.n, self._plan_host, size)
+        got = lib.pxh_ensemble_plan(self._code, self._flags, self._mem, self.n, self._plan_host, size)
         if got != size:
             raise ValueError(f"pxh_ensemble_plan: {lib.pxh_error_string(int(got)).decode()}")
-        self._plan_dev = torch.frombuffer(bytearray(self._plan_host), dtype=torch.uint8).to(device)
+        self._plan_dev = torch.frombuffer(bytearray(self._plan_host), dtype=torch.uint8).to(self.device)
         ws = lib.pxh_ensemble_workspace_size(self._plan_host)
-        self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
-        self.dt = torch.zeros(self.n, dtype=dtype, device=device)
-        self.dt_ready = False
+        if self.workspace is None or ws > self.workspace.numel():
+            self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=self.device)
+
+    def set_strip_rows(self, rows):
+        """Owned rows per strip of the fused stage sweep for every member
+        (pxh_grid.flags PXH_STRIP_ROWS); rebuilds the plan.  A captured graph
+        must be captured again."""
+        self._flags = (int(self._flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (int(rows) << _hip.STRIP_ROWS_SHIFT)
+        self._build_plan()
         self.graph = None
 
     def _stream(self):

@@ -87,6 +87,17 @@ class ExnerBackend:
         self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
         self.device = device
 
+    def set_strip_rows(self, rows):
+        """Owned rows per strip of the fused stage sweep (pyexner_amd.autotune);
+        the dt workspace follows the strip count."""
+        flags = (int(self.grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (int(rows) << _hip.STRIP_ROWS_SHIFT)
+        self.grid.flags = flags
+        ws = _hip.load().pxh_dt_workspace_size(self.grid)
+        if ws < 0:
+            raise ValueError(f"strip height {rows} is outside 8..128")
+        if ws > self.workspace.numel():
+            self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=self.device)
+
     # raw entry points on preallocated buffers (no allocation) -----------------
     def init_into(self, fs, mask0_ptr, stream):
         _hip.call("pxh_roe_exner_init", self.grid, fs, mask0_ptr, self.sed, stream)

@@ -435,3 +435,17 @@ def test_netcdf_writer_round_trip(tmp_path):
         for k, s in enumerate(states):
             want = np.where(mask, np.nan, getattr(s, name)).astype(np.float32)
             np.testing.assert_array_equal(arr[k], want)
+
+
+def test_strip_height_flags_are_checked():
+    """PXH_STRIP_ROWS (pxh_grid.flags bits 8-15): 0 or 8..128; the dt
+    workspace grows with the strip count (host-only size queries)."""
+    lib = _hip.load()
+    sh = _hip.STRIP_ROWS_SHIFT
+    sizes = {}
+    for rows in (0, 8, 32, 128):
+        sizes[rows] = lib.pxh_dt_workspace_size(_hip.Grid(4000, 3000, _hip.PXH_F32, rows << sh, 0.1))
+        assert sizes[rows] > 0
+    assert sizes[0] == sizes[32] and sizes[8] > sizes[32]
+    for rows in (4, 129, 255):
+        assert lib.pxh_dt_workspace_size(_hip.Grid(400, 300, _hip.PXH_F32, rows << sh, 0.1)) == -1
