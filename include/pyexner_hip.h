@@ -101,7 +101,8 @@ typedef struct {
     int32_t target, src;            /* linear cell indices (row*nx + col) */
     uint8_t group;                  /* PXH_G_* */
     uint8_t nchain;                 /* handlers applied to src, in order (momentum only); with
-                                       PXH_BC_VALUE set, a value entry: chain[0] indexes pxh_bc.values */
+                                       PXH_BC_VALUE set, a value entry: chain[0] indexes pxh_bc.values
+                                       (an index >= n_values makes the entry a no-op) */
     uint8_t chain[PXH_MAX_CHAIN];
 } pxh_bc_entry;                     /* 16 bytes */
 

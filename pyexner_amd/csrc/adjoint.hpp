@@ -112,7 +112,8 @@ __global__ __launch_bounds__(256) void k_adj_bc_take(F<double> lam, const pxh_bc
         // a set value carries no derivative from the state (the target's
         // adjoint stops there); an offset passes it on like a copy; a NaN
         // momentum component leaves the cell as it was (identity)
-        const pxh_bc_value v = bc.values[e.chain[0]];
+        pxh_bc_value v{};
+        if (e.chain[0] < bc.n_values) v = bc.values[e.chain[0]];
         double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
         const int t = e.target;
         const bool keep0 = v.kind == PXH_BV_CONST && e.group == PXH_G_HUV && !(v.a == v.a);

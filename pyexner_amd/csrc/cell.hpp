@@ -178,6 +178,7 @@ __device__ __forceinline__ T outflux_depth(T h_src, T hu_src, T hv_src, T q, T n
 // of a set bed is zero, of an offset bed the source's.
 template <typename T>
 __device__ __forceinline__ void bc_value_apply(const F<T> &s, const pxh_bc_entry &e, const pxh_bc &bc) {
+    if (e.chain[0] >= bc.n_values) return;  // no such value: the entry is ignored
     const pxh_bc_value v = bc.values[e.chain[0]];
     const int t = e.target, src = e.src;
     T *f = group_field(s, e.group);
