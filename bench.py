@@ -115,10 +115,11 @@ def cpu_baseline(n, steps):
 
 
 def tiled_run(a, n, params, torch):
-    """The same domain as a 4 x 4 decomposition of (n/4)^2 owned tiles, each
-    with two ghost rings per shared side, all on this GPU: halo strips packed
-    and unpacked on the device, exchanges as device copies, one hipGraph per
-    step.  Returns its throughput (not the headline metric)."""
+    """The same domain as a 4 x 4 decomposition of (n/4)^2 owned tiles, all on
+    this GPU: the fused stage per tile with three ghost rings per shared side,
+    halo strips packed and unpacked on the device once per stage, exchanges as
+    device copies, one hipGraph per step.  Returns its throughput (not the
+    headline metric)."""
     from pyexner_amd.runtime.driver import setup_case
     from pyexner_amd.tiles import TiledExner
     log(f"tiled run: 4x4 tiles of {n // 4}^2")
@@ -145,8 +146,10 @@ def tiled_run(a, n, params, torch):
     log(f"tiled: {ms:.3f} ms/step, {n * n / (ms * 1e-3) / 1e9:.3f} G cell-steps/s, halo {hb / 1e6:.1f} MB/step")
     t = tl.tiles[(1, 1)]
     return {"ms_per_step": ms, "cell_steps_per_s": n * n / (ms * 1e-3), "steps": a.tile_steps,
-            "tile_owned": [n // 4, n // 4], "tile_array": [t.grid.ny, t.grid.nx], "ghost_rings": 2,
-            "halo_bytes_per_step": hb, "path": "two-sweep stage, exchange after each sweep, dt reduced over tiles"}
+            "tile_owned": [n // 4, n // 4], "tile_array": [t.grid.ny, t.grid.nx], "ghost_rings": tl.hw,
+            "halo_bytes_per_step": hb,
+            "path": ("fused stage per tile, one exchange of every evolving field per stage, dt over tiles after it"
+                     if tl.fused else "two-sweep stage, exchange after each sweep, dt reduced over tiles")}
 
 
 def ensemble_run(a, torch):

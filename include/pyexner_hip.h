@@ -361,7 +361,7 @@ int pxh_bc_apply(const pxh_grid *grid, const pxh_fields *s, const pxh_bc *bc, vo
 
 /* Halo strips of a subdomain tile: make_halo_exchange (solvers/kernels/roe_exner.py:719-803),
  * with the MPI sendrecv left to the caller.  The tile is the grid: its owned
- * block plus hw ghost rings (hw = 1 or 2).  pack copies the send strips (the hw
+ * block plus hw ghost rings (hw = 1, 2 or 3).  pack copies the send strips (the hw
  * owned rows/columns next to each selected side) of the fields of `groups`
  * (bits 1 << PXH_G_*; PXH_G_ZB carries z_b_lo too when present) into buf;
  * unpack copies buf into the receive strips (the hw ghost rows/columns).  Depth

@@ -1,6 +1,6 @@
 // Halo strips of a subdomain tile (make_halo_exchange, roe_exner.py:719-803).
 //
-// A tile holds its owned block plus hw ghost rings (hw = 1 or 2).  The send
+// A tile holds its owned block plus hw ghost rings (hw = 1, 2 or 3).  The send
 // strip of a side is the hw owned rows/columns next to that side; the receive
 // strip is the hw ghost rows/columns on that side.  Depth d = 0 is the
 // row/column nearest the shared boundary on both tiles, so a neighbour's send

@@ -737,7 +737,7 @@ static int part_args_ok(const pxh_grid *g, const pxh_fields *in, const pxh_field
 template <typename T>
 static bool halo_plan(const pxh_grid *g, const pxh_fields *s_, int hw, uint32_t groups, int sides, HaloPlan &plan,
                       HaloFields<T> &hf) {
-    if (hw < 1 || hw > 2 || g->ny < 2 * hw + 1 || g->nx < 2 * hw + 1) return false;
+    if (hw < 1 || hw > 3 || g->ny < 2 * hw + 1 || g->nx < 2 * hw + 1) return false;
     // one phase per call: west/east, or north/south (their strips overlap in the corners)
     const bool we = sides & (PXH_HALO_W | PXH_HALO_E), ns = sides & (PXH_HALO_N | PXH_HALO_S);
     if (sides <= 0 || sides > 15 || (we && ns) || groups == 0 || (groups & ~0x7Fu)) return false;

@@ -10,18 +10,30 @@ device with `pxh_halo_pack` / `pxh_halo_unpack`, and the sendrecv between
 tiles is a device copy (on a multi-GPU machine it would be the RCCL
 send/recv of the same buffers).
 
-Two ghost rings make the tiled run equal the single-domain run cell for cell:
-the momentum corrections of an owned cell next to the tile edge read the
-updated depth of a ghost cell, whose own update needs one more ring.  (With
-the reference's single ring that depth is computed from an incomplete
-stencil.)  The step therefore runs the two-sweep stage with an exchange after
-each sweep, and the dt from every tile's own minimum, reduced over tiles:
+An owned cell's stage result reads the stage input up to three cells away:
+its bed update reads its neighbours' updated hydro, whose momentum
+corrections read the updated depth one cell further, whose hydro update
+reads one more.  (With the reference's single ring that depth is computed
+from an incomplete stencil.)  Two paths make the tiled run equal the
+single-domain run cell for cell:
 
-    dt   = min over tiles of compute_dt(a)            (the MIN allreduce)
-    b    = hydro(a); fill(b: h, hu, hv); exchange(b: h, hu, hv)
-    b    = bed(b, a); fill(b: z_b, G); exchange(b: z_b, G)
-    c    = hydro(b); fill(c: h, hu, hv); exchange(c: h, hu, hv)
-    a    = 0.5 a + 0.5 bed(c, b); fill(a: z_b, G); exchange(a: z_b, G)
+* fused (frame-local rings, the default): three ghost rings, so every owned
+  cell's stage is complete from the tile's own data; each tile runs the
+  fused stage (stage sweep -> fill (h, hu, hv) -> frame sweep -> fill
+  (z_b, G)) and the tiles swap every evolving field once per stage, then the
+  dt from every tile's own minimum after the swap, reduced over tiles:
+
+      b  = S(a) per tile;                      exchange(b)
+      a  = 0.5 a + 0.5 S(b) per tile;          exchange(a)
+      dt = min over tiles of compute_dt(a)     (the MIN allreduce)
+
+* two-sweep: two ghost rings, and an exchange after each sweep:
+
+      dt = min over tiles of compute_dt(a)
+      b  = hydro(a); fill(b: h, hu, hv); exchange(b: h, hu, hv)
+      b  = bed(b, a); fill(b: z_b, G); exchange(b: z_b, G)
+      c  = hydro(b); fill(c: h, hu, hv); exchange(c: h, hu, hv)
+      a  = 0.5 a + 0.5 bed(c, b); fill(a: z_b, G); exchange(a: z_b, G)
 """
 import numpy as np
 import torch
@@ -30,7 +42,8 @@ from . import _hip
 from .boundaries import ENTRY_DTYPE, set_values
 from .state import fields_struct
 
-HW = 2  # ghost rings on a shared side
+HW = 2  # ghost rings on a shared side, two-sweep path
+HW_FUSED = 3  # ghost rings on a shared side, fused path
 _SIDES = (_hip.HALO_W, _hip.HALO_N, _hip.HALO_E, _hip.HALO_S)
 _OPPOSITE = {_hip.HALO_W: _hip.HALO_E, _hip.HALO_E: _hip.HALO_W, _hip.HALO_N: _hip.HALO_S,
              _hip.HALO_S: _hip.HALO_N}
@@ -76,20 +89,21 @@ class Tile:
     """One tile: its extent in the global arrays, three buffer sets, grid,
     mask, ring-fill program and dt workspace."""
 
-    def __init__(self, backend, state, mask, entries, handlers, pos, par, rows, cols, device, values=()):
+    def __init__(self, backend, state, mask, entries, handlers, pos, par, rows, cols, device, values=(), hw=HW):
         ty, tx = pos
         pny, pnx = par
         r0, r1 = rows[ty]
         c0, c1 = cols[tx]
         self.pos = pos
-        self.ring = {_hip.HALO_N: HW if ty > 0 else 0, _hip.HALO_S: HW if ty < pny - 1 else 0,
-                     _hip.HALO_W: HW if tx > 0 else 0, _hip.HALO_E: HW if tx < pnx - 1 else 0}
+        self.hw = hw
+        self.ring = {_hip.HALO_N: hw if ty > 0 else 0, _hip.HALO_S: hw if ty < pny - 1 else 0,
+                     _hip.HALO_W: hw if tx > 0 else 0, _hip.HALO_E: hw if tx < pnx - 1 else 0}
         self.owned = (r0, r1, c0, c1)
         self.ext = (r0 - self.ring[_hip.HALO_N], r1 + self.ring[_hip.HALO_S],
                     c0 - self.ring[_hip.HALO_W], c1 + self.ring[_hip.HALO_E])
         e0, e1, f0, f1 = self.ext
-        if min(r1 - r0, c1 - c0) < 2 * HW + 1:
-            raise ValueError(f"tile {pos} owns {r1 - r0}x{c1 - c0} cells: too small for {HW} ghost rings")
+        if min(r1 - r0, c1 - c0) < 2 * hw + 1:
+            raise ValueError(f"tile {pos} owns {r1 - r0}x{c1 - c0} cells: too small for {hw} ghost rings")
         names = [n for n in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G", "z_b_lo") if getattr(state, n) is not None]
         self.a = state.replace(**{n: getattr(state, n)[e0:e1, f0:f1].contiguous() for n in names})
         evolving = ("h", "hu", "hv", "z_b", "G") + (("z_b_lo",) if state.z_b_lo is not None else ())
@@ -125,7 +139,7 @@ class Tile:
         self.send = {}
         for side in _SIDES:
             if self.ring[side]:
-                nbytes = lib.pxh_halo_bytes(self.grid, self.fa, HW, _STEP, side)
+                nbytes = lib.pxh_halo_bytes(self.grid, self.fa, hw, _STEP, side)
                 if nbytes < 0:
                     raise ValueError("invalid halo geometry")
                 self.send[side] = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
@@ -138,7 +152,9 @@ class TiledExner:
     """Roe-Exner SSPRK2 on a par = (parNy, parNx) decomposition of an
     initialised single-domain state (init_fn applied), every tile on this device."""
 
-    def __init__(self, backend, state, mask, boundaries, par, cfl):
+    def __init__(self, backend, state, mask, boundaries, par, cfl, fused=None):
+        """fused: the fused-stage path (three ghost rings); default when the
+        ring fill is frame-local."""
         self.be = backend
         self.cfl = float(cfl)
         self.par = tuple(par)
@@ -152,9 +168,15 @@ class TiledExner:
         handlers = [(hd.kind, float(hd.normal[0]), float(hd.normal[1])) for hd in
                     (boundaries.boundary_handlers if boundaries is not None else [])]
         device = state.h.device
+        frame_ok = boundaries is None or boundaries.n_entries == 0 or boundaries.frame_local
+        self.fused = frame_ok if fused is None else bool(fused)
+        if self.fused and not frame_ok:
+            raise NotImplementedError("the fused tile path needs a frame-local ring fill")
+        self.hw = HW_FUSED if self.fused else HW
         self.tiles = {(ty, tx): Tile(backend, state, mask, entries, handlers, (ty, tx), self.par, self.rows,
-                                     self.cols, device, values) for ty in range(pny) for tx in range(pnx)}
+                                     self.cols, device, values, self.hw) for ty in range(pny) for tx in range(pnx)}
         self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
+        self.dt_ready = False
 
     def _neighbour(self, pos, side):
         ty, tx = pos
@@ -169,13 +191,14 @@ class TiledExner:
             for t in self.tiles.values():
                 for side in phase:
                     if t.ring[side]:
-                        _hip.call("pxh_halo_pack", t.grid, t.fields(which), HW, groups, side,
+                        _hip.call("pxh_halo_pack", t.grid, t.fields(which), t.hw, groups, side,
                                   t.send[side].data_ptr(), st)
             for t in self.tiles.values():
                 for side in phase:
                     if t.ring[side]:
                         src = self._neighbour(t.pos, side).send[_OPPOSITE[side]]
-                        _hip.call("pxh_halo_unpack", t.grid, t.fields(which), HW, groups, side, src.data_ptr(), st)
+                        _hip.call("pxh_halo_unpack", t.grid, t.fields(which), t.hw, groups, side, src.data_ptr(),
+                                  st)
 
     def compute_dt(self):
         st = torch.cuda.current_stream().cuda_stream
@@ -200,11 +223,40 @@ class TiledExner:
                 _hip.call("pxh_bc_apply", t.grid, t.fields(out), t.programs[fill_bed], st)
         self.exchange(out, _BED)
 
+    def _fused_stage(self, src, dst, old):
+        """One fused stage of every tile, src -> dst (old None) or old =
+        0.5 old + 0.5 S(src) with dst holding S(src)'s hydro near the frame,
+        then one exchange of every evolving field of the result."""
+        st = torch.cuda.current_stream().cuda_stream
+        dtp = self.dt.data_ptr()
+        sed = self.be.sed
+        skip = _hip.SWEEP_SKIP_G if old is None else 0
+        for t in self.tiles.values():
+            fs, fd = t.fields(src), t.fields(dst)
+            fo = t.fields(old) if old is not None else None
+            _hip.call("pxh_roe_exner_stage_sweep", t.grid, fs, fd, fo, t.m0, sed, dtp, skip, None, st)
+            if t.n_entries:
+                _hip.call("pxh_bc_apply", t.grid, fd, t.programs["hy"], st)
+            _hip.call("pxh_roe_exner_frame_sweep", t.grid, fd, fs, None if old is not None else fd, fo, t.m0, sed,
+                      dtp, skip, self.cfl, None, st)
+            if t.n_entries:
+                _hip.call("pxh_bc_apply", t.grid, fo if old is not None else fd,
+                          t.programs["bed" if old is not None else "bed1"], st)
+        self.exchange(old if old is not None else dst, _STEP)
+
     def ssprk2_step(self):
         """One SSPRK2 step (integrators/SSPRK2.py:32-40) of every tile."""
-        self.compute_dt()
-        self._sweeps("a", "b", "a", None, "bed1")
-        self._sweeps("b", "c", "b", "a", "bed")
+        if not self.fused:
+            self.compute_dt()
+            self._sweeps("a", "b", "a", None, "bed1")
+            self._sweeps("b", "c", "b", "a", "bed")
+            return
+        if not self.dt_ready:
+            self.compute_dt()
+        self._fused_stage("a", "b", None)
+        self._fused_stage("b", "c", "a")
+        self.compute_dt()  # the next step's dt, from the exchanged state
+        self.dt_ready = True
 
     def halo_bytes_per_step(self):
         """Bytes the tiles send each other in one step (four exchanges)."""
@@ -213,8 +265,11 @@ class TiledExner:
         for t in self.tiles.values():
             for side in _SIDES:
                 if t.ring[side]:
-                    total += 2 * lib.pxh_halo_bytes(t.grid, t.fa, HW, _HYDRO, side)
-                    total += 2 * lib.pxh_halo_bytes(t.grid, t.fa, HW, _BED, side)
+                    if self.fused:
+                        total += 2 * lib.pxh_halo_bytes(t.grid, t.fa, t.hw, _STEP, side)
+                    else:
+                        total += 2 * lib.pxh_halo_bytes(t.grid, t.fa, t.hw, _HYDRO, side)
+                        total += 2 * lib.pxh_halo_bytes(t.grid, t.fa, t.hw, _BED, side)
         return int(total)
 
     def capture(self):

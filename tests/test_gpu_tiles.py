@@ -2,8 +2,9 @@
 against the single-domain run.
 
 Bars (written here, checked below): the strips are bit-exact copies; a tiled
-SSPRK2 run equals the single-domain two-sweep run bit for bit in every owned
-cell and in dt, in float64 and float32."""
+SSPRK2 run (the fused path with three ghost rings, and the two-sweep path
+with two) equals the single-domain run bit for bit in every owned cell and in
+dt in float64, and within the float32 parity bar in float32."""
 import numpy as np
 import pytest
 import torch
@@ -42,7 +43,7 @@ def _cell(side, d, i, ny, nx, hw, recv):
 
 
 @pytest.mark.parametrize("dtype,lo", [(torch.float32, True), (torch.float64, False)])
-@pytest.mark.parametrize("hw", [1, 2])
+@pytest.mark.parametrize("hw", [1, 2, 3])
 def test_halo_strips_pack_and_unpack(cuda, dtype, lo, hw):
     ny, nx = 23, 31
     st = _state(ny, nx, dtype, lo, seed=5)
@@ -86,9 +87,10 @@ def test_halo_strips_pack_and_unpack(cuda, dtype, lo, hw):
             getattr(st, n).copy_(before[n])
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("which,par", [("channel", (2, 2)), ("radial", (3, 2)), ("wetdry", (2, 3))])
-def test_tiled_run_equals_single_domain(cuda, dtype, which, par):
+def test_tiled_run_equals_single_domain(cuda, dtype, which, par, fused):
     """float64: every owned cell of a tiled SSPRK2 run, and every dt, equal the
     single-domain two-sweep run bit for bit.  float32 (hardware math, fp
     contraction per inlined copy of the edge, so an edge evaluated at the start
@@ -102,10 +104,16 @@ def test_tiled_run_equals_single_domain(cuda, dtype, which, par):
         mesh, arr, params = cases.radial_dambreak(nx=60, ny=66, dx=0.05, n_bumps=6, radius=0.9)
     else:
         mesh, arr, params = cases.wetdry_obstacle(nx=75, ny=44)
-    params = dict(params, backend={"fused_stage": False})
+    # the single-domain run of the same stage path: in float32 the fused and
+    # two-sweep stages round differently, and a rounding difference at a
+    # discontinuous switch of the reference (an SED_TOL threshold of the bed,
+    # the Exner upwind side) can grow into a local bed difference beyond the
+    # float32 bar between the two paths themselves
+    params = dict(params, backend={"fused_stage": fused})
     sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
     mono = ExnerStepper(sc.backend, sg, mask, 0.5)
-    tiled = TiledExner(sc.backend, sg, mask, sc.boundaries, par, 0.5)
+    tiled = TiledExner(sc.backend, sg, mask, sc.boundaries, par, 0.5, fused=fused)
+    assert tiled.hw == (3 if fused else 2)
     exact = dtype == torch.float64
     names = FIELDS_EXNER + (("z_b_lo",) if sg.z_b_lo is not None else ())
     zb0 = sg.z_b.double() + (sg.z_b_lo.double() if sg.z_b_lo is not None else 0)
@@ -116,10 +124,13 @@ def test_tiled_run_equals_single_domain(cuda, dtype, which, par):
         mono.ssprk2_step()
         tiled.ssprk2_step()
         torch.cuda.synchronize()
+        # the fused path leaves the next step's dt (as the stepper does); the
+        # two-sweep path the dt it used
+        want_dt = mono.dt if fused else dt_used
         if exact:
-            assert torch.equal(tiled.dt, dt_used), (k, float(tiled.dt), float(dt_used))
+            assert torch.equal(tiled.dt, want_dt), (k, float(tiled.dt), float(want_dt))
         else:
-            assert abs(float(tiled.dt) - float(dt_used)) <= 1e-6 * float(dt_used), k
+            assert abs(float(tiled.dt) - float(want_dt)) <= 1e-6 * float(want_dt), k
     if exact:
         for f in names:
             got, want = tiled.gather(f), getattr(mono.state, f)
@@ -138,13 +149,14 @@ def test_tiled_run_equals_single_domain(cuda, dtype, which, par):
     assert float(mono.state.h.sum()) > 0
 
 
-def test_tiled_graph_replays_equal_eager_steps(cuda):
+@pytest.mark.parametrize("fused", [True, False])
+def test_tiled_graph_replays_equal_eager_steps(cuda, fused):
     """A captured tiled step replayed equals eager tiled steps, replay after replay."""
     mesh, arr, params = cases.channel(nx=90, ny=52, n_bumps=6, seed=3)
     params = dict(params, backend={"fused_stage": False})
     sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
-    eager = TiledExner(sc.backend, sg, mask, sc.boundaries, (2, 3), 0.5)
-    graphed = TiledExner(sc.backend, sg, mask, sc.boundaries, (2, 3), 0.5)
+    eager = TiledExner(sc.backend, sg, mask, sc.boundaries, (2, 3), 0.5, fused=fused)
+    graphed = TiledExner(sc.backend, sg, mask, sc.boundaries, (2, 3), 0.5, fused=fused)
     graphed.capture()  # one warm-up step before the capture
     eager.ssprk2_step()
     for k in range(4):

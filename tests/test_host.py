@@ -321,7 +321,8 @@ def test_halo_sizes_and_rejected_geometry():
     groups = (1 << _hip.G_H) | (1 << _hip.G_HUV) | (1 << _hip.G_ZB) | (1 << _hip.G_G)  # h hu hv z_b G
     assert lib.pxh_halo_bytes(grid, fs, 2, groups, _hip.HALO_N | _hip.HALO_S) == 2 * 2 * 9 * 5 * 4
     assert lib.pxh_halo_bytes(grid, fs, 1, groups, _hip.HALO_W) == 1 * 8 * 5 * 4
-    assert lib.pxh_halo_bytes(grid, fs, 3, groups, _hip.HALO_W) < 0               # hw beyond 2
+    assert lib.pxh_halo_bytes(grid, fs, 4, groups, _hip.HALO_W) < 0               # hw beyond 3
+    assert lib.pxh_halo_bytes(grid, fs, 3, groups, _hip.HALO_W) == 3 * 8 * 5 * 4   # three rings (8 rows >= 7)
     assert lib.pxh_halo_bytes(grid, fs, 2, groups, 0) < 0                          # no side
     assert lib.pxh_halo_bytes(grid, fs, 2, groups, 15) < 0                         # both phases in one call
     assert lib.pxh_halo_bytes(grid, fs, 2, groups, _hip.HALO_W | _hip.HALO_N) < 0
