@@ -243,12 +243,13 @@ int pxh_dt_finalize(const pxh_grid *grid, const void *dt_partial, double cfl, vo
  *     out.{h,hu,hv} of the cells within three cells of the edge (what the fill
  *     and the frame sweep read), old = 0.5 old + 0.5 S(in) over h, hu, hv,
  *     z_b, z_b_lo, G off the frame; with dt_partial, the per-block CFL minima of
- *     that combined state.
+ *     that combined state over the edges inside each strip.
  *   frame sweep: compute_G + exner_solve_2D of the frame's cells from the
  *     filled hyd.{h,hu,hv} and bed.{z_b,z_b_lo,G} (the stage input):
  *     old == NULL: into out.{z_b,z_b_lo,G}; old != NULL: combined into old as
- *     above, and with dt_partial (the one the stage sweep filled) the frame's
- *     minima and the final reduction: dt_dev = compute_dt_fn(old), so it holds
+ *     above, and with dt_partial (the one the stage sweep filled) the minima of
+ *     the frame's edges and of the edges across strip seams, and the final
+ *     reduction: dt_dev = compute_dt_fn(old), so it holds
  *     the stage's dt on entry and the next step's on return.
  * flags: PXH_SWEEP_SKIP_G leaves out.G unwritten in MPM mode (the next stage
  * recomputes G; the first SSPRK2 stage uses it).  dt_partial holds

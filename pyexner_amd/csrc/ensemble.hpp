@@ -29,6 +29,8 @@ struct EnsMember {
     int nstage;    // stage-sweep blocks
     int nframe;    // frame blocks
     int ndt;       // standalone-dt blocks
+    int tblk0;     // first frame + seam dt block
+    int ntdt;      // frame + seam dt blocks
     int pad;
     double dx;
 };
@@ -84,16 +86,17 @@ __global__ __launch_bounds__(256) void k_ens_band_bed(F<T> hyd, F<T> bed, F<T> o
                                      seds[i], write_G);
 }
 
+// The dt minima of each member's frame edges and strip-seam edges.
 template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_ens_band_dt(F<T> s, const uint8_t *__restrict__ m0,
-                                                     const EnsMember *__restrict__ tab, int n,
+                                                     const EnsMember *__restrict__ tab, int n, int sr,
                                                      T *__restrict__ partial) {
-    const int i = find_member<&EnsMember::fblk0>(tab, n, blockIdx.x);
+    const int i = find_member<&EnsMember::tblk0>(tab, n, blockIdx.x);
     const EnsMember m = tab[i];
-    const int local = blockIdx.x - m.fblk0;
+    const int local = blockIdx.x - m.tblk0;
     const long t = long(local) * 256 + threadIdx.x;
-    block256_min_to(band_dt_cell<T, FAST>(t, m.nband, shifted(s, m.off), m0 + m.off, m.ny, m.nx,
-                                          static_cast<T>(m.dx)),
+    block256_min_to(band_dt_cell<T, FAST>(t, m.nband, band_dt_work(m.ny, m.nx, sr), shifted(s, m.off), m0 + m.off,
+                                          m.ny, m.nx, static_cast<T>(m.dx), sr),
                     partial + m.part0 + m.nstage + local);
 }
 
@@ -116,14 +119,14 @@ __global__ __launch_bounds__(SW *SWPB) void k_ens_dt_strip(F<T> s, const uint8_t
 
 // One block per member: dts[i] = max(cfl * min of its minima, 1e-6)
 // (compute_dt_roeexner, roe_exner_solver.py:118-122).  step: the minima of a
-// fused SSPRK2 step (stage + frame blocks); else of the standalone dt sweep.
+// fused SSPRK2 step (stage + frame/seam blocks); else of the standalone dt sweep.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ens_dt_final(const T *__restrict__ partial,
                                                       const EnsMember *__restrict__ tab, T cfl, int step,
                                                       T *__restrict__ dts) {
     __shared__ T red[4];
     const EnsMember m = tab[blockIdx.x];
-    const int cnt = step ? m.nstage + m.nframe : m.ndt;
+    const int cnt = step ? m.nstage + m.ntdt : m.ndt;
     T v = inf_v<T>();
     for (int k = threadIdx.x; k < cnt; k += 256) v = pmin(v, partial[m.part0 + k]);
 #pragma unroll

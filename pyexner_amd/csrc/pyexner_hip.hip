@@ -377,11 +377,13 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
 //     (, G when write_G)} of the cells off the two-cell frame;
 //   stage sweep, old set: out.{h,hu,hv} of the cells within three cells of the
 //     edge, old = 0.5 old + 0.5 S(in) off the frame, and with partial the
-//     per-block dt minima of that combined state in partial[0, nf);
+//     per-block dt minima of that combined state within each strip in
+//     partial[0, nf);
 //   frame sweep: the frame's bed update from the filled hyd and in's bed
-//     (MODE 0 into out, MODE 1 combined into old), and with partial the frame's
-//     dt minima in partial[nf, nf + nbb) followed by the final reduction of
-//     all nf + nbb minima into *dtp.
+//     (MODE 0 into out, MODE 1 combined into old), and with partial the dt
+//     minima of the frame's edges and of the edges across strip seams in
+//     partial[nf, nf + nbd) followed by the final reduction of all nf + nbd
+//     minima into *dtp.
 template <typename T, bool FAST>
 static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, const F<T> *old, const uint8_t *m0,
                          const pxh_sediments *sd, const T *dtp, int write_G, T *part, hipStream_t st) {
@@ -419,11 +421,15 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
                        dx, sed_k(*sd), 1, nband);
     if (int e = launch_status()) return e;
     if (!part) return PXH_OK;
-    const dim3 fg = stage_grid(ny, nx, strip_rows(g->flags));
+    const int sr = strip_rows(g->flags);
+    const dim3 fg = stage_grid(ny, nx, sr);
     const int nf = int(fg.x * fg.y);
-    hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbb), dim3(256), 0, st, *old, m0, ny, nx, dx, nband, part + nf);
+    const long nwork = band_dt_work(ny, nx, sr);
+    const int nbd = int((nwork + 255) / 256);
+    hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbd), dim3(256), 0, st, *old, m0, ny, nx, dx, sr, nband, nwork,
+                       part + nf);
     if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nf + nbb, static_cast<T>(cfl), 1, dtp);
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nf + nbd, static_cast<T>(cfl), 1, dtp);
     return launch_status();
 }
 
@@ -474,10 +480,10 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
 // The seam edges of a two-sweep combine (k_rx_seam_dt) into part[0, seam_blocks).
 template <typename T, bool FAST>
 static int seam_dt(const pxh_grid *g, const F<T> &s, const uint8_t *m0, T *part, hipStream_t st) {
-    const int ns = seam_count(g->ny);
-    if (ns == 0) return PXH_OK;
-    hipLaunchKernelGGL((k_rx_seam_dt<T, FAST>), dim3(seam_blocks(g->ny, g->nx)), dim3(256), 0, st, s, m0, g->ny,
-                       g->nx, static_cast<T>(g->dx), ns, part);
+    const int nsb = seam_blocks(g->ny, g->nx);
+    if (nsb == 0) return PXH_OK;
+    hipLaunchKernelGGL((k_rx_seam_dt<T, FAST>), dim3(nsb), dim3(256), 0, st, s, m0, g->ny, g->nx,
+                       static_cast<T>(g->dx), seam_edges_2s(g->ny, g->nx), part);
     return launch_status();
 }
 
@@ -795,7 +801,7 @@ struct EnsPlanHeader {
     int32_t fblocks;       // frame blocks
     int32_t dblocks;       // standalone-dt blocks
     int32_t nparts;        // partial minima
-    int32_t pad;
+    int32_t tblocks;       // frame + seam dt blocks
     int64_t tab_off, sed_off, bytes;
     int64_t cells;         // arena elements the members span
 };
@@ -821,7 +827,7 @@ static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *me
     h.mpm = mem[0].sed.mpm;
     EnsMember *tab = reinterpret_cast<EnsMember *>(base + tab_off);
     SedK *seds = reinterpret_cast<SedK *>(base + sed_off);
-    int64_t sb = 0, fb = 0, db = 0, np = 0, cells = 0;
+    int64_t sb = 0, fb = 0, db = 0, tb = 0, np = 0, cells = 0;
     const long esz = dtype == PXH_F64 ? 8 : 4;
     for (int i = 0; i < n; ++i) {
         const pxh_member &q = mem[i];
@@ -839,31 +845,36 @@ static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *me
         m.nstage = int(fg.x * fg.y);
         m.nband = frame_cells(q.ny, q.nx);
         m.nframe = int((m.nband + 255) / 256);
+        m.ntdt = int((band_dt_work(q.ny, q.nx, strip_rows(flags)) + 255) / 256);
         m.dgx = int(dg.x);
         m.ndt = int(dg.x * dg.y);
         m.sblk0 = int(sb);
         m.fblk0 = int(fb);
         m.dblk0 = int(db);
+        m.tblk0 = int(tb);
         m.part0 = int(np);
         sb += m.nstage;
         fb += m.nframe;
         db += m.ndt;
-        np += (m.nstage + m.nframe > m.ndt ? m.nstage + m.nframe : m.ndt);
+        tb += m.ntdt;
+        np += (m.nstage + m.ntdt > m.ndt ? m.nstage + m.ntdt : m.ndt);
         const int64_t end = q.off + int64_t(q.ny) * q.nx;
         if (end > cells) cells = end;
         tab[i] = m;
         seds[i] = sed_k(sd);
     }
-    if (sb >= (1L << 31) || np >= (1L << 31) || cells >= (1L << 31)) return PXH_E_RANGE;
+    if (sb >= (1L << 31) || tb >= (1L << 31) || np >= (1L << 31) || cells >= (1L << 31)) return PXH_E_RANGE;
     EnsMember sentinel{};
     sentinel.sblk0 = int(sb);
     sentinel.fblk0 = int(fb);
     sentinel.dblk0 = int(db);
+    sentinel.tblk0 = int(tb);
     sentinel.part0 = int(np);
     tab[n] = sentinel;
     h.sblocks = int(sb);
     h.fblocks = int(fb);
     h.dblocks = int(db);
+    h.tblocks = int(tb);
     h.nparts = int(np);
     h.tab_off = tab_off;
     h.sed_off = sed_off;
@@ -936,7 +947,8 @@ static int ens_ssprk2_t(const EnsPlanHeader &h, const char *pd, const pxh_fields
     hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 1>), dim3(h.fblocks), dim3(256), 0, st, c, b, none, a, m0, dts, tab,
                        h.n, seds, 1);
     if (int e = launch_status()) return e;
-    hipLaunchKernelGGL((k_ens_band_dt<T, FAST>), dim3(h.fblocks), dim3(256), 0, st, a, m0, tab, h.n, part);
+    hipLaunchKernelGGL((k_ens_band_dt<T, FAST>), dim3(h.tblocks), dim3(256), 0, st, a, m0, tab, h.n,
+                       strip_rows(h.flags), part);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(&g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
     hipLaunchKernelGGL(k_ens_dt_final<T>, dim3(h.n), dim3(256), 0, st, part, tab, static_cast<T>(cfl), 1, dts);
@@ -1124,7 +1136,7 @@ int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
     const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx, strip_rows(g->flags));
     int64_t nb = int64_t(sa.x) * sa.y + seam_blocks(g->ny, g->nx);
-    const int64_t nfused = int64_t(fg.x) * fg.y + (frame_cells(g->ny, g->nx) + 255) / 256;
+    const int64_t nfused = int64_t(fg.x) * fg.y + (band_dt_work(g->ny, g->nx, strip_rows(g->flags)) + 255) / 256;
     if (nfused > nb) nb = nfused;
     return (nb > DT_MAX_BLOCKS ? nb : DT_MAX_BLOCKS) * 8;
 }

@@ -137,7 +137,8 @@ __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, co
 //   three cells of the grid edge (what the frame's bed update and the ring
 //   fill read); old = 0.5 old + 0.5 result over h, hu, hv, G, z_b, z_b_lo of
 //   the interior owned cells; with DT, the CFL minimum over the edges between
-//   interior cells of the combined state into partial[block].
+//   interior cells of the strip of the combined state into partial[block]
+//   (the edges across strip seams are left to band_dt_cell).
 // in: the stage input (hydro fields, z, n_b, z_b, z_b_lo, G).
 // The strip p of one domain; returns this lane's dt minimum (DT).
 // Everything by value: a reference to a kernel argument reaches the inlined
@@ -160,6 +161,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
         const bool colI = c >= fr.lft && c < fr.rgt;           // interior column
         const bool colIE = c >= fr.lft && c + 1 < fr.rgt;      // x edge (c | c+1) between interior cells
         const bool frame3 = c <= 2 || c >= nx - 3;             // MODE 1: store the stage hydro here
+        const bool eown = p.lane + 1 < FHW + FO;                // the lane right of this one is owned
         // hydro rows [hr0, hr1): the owned rows and one either side
         const int hr0 = p.r0 > 0 ? p.r0 - 1 : 0;
         const int hr1 = p.r1 + 1 < ny ? p.r1 + 1 : ny;
@@ -266,11 +268,14 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                         }
                     }
                     if (MODE == 1 && DT) {
-                        // edges between interior cells: (c | c+1) of row rb,
-                        // and (rb-1 | rb) when row rb-1 is interior too
+                        // edges between interior cells of this strip: (c | c+1)
+                        // of row rb, and (rb-1 | rb) when row rb-1 is owned too.
+                        // The combined value of a neighbouring strip's cell is not
+                        // in reach: that strip may already have stored it over
+                        // `old`.  The frame dt kernel takes the seam edges.
                         const DD<T> dcur = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, edp.flags & 2);
                         const DD<T> deast = dd_shdn(dcur);
-                        if (p.out && colIE && dd_active(dcur, deast))
+                        if (p.out && eown && colIE && dd_active(dcur, deast))
                             vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
                         if (p.out && colI && have_dprev && dd_active(dprev, dcur))
                             vmin = pmin(vmin, dt_edge_d<1, T, FAST>(dprev, dcur, dx));
@@ -436,42 +441,78 @@ __device__ __forceinline__ DD<T> dd_at(const F<T> &s, const uint8_t *m0, long i)
     return dd_make<T, FAST>(s.h[i], s.hu[i], s.hv[i], s.G[i], m0[i]);
 }
 
-// The y edges (r-1 | r) at the first row r of every two-sweep bed strip but
-// the first, from the stored combined state: the strip above cannot take row
-// r's combined value from `old` while the strip below combines row r in place
-// (it may already hold the result), so those edges are reduced here, after the
-// combine.  nseam = (ny - 1) / SR_BED seams; one partial per block.
-template <typename T, bool FAST>
-__global__ __launch_bounds__(256) void k_rx_seam_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
-                                                     int nseam, T *__restrict__ partial) {
+// Minimum over a 256-thread block, written by thread 0 to *slot.
+template <typename T>
+__device__ __forceinline__ void block256_min_to(T v, T *slot) {
     __shared__ T red[4];
-    const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    T v = inf_v<T>();
-    if (t < long(nseam) * nx) {
-        const int k = int(t / nx), c = int(t - long(k) * nx);
-        const long i = long(k + 1) * SR_BED * nx + c;
-        const DD<T> n = dd_at<T, FAST>(s, m0, i - nx), me = dd_at<T, FAST>(s, m0, i);
-        if (dd_active(n, me)) v = dt_edge_d<1, T, FAST>(n, me, dx);
-    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 1; k < 4; ++k) v = pmin(v, red[k]);
-        partial[blockIdx.x] = v;
+        *slot = v;
     }
 }
 
-static inline int seam_count(int ny) { return (ny - 1) / SR_BED; }
-static inline int seam_blocks(int ny, int nx) { return int((long(seam_count(ny)) * nx + 255) / 256); }
+// Seam edges of a strip decomposition with rs-row, cs-column strips: the y
+// edges (r-1 | r) at r = rs, 2 rs, ... < ny, then the x edges (c-1 | c) at
+// c = cs, 2 cs, ... < nx.  A combine sweep writes the combined state over
+// `old` in place, so a strip cannot take the combined value of a cell another
+// strip owns; the edges between two strips are reduced after the sweep, from
+// the stored combined state.
+static __host__ __device__ inline long seam_edges(int ny, int nx, int rs, int cs) {
+    return long((ny - 1) / rs) * nx + long((nx - 1) / cs) * ny;
+}
+
+// The CFL bound of seam edge u (0 <= u < seam_edges): y edge u = k nx + c,
+// then x edge u = r nxs + j (the seams of one row side by side).
+template <typename T, bool FAST>
+__device__ __forceinline__ T seam_dt_edge(long u, const F<T> &s, const uint8_t *__restrict__ m0, int ny, int nx,
+                                          T dx, int rs, int cs) {
+    const long nyw = long((ny - 1) / rs) * nx;
+    const int nxs = (nx - 1) / cs;
+    T v = inf_v<T>();
+    if (u < nyw) {
+        const long k = u / nx;
+        const int c = int(u - k * nx);
+        const long i = (k + 1) * rs * long(nx) + c;
+        const DD<T> n = dd_at<T, FAST>(s, m0, i - nx), me = dd_at<T, FAST>(s, m0, i);
+        if (dd_active(n, me)) v = dt_edge_d<1, T, FAST>(n, me, dx);
+    } else {
+        u -= nyw;
+        if (u < long(nxs) * ny) {
+            const long r = u / nxs;
+            const int j = int(u - r * nxs);
+            const long i = r * nx + long(j + 1) * cs;
+            const DD<T> w = dd_at<T, FAST>(s, m0, i - 1), me = dd_at<T, FAST>(s, m0, i);
+            if (dd_active(w, me)) v = dt_edge_d<0, T, FAST>(w, me, dx);
+        }
+    }
+    return v;
+}
+
+// The seam edges of the two-sweep combine (SR_BED-row, SO_WIDE-column bed
+// strips), one partial per 256 edges.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_rx_seam_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
+                                                     long nedge, T *__restrict__ partial) {
+    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    T v = inf_v<T>();
+    if (t < nedge) v = seam_dt_edge<T, FAST>(t, s, m0, ny, nx, dx, SR_BED, SO_WIDE);
+    block256_min_to(v, partial + blockIdx.x);
+}
+
+static inline long seam_edges_2s(int ny, int nx) { return seam_edges(ny, nx, SR_BED, SO_WIDE); }
+static inline int seam_blocks(int ny, int nx) { return int((seam_edges_2s(ny, nx) + 255) / 256); }
 
 // CFL minimum over every edge with a frame cell on either side (an edge
 // between two frame cells may be evaluated twice; the minimum does not mind),
-// of the combined state s; one partial per block.
+// of the combined state s, for t < nband; for nband <= t the seam edges of the
+// fused stage's sr-row, FO-column strips (seam_dt_edge); one partial per block.
 template <typename T, bool FAST>
-__device__ __forceinline__ T band_dt_cell(long t, long nband, const F<T> s, const uint8_t *__restrict__ m0, int ny,
-                                          int nx, T dx) {
+__device__ __forceinline__ T band_dt_cell(long t, long nband, long nwork, const F<T> s,
+                                          const uint8_t *__restrict__ m0, int ny, int nx, T dx, int sr) {
     T v = inf_v<T>();
     if (t < nband) {
         int r, c;
@@ -494,29 +535,23 @@ __device__ __forceinline__ T band_dt_cell(long t, long nband, const F<T> s, cons
             const DD<T> n = dd_at<T, FAST>(s, m0, i - nx);
             if (dd_active(n, me)) v = pmin(v, dt_edge_d<1, T, FAST>(n, me, dx));
         }
+    } else if (t < nwork) {
+        v = seam_dt_edge<T, FAST>(t - nband, s, m0, ny, nx, dx, sr, FO);
     }
     return v;
 }
 
-// Minimum over a 256-thread block, written by thread 0 to *slot.
-template <typename T>
-__device__ __forceinline__ void block256_min_to(T v, T *slot) {
-    __shared__ T red[4];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; ++k) v = pmin(v, red[k]);
-        *slot = v;
-    }
+// Work items of the frame dt kernel: the frame's cells, then the fused
+// stage's seam edges.
+static __host__ __device__ inline long band_dt_work(int ny, int nx, int sr) {
+    return frame_cells(ny, nx) + seam_edges(ny, nx, sr, FO);
 }
 
 template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
-                                                     long nband, T *__restrict__ partial) {
+                                                     int sr, long nband, long nwork, T *__restrict__ partial) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    block256_min_to(band_dt_cell<T, FAST>(t, nband, s, m0, ny, nx, dx), partial + blockIdx.x);
+    block256_min_to(band_dt_cell<T, FAST>(t, nband, nwork, s, m0, ny, nx, dx, sr), partial + blockIdx.x);
 }
 
 }  // namespace pxh

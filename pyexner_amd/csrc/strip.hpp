@@ -1059,13 +1059,14 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
             if (MODE == 1) {
                 if (DT) {
                     // edges (c | c+1) of row r and (r | r+1), owned by the output
-                    // lanes; (r | r+1) only inside the strip: row r1's combined value
-                    // belongs to the strip below, which may already have stored it
-                    // over `old` (k_rx_seam_dt takes those edges)
+                    // lanes, only inside the strip: the combined value of row r1 and
+                    // of the column right of the last output lane belongs to a
+                    // neighbouring strip, which may already have stored it over
+                    // `old` (k_rx_seam_dt takes those edges)
                     DD<T> dnxt = dcur;
                     if (hasS) dnxt = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, nxt.flags & 2);
                     const DD<T> deast = dd_shdn(dcur);
-                    if (p.out && c + 1 < nx && dd_active(dcur, deast))
+                    if (p.out && p.lane < SO_WIDE && c + 1 < nx && dd_active(dcur, deast))
                         vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
                     if (p.out && r + 1 < p.r1 && dd_active(dcur, dnxt))
                         vmin = pmin(vmin, dt_edge_d<1, T, FAST>(dcur, dnxt, dx));

@@ -359,3 +359,47 @@ def test_next_dt_of_ssprk2_matches_oracle_f64(cuda, fused):
         st.ssprk2_step()
         want = float(prob.dt(so))
         assert abs(float(st.dt) - want) <= 1e-13 * want, (k, float(st.dt), want)
+
+
+# (fused, strip rows, pair of cells, direction): a pair of fast cells whose
+# shared edge crosses a strip seam of the combine sweep that reduces the next
+# dt.  Fused stage: 60-column strips of 16 rows here; two-sweep combine:
+# 62-column strips of 16 rows.
+SEAM_CASES = [
+    (True, 16, ((31, 20), (32, 20)), "y"),
+    (True, 16, ((20, 59), (20, 60)), "x"),
+    (True, 24, ((47, 20), (48, 20)), "y"),
+    (False, 0, ((31, 20), (32, 20)), "y"),
+    (False, 0, ((20, 61), (20, 62)), "x"),
+]
+
+
+@pytest.mark.parametrize("fused,rows,pair,axis", SEAM_CASES)
+def test_next_dt_across_strip_seams_f64(cuda, fused, rows, pair, axis):
+    """The next dt an SSPRK2 step leaves equals compute_dt_fn of the new state
+    (oracle) when the fastest edge lies between two strips of the combine
+    sweep.  A strip stores its combined cells over the step's input in place,
+    so the edges between two strips are reduced after the sweep from the stored
+    state; they were once left out (row seams of the fused stage) or taken from
+    a neighbour's cell that might already hold its combined value."""
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = cases.periodic_box(nx=130, ny=70)
+    arr = dict(arr)
+    hu, hv = arr["hu"].copy(), arr["hv"].copy()
+    for r, c in pair:
+        if axis == "y":
+            hv[r, c] = 2.0 * arr["h"][r, c]
+        else:
+            hu[r, c] = 2.0 * arr["h"][r, c]
+    arr["hu"], arr["hv"] = hu, hv
+    params = dict(params, backend={"fused_stage": fused})
+    so, prob = oracle_exner(mesh, arr, params)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    if rows:
+        sc.backend.set_strip_rows(rows)
+    st = ExnerStepper(sc.backend, sg, mask, prob.cfl)
+    for k in range(4):
+        so, _ = osteps.ssprk2_step(prob, so)
+        st.ssprk2_step()
+        want = float(prob.dt(so))
+        assert abs(float(st.dt) - want) <= 1e-13 * want, (k, float(st.dt), want)
