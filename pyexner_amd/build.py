@@ -34,12 +34,16 @@ def build(verbose=False):
         cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, src]
         procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), cmd))
         objs.append(obj)
+    # wait for every compile before reporting a failure, so that none is left
+    # running (and writing its temporary object) behind the error
+    results = []
     for p, cmd in procs:
         out, err = p.communicate()
-        r = subprocess.CompletedProcess(cmd, p.returncode, out, err)
-        _check(r, cmd)
+        results.append(subprocess.CompletedProcess(cmd, p.returncode, out, err))
+    for r in results:
+        _check(r, r.args)
         if verbose:
-            sys.stderr.write(err)
+            sys.stderr.write(r.stderr)
     tmp = OUT + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     _check(subprocess.run(cmd, capture_output=True, text=True), cmd)

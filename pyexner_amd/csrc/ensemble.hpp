@@ -19,6 +19,7 @@ namespace pxh {
 struct EnsMember {
     long off;      // first element of the member's arrays in the arenas
     long nband;    // cells of its two-cell frame
+    long xs0;      // first element of its seam-column buffer (seam_rec)
     int ny, nx;
     int sgx;       // stage-grid columns
     int sblk0;     // first stage-sweep block
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> 
                                                         const uint8_t *__restrict__ m0, const T *__restrict__ dts,
                                                         const EnsMember *__restrict__ tab, int n, int sr,
                                                         const SedK *__restrict__ seds, int write_G,
-                                                        T *__restrict__ partial) {
+                                                        T *__restrict__ partial, T *__restrict__ xs) {
     __shared__ T red[FWPB];
     int lb, unused;
     xcd_block(lb, unused);  // one-dimensional grid: lb is the XCD-ordered block
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> 
     T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, shifted(in, m.off), shifted(out, m.off),
                                                          shifted(cb, m.off), shifted(old, m.off), m0 + m.off,
                                                          dts + i, m.ny, m.nx, static_cast<T>(m.dx), seds[i],
-                                                         write_G);
+                                                         write_G, DT ? xs + m.xs0 : nullptr);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
         if (threadIdx.x == 0) partial[m.part0 + local] = vmin;
@@ -90,13 +91,13 @@ __global__ __launch_bounds__(256) void k_ens_band_bed(F<T> hyd, F<T> bed, F<T> o
 template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_ens_band_dt(F<T> s, const uint8_t *__restrict__ m0,
                                                      const EnsMember *__restrict__ tab, int n, int sr,
-                                                     T *__restrict__ partial) {
+                                                     const T *__restrict__ xs, T *__restrict__ partial) {
     const int i = find_member<&EnsMember::tblk0>(tab, n, blockIdx.x);
     const EnsMember m = tab[i];
     const int local = blockIdx.x - m.tblk0;
     const long t = long(local) * 256 + threadIdx.x;
     block256_min_to(band_dt_cell<T, FAST>(t, m.nband, band_dt_work(m.ny, m.nx, sr), shifted(s, m.off), m0 + m.off,
-                                          m.ny, m.nx, static_cast<T>(m.dx), sr),
+                                          xs + m.xs0, m.ny, m.nx, static_cast<T>(m.dx), sr),
                     partial + m.part0 + m.nstage + local);
 }
 

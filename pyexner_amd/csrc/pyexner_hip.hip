@@ -345,6 +345,23 @@ static inline int dt_blocks(const pxh_grid *g) {
     return int(n < DT_MAX_BLOCKS ? n : DT_MAX_BLOCKS);
 }
 
+// The dt workspace (pxh_dt_workspace_size): partial minima, 8 bytes each, for
+// the largest launch sequence that reduces a dt, then the seam-column buffer
+// of the combine sweeps (seam_rec).
+static int64_t dt_part_slots(const pxh_grid *g) {
+    const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx, strip_rows(g->flags));
+    int64_t nb = int64_t(sa.x) * sa.y + seam_blocks(g->ny, g->nx);
+    const int64_t nfused = int64_t(fg.x) * fg.y + (band_dt_work(g->ny, g->nx, strip_rows(g->flags)) + 255) / 256;
+    if (nfused > nb) nb = nfused;
+    nb = nb > DT_MAX_BLOCKS ? nb : DT_MAX_BLOCKS;
+    return (nb + 31) & ~int64_t(31);  // the buffer after it starts 256-byte aligned
+}
+
+template <typename T>
+static T *seam_cols(const pxh_grid *g, void *ws) {
+    return reinterpret_cast<T *>(static_cast<char *>(ws) + dt_part_slots(g) * 8);
+}
+
 template <typename T>
 static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_t groups, hipStream_t st) {
     if (!bc || bc->n_entries <= 0) return PXH_OK;
@@ -394,12 +411,13 @@ static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, con
     const dim3 fg = stage_grid(ny, nx, sr);
     if (!old)
         launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, sr, dx, sed_k(*sd), write_G,
-                                        static_cast<T *>(nullptr));
+                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr));
     else if (part)
-        launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1, part);
+        launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1, part,
+                                       seam_cols<T>(g, part));
     else
         launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1,
-                                        static_cast<T *>(nullptr));
+                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr));
     return launch_status();
 }
 
@@ -426,8 +444,8 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
     const int nf = int(fg.x * fg.y);
     const long nwork = band_dt_work(ny, nx, sr);
     const int nbd = int((nwork + 255) / 256);
-    hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbd), dim3(256), 0, st, *old, m0, ny, nx, dx, sr, nband, nwork,
-                       part + nf);
+    hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbd), dim3(256), 0, st, *old, m0, seam_cols<T>(g, part), ny, nx,
+                       dx, sr, nband, nwork, part + nf);
     if (int e = launch_status()) return e;
     hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nf + nbd, static_cast<T>(cfl), 1, dtp);
     return launch_status();
@@ -472,18 +490,18 @@ static int rx_step_t(const pxh_grid *g, const pxh_fields *in_, const pxh_fields 
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
     hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), bg, sb, 0, st, out, in.z_b, in.z_b_lo, in.G, out, out,
-                       m0, dtp, ny, nx, dx, sed_k(*sd), 1, nullptr);
+                       m0, dtp, ny, nx, dx, sed_k(*sd), 1, nullptr, nullptr);
     if (int e = launch_status()) return e;
     return bc_launch<T>(g, out, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
 }
 
 // The seam edges of a two-sweep combine (k_rx_seam_dt) into part[0, seam_blocks).
 template <typename T, bool FAST>
-static int seam_dt(const pxh_grid *g, const F<T> &s, const uint8_t *m0, T *part, hipStream_t st) {
+static int seam_dt(const pxh_grid *g, const F<T> &s, const uint8_t *m0, const T *xs, T *part, hipStream_t st) {
     const int nsb = seam_blocks(g->ny, g->nx);
     if (nsb == 0) return PXH_OK;
-    hipLaunchKernelGGL((k_rx_seam_dt<T, FAST>), dim3(nsb), dim3(256), 0, st, s, m0, g->ny, g->nx,
-                       static_cast<T>(g->dx), seam_edges_2s(g->ny, g->nx), part);
+    hipLaunchKernelGGL((k_rx_seam_dt<T, FAST>), dim3(nsb), dim3(256), 0, st, s, m0, xs, g->ny, g->nx,
+                       static_cast<T>(g->dx), part);
     return launch_status();
 }
 
@@ -531,16 +549,16 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, HY, st)) return e;
     hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), bg, sb, 0, st, b, a.z_b, a.z_b_lo, a.G, b, b, m0, dtp,
-                       ny, nx, dx, sed_k(*sd), sd->mpm ? 0 : 1, nullptr);
+                       ny, nx, dx, sed_k(*sd), sd->mpm ? 0 : 1, nullptr, nullptr);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, b, c, m0, dtp, ny, nx, dx, 1);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
     hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), bg, sb, 0, st, c, b.z_b, b.z_b_lo, b.G, a, a, m0, dtp, ny,
-                       nx, dx, sed_k(*sd), 1, part);
+                       nx, dx, sed_k(*sd), 1, part, seam_cols<T>(g, ws));
     if (int e = launch_status()) return e;
-    if (int e = seam_dt<T, FAST>(g, a, m0, part + nb, st)) return e;
+    if (int e = seam_dt<T, FAST>(g, a, m0, seam_cols<T>(g, ws), part + nb, st)) return e;
     if (int e = bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
     hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nb + seam_blocks(ny, nx),
                        static_cast<T>(cfl), 1, dtp);
@@ -568,17 +586,18 @@ static int rx_bed_sweep_t(const pxh_grid *g, const pxh_fields *hyd_, const pxh_f
     const T dx = static_cast<T>(g->dx);
     if (!old_) {
         hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 0, false>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G, out,
-                           out, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr);
+                           out, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr, nullptr);
     } else {
         F<T> old = cast_fields<T>(old_);
         if (partial) {
+            T *xs = seam_cols<T>(g, partial);
             hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, true>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
-                               out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, static_cast<T *>(partial));
+                               out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, static_cast<T *>(partial), xs);
             if (int e = launch_status()) return e;
-            return seam_dt<T, FAST>(g, out, m0, static_cast<T *>(partial) + sg.x * sg.y, st);
+            return seam_dt<T, FAST>(g, out, m0, xs, static_cast<T *>(partial) + sg.x * sg.y, st);
         } else
             hipLaunchKernelGGL((k_rx_bed_strip<T, FAST, 1, false>), sg, sb, 0, st, hyd, bed.z_b, bed.z_b_lo, bed.G,
-                               out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr);
+                               out, old, m0, dtp, g->ny, g->nx, dx, sed_k(*sd), 1, nullptr, nullptr);
     }
     return launch_status();
 }
@@ -804,6 +823,7 @@ struct EnsPlanHeader {
     int32_t tblocks;       // frame + seam dt blocks
     int64_t tab_off, sed_off, bytes;
     int64_t cells;         // arena elements the members span
+    int64_t xs_elems;      // seam-column buffers of all members
 };
 constexpr uint32_t ENS_MAGIC = 0x50584845u;
 
@@ -827,7 +847,7 @@ static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *me
     h.mpm = mem[0].sed.mpm;
     EnsMember *tab = reinterpret_cast<EnsMember *>(base + tab_off);
     SedK *seds = reinterpret_cast<SedK *>(base + sed_off);
-    int64_t sb = 0, fb = 0, db = 0, tb = 0, np = 0, cells = 0;
+    int64_t sb = 0, fb = 0, db = 0, tb = 0, np = 0, cells = 0, xe = 0;
     const long esz = dtype == PXH_F64 ? 8 : 4;
     for (int i = 0; i < n; ++i) {
         const pxh_member &q = mem[i];
@@ -853,6 +873,8 @@ static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *me
         m.dblk0 = int(db);
         m.tblk0 = int(tb);
         m.part0 = int(np);
+        m.xs0 = xe;
+        xe += seam_col_elems(q.ny, q.nx);
         sb += m.nstage;
         fb += m.nframe;
         db += m.ndt;
@@ -880,9 +902,14 @@ static int64_t ens_plan_build(int32_t dtype, int32_t flags, const pxh_member *me
     h.sed_off = sed_off;
     h.bytes = bytes;
     h.cells = cells;
+    h.xs_elems = xe;
     *reinterpret_cast<EnsPlanHeader *>(base) = h;
     return bytes;
 }
+
+// The ensemble workspace: partial minima (8 bytes each), then the members'
+// seam-column buffers from this byte offset.
+static int64_t ens_xs_offset(const EnsPlanHeader &h) { return (int64_t(h.nparts) * 8 + 255) & ~int64_t(255); }
 
 static const EnsPlanHeader *ens_header(const void *plan) {
     const EnsPlanHeader *h = static_cast<const EnsPlanHeader *>(plan);
@@ -892,11 +919,11 @@ static const EnsPlanHeader *ens_header(const void *plan) {
 template <typename T, bool FAST, int MODE, bool DT>
 static void ens_stage(const EnsPlanHeader &h, const EnsMember *tab, const SedK *seds, hipStream_t st, const F<T> &in,
                       const F<T> &out, const F<T> &cb, const F<T> &old, const uint8_t *m0, const T *dts,
-                      int write_G, T *partial) {
+                      int write_G, T *partial, T *xs) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
 #define PXH_ENS(LO_, MPM_)                                                                                       \
     hipLaunchKernelGGL((k_ens_stage<T, FAST, MODE, DT, LO_, MPM_>), dim3(h.sblocks), dim3(SW * FWPB), 0, st, in,  \
-                       out, cb, old, m0, dts, tab, h.n, strip_rows(h.flags), seds, write_G, partial)
+                       out, cb, old, m0, dts, tab, h.n, strip_rows(h.flags), seds, write_G, partial, xs)
     if constexpr (sizeof(T) == 4) {
         if (lo) {
             if (h.mpm) PXH_ENS(true, true);
@@ -933,7 +960,8 @@ static int ens_ssprk2_t(const EnsPlanHeader &h, const char *pd, const pxh_fields
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (h.mpm ? 0u : (1u << PXH_G_G));
     // stage 1: b = S(a)
-    ens_stage<T, FAST, 0, false>(h, tab, seds, st, a, b, none, none, m0, dts, h.mpm ? 0 : 1, nullptr);
+    T *xs = reinterpret_cast<T *>(reinterpret_cast<char *>(part) + ens_xs_offset(h));
+    ens_stage<T, FAST, 0, false>(h, tab, seds, st, a, b, none, none, m0, dts, h.mpm ? 0 : 1, nullptr, nullptr);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(&g, b, bc, HY, st)) return e;
     hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 0>), dim3(h.fblocks), dim3(256), 0, st, b, a, b, none, m0, dts, tab,
@@ -941,14 +969,14 @@ static int ens_ssprk2_t(const EnsPlanHeader &h, const char *pd, const pxh_fields
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(&g, b, bc, BED, st)) return e;
     // stage 2: c = S(b), a = 0.5 a + 0.5 c, and each member's dt of the new a
-    ens_stage<T, FAST, 1, true>(h, tab, seds, st, b, none, c, a, m0, dts, 1, part);
+    ens_stage<T, FAST, 1, true>(h, tab, seds, st, b, none, c, a, m0, dts, 1, part, xs);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(&g, c, bc, HY, st)) return e;
     hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 1>), dim3(h.fblocks), dim3(256), 0, st, c, b, none, a, m0, dts, tab,
                        h.n, seds, 1);
     if (int e = launch_status()) return e;
     hipLaunchKernelGGL((k_ens_band_dt<T, FAST>), dim3(h.tblocks), dim3(256), 0, st, a, m0, tab, h.n,
-                       strip_rows(h.flags), part);
+                       strip_rows(h.flags), xs, part);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(&g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st)) return e;
     hipLaunchKernelGGL(k_ens_dt_final<T>, dim3(h.n), dim3(256), 0, st, part, tab, static_cast<T>(cfl), 1, dts);
@@ -1134,11 +1162,8 @@ int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const px
 
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
-    const dim3 sa = strip_grid<SO_WIDE, SR_BED>(g->ny, g->nx), fg = stage_grid(g->ny, g->nx, strip_rows(g->flags));
-    int64_t nb = int64_t(sa.x) * sa.y + seam_blocks(g->ny, g->nx);
-    const int64_t nfused = int64_t(fg.x) * fg.y + (band_dt_work(g->ny, g->nx, strip_rows(g->flags)) + 255) / 256;
-    if (nfused > nb) nb = nfused;
-    return (nb > DT_MAX_BLOCKS ? nb : DT_MAX_BLOCKS) * 8;
+    const int64_t esz = g->dtype == PXH_F64 ? 8 : 4;
+    return dt_part_slots(g) * 8 + seam_col_elems(g->ny, g->nx) * esz;
 }
 
 int pxh_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
@@ -1234,7 +1259,7 @@ int64_t pxh_ensemble_plan(int32_t dtype, int32_t flags, const pxh_member *member
 int64_t pxh_ensemble_workspace_size(const void *plan) {
     const EnsPlanHeader *h = ens_header(plan);
     if (!h) return PXH_E_ARG;
-    return int64_t(h->nparts) * 8;
+    return ens_xs_offset(*h) + h->xs_elems * (h->dtype == PXH_F64 ? 8 : 4);
 }
 
 int pxh_ensemble_dt(const void *plan, const void *plan_dev, const pxh_fields *a, const uint8_t *m0, double cfl,

@@ -34,6 +34,7 @@ constexpr int FWPB = PXH_FWPB;  // fused strips (wavefronts) per block
 
 struct StagePos {
     int lane, c, cc, r0, r1;
+    int bx;     // strip column
     bool out;   // owned column
     bool hval;  // updated depth complete in this lane
 };
@@ -49,6 +50,7 @@ __device__ __forceinline__ StagePos stage_pos_at(int bx, int by, int ny, int nx,
     StagePos p;
     p.lane = threadIdx.x & (SW - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SW);
+    p.bx = bx;
     p.c = bx * FO - FHW + p.lane;
     p.cc = p.c < 0 ? 0 : (p.c >= nx ? nx - 1 : p.c);
     p.out = p.lane >= FHW && p.lane < FHW + FO && p.c < nx;
@@ -131,6 +133,10 @@ __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, co
     return q;
 }
 
+// Elements of a domain's seam-column buffer: stage_grid(ny, nx).x strip
+// columns (the narrower fused strips; the two-sweep combine's fit in it).
+static inline long seam_col_elems(int ny, int nx) { return long((nx + FO - 1) / FO) * ny * SEAM_REC; }
+
 // MODE 0 (a stage on its own): out.{h,hu,hv} of every owned cell;
 //   out.{z_b, z_b_lo (, G when write_G)} of the interior owned cells.
 // MODE 1 (the second SSPRK2 stage): cb.{h,hu,hv} of the owned cells within
@@ -138,7 +144,8 @@ __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, co
 //   fill read); old = 0.5 old + 0.5 result over h, hu, hv, G, z_b, z_b_lo of
 //   the interior owned cells; with DT, the CFL minimum over the edges between
 //   interior cells of the strip of the combined state into partial[block]
-//   (the edges across strip seams are left to band_dt_cell).
+//   (the edges across strip seams are left to band_dt_cell), and the combined
+//   dt inputs of the strip's first and last owned columns into xs (seam_rec).
 // in: the stage input (hydro fields, z, n_b, z_b, z_b_lo, G).
 // The strip p of one domain; returns this lane's dt minimum (DT).
 // Everything by value: a reference to a kernel argument reaches the inlined
@@ -148,7 +155,7 @@ template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
 __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, const F<T> out, const F<T> cb,
                                              const F<T> old, const uint8_t *__restrict__ m0,
                                              const T *__restrict__ dtp, int ny, int nx, T dx, const SedK &sd,
-                                             int write_G) {
+                                             int write_G, T *__restrict__ xs) {
     const Frame fr = frame_of(ny, nx);
     const T zero = T(0), hf = T(0.5);
     const int c = p.c;
@@ -274,6 +281,9 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                         // in reach: that strip may already have stored it over
                         // `old`.  The frame dt kernel takes the seam edges.
                         const DD<T> dcur = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, edp.flags & 2);
+                        if (p.out && colI && (p.lane == FHW || p.lane == FHW + FO - 1))
+                            seam_store(xs, p.bx, rb, ny, p.lane != FHW, cbn.h, cbn.hu, cbn.hv, cbn.G,
+                                       (edp.flags & 2) != 0);
                         const DD<T> deast = dd_shdn(dcur);
                         if (p.out && eown && colIE && dd_active(dcur, deast))
                             vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
@@ -296,10 +306,10 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
                                                              const uint8_t *__restrict__ m0,
                                                              const T *__restrict__ dtp, int ny, int nx, int sr,
                                                              T dx, SedK sd, int write_G,
-                                                             T *__restrict__ partial) {
+                                                             T *__restrict__ partial, T *__restrict__ xs) {
     __shared__ T red[FWPB];
     const StagePos p = stage_pos(ny, nx, sr);
-    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G);
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G, xs);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
         if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = vmin;
@@ -311,11 +321,11 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
 template <typename T, bool FAST, int MODE, bool DT>
 static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
                          const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, int sr, T dx,
-                         const SedK &sd, int write_G, T *partial) {
+                         const SedK &sd, int write_G, T *partial, T *xs) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
 #define PXH_STAGE(LO_, MPM_)                                                                                   \
     hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_>), grid, dim3(SW * FWPB), 0, st, in, out, cb, old, \
-                       m0, dtp, ny, nx, sr, dx, sd, write_G, partial)
+                       m0, dtp, ny, nx, sr, dx, sd, write_G, partial, xs)
     if constexpr (sizeof(T) == 4) {
         if (lo) {
             if (sd.mpm) PXH_STAGE(true, true);
@@ -455,64 +465,75 @@ __device__ __forceinline__ void block256_min_to(T v, T *slot) {
     }
 }
 
-// Seam edges of a strip decomposition with rs-row, cs-column strips: the y
-// edges (r-1 | r) at r = rs, 2 rs, ... < ny, then the x edges (c-1 | c) at
-// c = cs, 2 cs, ... < nx.  A combine sweep writes the combined state over
+// Seam edges of a combine sweep.  The sweep writes the combined state over
 // `old` in place, so a strip cannot take the combined value of a cell another
-// strip owns; the edges between two strips are reduced after the sweep, from
-// the stored combined state.
-static __host__ __device__ inline long seam_edges(int ny, int nx, int rs, int cs) {
-    return long((ny - 1) / rs) * nx + long((nx - 1) / cs) * ny;
+// strip owns; the edges between two strips are reduced after the sweep: the
+// y edges (r-1 | r) at r = rs, 2 rs, ... < ny from the stored state (rows
+// read whole), the x edges from the seam-column buffer the sweep filled
+// (seam_rec), x seam k lying between strip columns k and k+1.
+static __host__ __device__ inline long seam_y_edges(int ny, int nx, int rs) { return long((ny - 1) / rs) * nx; }
+
+// The CFL bound of y seam edge u = k nx + c.
+template <typename T, bool FAST>
+__device__ __forceinline__ T seam_dt_y(long u, const F<T> &s, const uint8_t *__restrict__ m0, int nx, T dx,
+                                       int rs) {
+    const long k = u / nx;
+    const int c = int(u - k * nx);
+    const long i = (k + 1) * rs * long(nx) + c;
+    const DD<T> n = dd_at<T, FAST>(s, m0, i - nx), me = dd_at<T, FAST>(s, m0, i);
+    return dd_active(n, me) ? dt_edge_d<1, T, FAST>(n, me, dx) : inf_v<T>();
 }
 
-// The CFL bound of seam edge u (0 <= u < seam_edges): y edge u = k nx + c,
-// then x edge u = r nxs + j (the seams of one row side by side).
+// The CFL bound of x seam edge u = k nrows + (r - r_lo) (rows side by side).
 template <typename T, bool FAST>
-__device__ __forceinline__ T seam_dt_edge(long u, const F<T> &s, const uint8_t *__restrict__ m0, int ny, int nx,
-                                          T dx, int rs, int cs) {
-    const long nyw = long((ny - 1) / rs) * nx;
-    const int nxs = (nx - 1) / cs;
-    T v = inf_v<T>();
-    if (u < nyw) {
-        const long k = u / nx;
-        const int c = int(u - k * nx);
-        const long i = (k + 1) * rs * long(nx) + c;
-        const DD<T> n = dd_at<T, FAST>(s, m0, i - nx), me = dd_at<T, FAST>(s, m0, i);
-        if (dd_active(n, me)) v = dt_edge_d<1, T, FAST>(n, me, dx);
-    } else {
-        u -= nyw;
-        if (u < long(nxs) * ny) {
-            const long r = u / nxs;
-            const int j = int(u - r * nxs);
-            const long i = r * nx + long(j + 1) * cs;
-            const DD<T> w = dd_at<T, FAST>(s, m0, i - 1), me = dd_at<T, FAST>(s, m0, i);
-            if (dd_active(w, me)) v = dt_edge_d<0, T, FAST>(w, me, dx);
-        }
-    }
-    return v;
+__device__ __forceinline__ T seam_dt_x(long u, const T *__restrict__ xs, int ny, T dx, int r_lo, int nrows) {
+    const int k = int(u / nrows);
+    const int r = r_lo + int(u - long(k) * nrows);
+    const T *L = seam_rec(xs, k, r, ny);      // side 1: the left strip's last column
+    const T *R = seam_rec(xs, k + 1, r, ny);  // side 0: the right strip's first column
+    const DD<T> w = dd_make<T, FAST>(L[4], L[5], L[6], L[7], L[9] != T(0));
+    const DD<T> me = dd_make<T, FAST>(R[0], R[1], R[2], R[3], R[8] != T(0));
+    return dd_active(w, me) ? dt_edge_d<0, T, FAST>(w, me, dx) : inf_v<T>();
 }
 
-// The seam edges of the two-sweep combine (SR_BED-row, SO_WIDE-column bed
-// strips), one partial per 256 edges.
+// The two-sweep combine (SR_BED-row, SO_WIDE-column bed strips, every row
+// combined by the strips): y seams, then x seams over rows [0, ny).
+static __host__ __device__ inline long seam_x_edges_2s(int ny, int nx) { return long((nx - 1) / SO_WIDE) * ny; }
+static inline long seam_edges_2s(int ny, int nx) { return seam_y_edges(ny, nx, SR_BED) + seam_x_edges_2s(ny, nx); }
+static inline int seam_blocks(int ny, int nx) { return int((seam_edges_2s(ny, nx) + 255) / 256); }
+
 template <typename T, bool FAST>
-__global__ __launch_bounds__(256) void k_rx_seam_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
-                                                     long nedge, T *__restrict__ partial) {
+__global__ __launch_bounds__(256) void k_rx_seam_dt(F<T> s, const uint8_t *__restrict__ m0, const T *__restrict__ xs,
+                                                     int ny, int nx, T dx, T *__restrict__ partial) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    const long ny_e = seam_y_edges(ny, nx, SR_BED);
     T v = inf_v<T>();
-    if (t < nedge) v = seam_dt_edge<T, FAST>(t, s, m0, ny, nx, dx, SR_BED, SO_WIDE);
+    if (t < ny_e) v = seam_dt_y<T, FAST>(t, s, m0, nx, dx, SR_BED);
+    else if (t - ny_e < seam_x_edges_2s(ny, nx)) v = seam_dt_x<T, FAST>(t - ny_e, xs, ny, dx, 0, ny);
     block256_min_to(v, partial + blockIdx.x);
 }
 
-static inline long seam_edges_2s(int ny, int nx) { return seam_edges(ny, nx, SR_BED, SO_WIDE); }
-static inline int seam_blocks(int ny, int nx) { return int((seam_edges_2s(ny, nx) + 255) / 256); }
+// The fused stage's seams (sr-row, FO-column strips): its sweep combines the
+// cells off the two-cell frame only, so its x seams are those between two
+// interior columns, over the interior rows; the frame's edges are the frame
+// band's.
+__host__ __device__ inline int fused_x_seams(int ny, int nx) {
+    const Frame f = frame_of(ny, nx);
+    return f.rgt > 0 ? (f.rgt - 1) / FO : 0;
+}
+__host__ __device__ inline long fused_seam_edges(int ny, int nx, int sr) {
+    const Frame f = frame_of(ny, nx);
+    return seam_y_edges(ny, nx, sr) + long(fused_x_seams(ny, nx)) * (f.bot - f.top);
+}
 
 // CFL minimum over every edge with a frame cell on either side (an edge
 // between two frame cells may be evaluated twice; the minimum does not mind),
 // of the combined state s, for t < nband; for nband <= t the seam edges of the
-// fused stage's sr-row, FO-column strips (seam_dt_edge); one partial per block.
+// fused stage's sr-row, FO-column strips (xs: its seam-column buffer).
 template <typename T, bool FAST>
 __device__ __forceinline__ T band_dt_cell(long t, long nband, long nwork, const F<T> s,
-                                          const uint8_t *__restrict__ m0, int ny, int nx, T dx, int sr) {
+                                          const uint8_t *__restrict__ m0, const T *__restrict__ xs, int ny, int nx,
+                                          T dx, int sr) {
     T v = inf_v<T>();
     if (t < nband) {
         int r, c;
@@ -536,7 +557,13 @@ __device__ __forceinline__ T band_dt_cell(long t, long nband, long nwork, const 
             if (dd_active(n, me)) v = pmin(v, dt_edge_d<1, T, FAST>(n, me, dx));
         }
     } else if (t < nwork) {
-        v = seam_dt_edge<T, FAST>(t - nband, s, m0, ny, nx, dx, sr, FO);
+        const long u = t - nband, ny_e = seam_y_edges(ny, nx, sr);
+        if (u < ny_e) {
+            v = seam_dt_y<T, FAST>(u, s, m0, nx, dx, sr);
+        } else {
+            const Frame f = frame_of(ny, nx);
+            v = seam_dt_x<T, FAST>(u - ny_e, xs, ny, dx, f.top, f.bot - f.top);
+        }
     }
     return v;
 }
@@ -544,14 +571,15 @@ __device__ __forceinline__ T band_dt_cell(long t, long nband, long nwork, const 
 // Work items of the frame dt kernel: the frame's cells, then the fused
 // stage's seam edges.
 static __host__ __device__ inline long band_dt_work(int ny, int nx, int sr) {
-    return frame_cells(ny, nx) + seam_edges(ny, nx, sr, FO);
+    return frame_cells(ny, nx) + fused_seam_edges(ny, nx, sr);
 }
 
 template <typename T, bool FAST>
-__global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__restrict__ m0, int ny, int nx, T dx,
-                                                     int sr, long nband, long nwork, T *__restrict__ partial) {
+__global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__restrict__ m0, const T *__restrict__ xs,
+                                                     int ny, int nx, T dx, int sr, long nband, long nwork,
+                                                     T *__restrict__ partial) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    block256_min_to(band_dt_cell<T, FAST>(t, nband, nwork, s, m0, ny, nx, dx, sr), partial + blockIdx.x);
+    block256_min_to(band_dt_cell<T, FAST>(t, nband, nwork, s, m0, xs, ny, nx, dx, sr), partial + blockIdx.x);
 }
 
 }  // namespace pxh

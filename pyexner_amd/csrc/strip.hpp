@@ -55,6 +55,28 @@ __device__ __forceinline__ void xcd_block(int &bx, int &by) {
     by = lb / gx;
 }
 
+// The seam-column buffer of a combine sweep: one record of 10 values per strip
+// column bx and row r (records of a strip column in row order): the combined
+// h, hu, hv, G of the strip's first owned column (side 0) at [0, 4), of its
+// last (side 1) at [4, 8), and their nodata flags (0 or 1) at 8 and 9.  The
+// edges between two strips' columns are reduced from it after the sweep.  A
+// row's values sit together, so the sweep's stores of one row fill one span
+// instead of ten scattered cache lines.
+constexpr int SEAM_REC = 10;
+template <typename T>
+__host__ __device__ __forceinline__ T *seam_rec(T *xs, int bx, int r, int ny) {
+    return xs + (long(bx) * ny + r) * SEAM_REC;
+}
+template <typename T>
+__device__ __forceinline__ void seam_store(T *xs, int bx, int r, int ny, int side, T h, T hu, T hv, T G, bool nod) {
+    T *q = seam_rec(xs, bx, r, ny);
+    q[4 * side] = h;
+    q[4 * side + 1] = hu;
+    q[4 * side + 2] = hv;
+    q[4 * side + 3] = G;
+    q[8 + side] = nod ? T(1) : T(0);
+}
+
 template <int SO, int SROWS>
 __device__ __forceinline__ StripPos strip_pos_at(int bx, int by, int ny, int nx) {
     StripPos p;
@@ -961,7 +983,8 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
                                                            const T *__restrict__ zb_lo_in, const T *__restrict__ G_in,
                                                            F<T> out, F<T> old, const uint8_t *__restrict__ m0,
                                                            const T *__restrict__ dtp, int ny, int nx, T dx,
-                                                           SedK sd, int write_G, T *__restrict__ partial) {
+                                                           SedK sd, int write_G, T *__restrict__ partial,
+                                                           T *__restrict__ xs) {
     __shared__ T red[SWPB];
     const StripPos p = strip_pos<SO_WIDE, SR_BED>(ny, nx);
     const T zero = T(0), hf = T(0.5);
@@ -1063,6 +1086,9 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
                     // of the column right of the last output lane belongs to a
                     // neighbouring strip, which may already have stored it over
                     // `old` (k_rx_seam_dt takes those edges)
+                    if (p.out && (p.lane == 1 || p.lane == SO_WIDE))
+                        seam_store(xs, (p.c + 1 - p.lane) / SO_WIDE, r, ny, p.lane != 1, cbc.h, cbc.hu, cbc.hv,
+                                   cbc.G, (cur.flags & 2) != 0);
                     DD<T> dnxt = dcur;
                     if (hasS) dnxt = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, nxt.flags & 2);
                     const DD<T> deast = dd_shdn(dcur);

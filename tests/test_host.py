@@ -450,3 +450,17 @@ def test_strip_height_flags_are_checked():
     assert sizes[0] == sizes[32] and sizes[8] > sizes[32]
     for rows in (4, 129, 255):
         assert lib.pxh_dt_workspace_size(_hip.Grid(400, 300, _hip.PXH_F32, rows << sh, 0.1)) == -1
+
+
+def test_dt_workspace_holds_the_seam_columns():
+    """The dt workspace ends with the combine sweeps' seam-column buffer: per
+    60-column strip and row, a record of the combined h, hu, hv, G and the
+    nodata flag of the strip's first and last owned column (10 values), in
+    the grid's dtype."""
+    lib = _hip.load()
+    ny, nx = 4000, 3000
+    f32 = lib.pxh_dt_workspace_size(_hip.Grid(ny, nx, _hip.PXH_F32, 0, 0.1))
+    f64 = lib.pxh_dt_workspace_size(_hip.Grid(ny, nx, _hip.PXH_F64, 0, 0.1))
+    cols = -(-nx // 60) * 2 * 5 * ny
+    assert f64 - f32 == cols * 4
+    assert f32 > cols * 4
