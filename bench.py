@@ -81,6 +81,8 @@ def parse():
     p.add_argument("--adj-steps", type=int, default=200,
                    help="steps of the gradient run (bed calibration, 1024x256 float64; 0: skip)")
     p.add_argument("--adj-every", type=int, default=20, help="checkpoint interval of the gradient run")
+    p.add_argument("--accuracy", type=int, default=1,
+                   help="also run the timed steps in float64 and report the float32 result's deviation (1) or not (0)")
     p.add_argument("--autotune", type=int, default=1,
                    help="choose the fused stage's strip height by timing the candidates (1) or keep 32 (0)")
     return p.parse_args()
@@ -222,6 +224,46 @@ def ensemble_run(a, torch):
             "strip_rows": rows, "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}}
 
 
+def accuracy_run(a, n, f32_end, torch):
+    """The timed run's float32 state against the same a.steps SSPRK2 steps in
+    float64 (which matches the float64 oracle to 1e-12): max-norm and L1
+    deviations, relative to max|field| (hv: max|hu|) and, for the bed, to the
+    bed change."""
+    from pyexner_amd.runtime.driver import setup_case
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = case(n)
+    state, bm, solver, sc, integ, ic = setup_case(mesh, arr, params, dtype=torch.float64)
+    del arr
+    mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
+    state = solver.init_fn(state, mask, sc)
+    zb0 = state.z_b.clone()
+    st = ExnerStepper(sc.backend, state, mask, params["cfl"])
+    del state
+    t0 = time.perf_counter()
+    st.compute_dt()
+    for _ in range(a.steps):
+        st.ssprk2_step()
+    torch.cuda.synchronize()
+    r = st.a
+    out = {"steps": a.steps, "reference": "float64 run of the same steps (fused stage)"}
+    hu_max = float(r.hu.abs().max())
+    for f in ("h", "hu", "hv"):
+        d = (f32_end[f].double() - getattr(r, f)).abs()
+        scale = hu_max if f == "hv" else float(getattr(r, f).abs().max())
+        out[f"{f}_max_rel"] = float(d.max()) / scale
+        out[f"{f}_l1_rel"] = float(d.sum() / (r.hu.abs().sum() if f == "hv" else getattr(r, f).abs().sum()))
+    bed = f32_end["z_b"].double() + (f32_end["z_b_lo"].double() if "z_b_lo" in f32_end else 0)
+    d = (bed - r.z_b).abs()
+    change = (r.z_b - zb0).abs()
+    out["bed_max_over_max_dz"] = float(d.max() / change.max())
+    out["bed_l1_over_l1_dz"] = float(d.sum() / change.sum())
+    out["bed_cells_over_1e-4_max_dz"] = int((d > 1e-4 * change.max()).sum())
+    log("float32 vs float64 after {} steps ({:.1f} s): ".format(a.steps, time.perf_counter() - t0)
+        + ", ".join(f"{k} {v:.2e}" if isinstance(v, float) else f"{k} {v}" for k, v in out.items()
+                    if k not in ("steps", "reference")))
+    return out
+
+
 def adjoint_run(a, torch):
     """Bed calibration: the gradient of 0.5 |z_b(T) - target|^2 with respect to
     z_b0, d50 and porosity over a.adj_steps SSPRK2 steps of the 1024x256
@@ -342,6 +384,10 @@ def main():
         raise RuntimeError("non-finite depth after the timed steps")
     t_end = float(st.dt)  # next dt (sanity)
     log(f"next dt {t_end:.4e} s")
+    f32_end = None
+    if a.accuracy:
+        f32_end = {f: getattr(st.a, f).clone() for f in ("h", "hu", "hv", "z_b", "z_b_lo")
+                   if getattr(st.a, f) is not None}
 
     # Per-kernel durations inside the step: SSPRK2 steps issued launch group by
     # launch group through the C-ABI (the same launches pxh_roe_exner_ssprk2
@@ -416,10 +462,15 @@ def main():
         tile = tiled_run(a, n, params, torch)
     ens = None
     if a.ens_members > 0 and a.ens_steps > 0:
-        del st
         torch.cuda.empty_cache()
         ens = ensemble_run(a, torch)
 
+    acc = None
+    if f32_end is not None:
+        del st
+        torch.cuda.empty_cache()
+        acc = accuracy_run(a, n, f32_end, torch)
+        del f32_end
     adj = None
     if a.adj_steps > 0:
         torch.cuda.empty_cache()
@@ -455,6 +506,7 @@ def main():
         "tiles_4x4": tile,
         "ensemble": ens,
         "adjoint": adj,
+        "accuracy_f32_vs_f64": acc,
     }
     print(json.dumps(out), flush=True)
 
