@@ -31,6 +31,7 @@ constexpr int FO = 60;          // owned columns per fused strip
 constexpr int FHW = 2;          // halo lanes on each side
 constexpr int SR_STAGE = PXH_SR_STAGE;  // owned rows per fused strip unless the grid's flags choose
 constexpr int FWPB = PXH_FWPB;  // fused strips (wavefronts) per block
+constexpr int SR_MAX = 128;     // the largest strip height a grid may choose
 
 struct StagePos {
     int lane, c, cc, r0, r1;
@@ -160,6 +161,14 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
     const T zero = T(0), hf = T(0.5);
     const int c = p.c;
     T vmin = inf_v<T>();
+    // MODE 1 with DT: this strip's seam records, staged in LDS row by row and
+    // stored as one span after the rows (whole cache lines, not one partial
+    // line per row)
+    T *srec = nullptr;
+    if constexpr (MODE == 1 && DT) {
+        __shared__ T seam_lds[FWPB * SR_MAX * SEAM_REC];
+        srec = seam_lds + __builtin_amdgcn_readfirstlane(threadIdx.x / SW) * (SR_MAX * SEAM_REC);
+    }
     if (p.r0 < ny) {
         const T dt = *dtp;
         const T dtdx = dt / dx;
@@ -282,8 +291,8 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                         // `old`.  The frame dt kernel takes the seam edges.
                         const DD<T> dcur = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, edp.flags & 2);
                         if (p.out && colI && (p.lane == FHW || p.lane == FHW + FO - 1))
-                            seam_store(xs, p.bx, rb, ny, p.lane != FHW, cbn.h, cbn.hu, cbn.hv, cbn.G,
-                                       (edp.flags & 2) != 0);
+                            seam_put(srec + (rb - p.r0) * SEAM_REC, p.lane != FHW, cbn.h, cbn.hu, cbn.hv, cbn.G,
+                                     (edp.flags & 2) != 0);
                         const DD<T> deast = dd_shdn(dcur);
                         if (p.out && eown && colIE && dd_active(dcur, deast))
                             vmin = pmin(vmin, dt_edge_d<0, T, FAST>(dcur, deast, dx));
@@ -296,6 +305,18 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                 fn = fy;
             }
             edp = edk;
+        }
+    }
+    if constexpr (MODE == 1 && DT) {
+        __syncthreads();  // every wave of the block reaches it
+        const int rlo = p.r0 > fr.top ? p.r0 : fr.top, rhi = p.r1 < fr.bot ? p.r1 : fr.bot;
+        if (p.r0 < ny && rhi > rlo) {
+            // rows [rlo, rhi) hold the records of the interior owned rows; a
+            // side whose column is not interior holds no record and is never read
+            const int n = (rhi - rlo) * SEAM_REC;
+            T *dst = seam_rec(xs, p.bx, rlo, ny);
+            const T *src = srec + (rlo - p.r0) * SEAM_REC;
+            for (int i = p.lane; i < n; i += SW) dst[i] = src[i];
         }
     }
     return vmin;

@@ -67,9 +67,9 @@ template <typename T>
 __host__ __device__ __forceinline__ T *seam_rec(T *xs, int bx, int r, int ny) {
     return xs + (long(bx) * ny + r) * SEAM_REC;
 }
+// One side of seam record q.
 template <typename T>
-__device__ __forceinline__ void seam_store(T *xs, int bx, int r, int ny, int side, T h, T hu, T hv, T G, bool nod) {
-    T *q = seam_rec(xs, bx, r, ny);
+__device__ __forceinline__ void seam_put(T *q, int side, T h, T hu, T hv, T G, bool nod) {
     q[4 * side] = h;
     q[4 * side + 1] = hu;
     q[4 * side + 2] = hv;
@@ -1087,8 +1087,8 @@ __global__ __launch_bounds__(SW *SWPB) void k_rx_bed_strip(F<T> hyd, const T *__
                     // neighbouring strip, which may already have stored it over
                     // `old` (k_rx_seam_dt takes those edges)
                     if (p.out && (p.lane == 1 || p.lane == SO_WIDE))
-                        seam_store(xs, (p.c + 1 - p.lane) / SO_WIDE, r, ny, p.lane != 1, cbc.h, cbc.hu, cbc.hv,
-                                   cbc.G, (cur.flags & 2) != 0);
+                        seam_put(seam_rec(xs, (p.c + 1 - p.lane) / SO_WIDE, r, ny), p.lane != 1, cbc.h, cbc.hu,
+                                 cbc.hv, cbc.G, (cur.flags & 2) != 0);
                     DD<T> dnxt = dcur;
                     if (hasS) dnxt = dd_make<T, FAST>(cbn.h, cbn.hu, cbn.hv, cbn.G, nxt.flags & 2);
                     const DD<T> deast = dd_shdn(dcur);
