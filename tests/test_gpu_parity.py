@@ -374,6 +374,31 @@ SEAM_CASES = [
 ]
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_float32_runs_are_reproducible(cuda, fused):
+    """Two float32 runs of the same case give the same state and dt bit for
+    bit: no sweep reads a value another block may or may not have written yet
+    (the combine sweeps store over the step input in place).  A radial dam
+    break on a grid of many strips in both directions (16-row fused strips)."""
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = cases.radial_dambreak(nx=700, ny=300, dx=0.05, n_bumps=24, radius=6.0)
+    params = dict(params, backend={"fused_stage": fused})
+    runs = []
+    for _ in range(2):
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+        sc.backend.set_strip_rows(16)
+        st = ExnerStepper(sc.backend, sg, mask, params["cfl"])
+        dts = []
+        for _ in range(40):
+            st.ssprk2_step()
+            dts.append(st.dt.clone())
+        torch.cuda.synchronize()
+        runs.append((torch.stack(dts), {f: getattr(st.a, f).clone() for f in FIELDS_EXNER}))
+    assert torch.equal(runs[0][0], runs[1][0])
+    for f in FIELDS_EXNER:
+        assert torch.equal(runs[0][1][f], runs[1][1][f]), f
+
+
 @pytest.mark.parametrize("fused,rows,pair,axis", SEAM_CASES)
 def test_next_dt_across_strip_seams_f64(cuda, fused, rows, pair, axis):
     """The next dt an SSPRK2 step leaves equals compute_dt_fn of the new state
