@@ -20,6 +20,7 @@
  *   BoundaryManager.apply  domain/boundary_registry.py:183-187 -> pxh_bc_apply
  *   SSPRK2 combine          integrators/SSPRK2.py:40            -> pxh_ssprk2_combine
  *   make_halo_exchange      solvers/kernels/roe_exner.py:719-803 -> pxh_halo_pack / pxh_halo_unpack
+ *                                                                   (pxh_copy_batch: tiles on one device)
  *   (no counterpart: reverse mode of step_fn / compute_dt_fn / init_fn) -> pxh_adj_roe_exner_*
  */
 #ifndef PYEXNER_HIP_H
@@ -383,6 +384,25 @@ int pxh_halo_pack(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t gr
                   void *stream);
 int pxh_halo_unpack(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t groups, int sides,
                     const void *buf, void *stream);
+
+/* Batched strip copies on one device: the exchange of the tiles of a
+ * decomposition that share one GPU (make_halo_exchange, roe_exner.py:719-803,
+ * has no such form: its ranks send packed strips).  Each ghost strip is copied
+ * straight from the neighbour's owned strip, every strip of an exchange phase
+ * in one launch, in place of a pack and an unpack per side of every tile.
+ * jobs: device memory, n entries; job k copies rows x cols elements of dtype
+ * from src (row stride src_ld elements) to dst (row stride dst_ld); start is
+ * the sum of rows * cols over the jobs before it (job 0: 0) and total the sum
+ * over all n.  No destination may overlap a source of the same call.  The
+ * table lives on the device, so only n, total and dtype are checked here. */
+typedef struct {
+    const void *src;
+    void *dst;
+    int64_t src_ld, dst_ld;
+    int64_t start;
+    int32_t rows, cols;
+} pxh_copy_job;
+int pxh_copy_batch(int32_t dtype, const pxh_copy_job *jobs, int32_t n, int64_t total, void *stream);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,12 @@ class BCHandler(ctypes.Structure):
                 ("ny", ctypes.c_double)]
 
 
+class CopyJob(ctypes.Structure):     # pxh_copy_job
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("src_ld", ctypes.c_int64),
+                ("dst_ld", ctypes.c_int64), ("start", ctypes.c_int64), ("rows", ctypes.c_int32),
+                ("cols", ctypes.c_int32)]
+
+
 class BCValue(ctypes.Structure):     # pxh_bc_value
     _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("a", ctypes.c_double),
                 ("b", ctypes.c_double), ("c", ctypes.c_double)]
@@ -120,6 +126,8 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_halo_unpack": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_copy_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
+                                      ctypes.c_void_p]),
     "pxh_ensemble_plan": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, _P(Member), ctypes.c_int32,
                                            ctypes.c_void_p, ctypes.c_int64]),
     "pxh_ensemble_workspace_size": (ctypes.c_int64, [ctypes.c_void_p]),

@@ -78,4 +78,22 @@ __global__ __launch_bounds__(256) void k_halo(HaloFields<T> f, HaloPlan plan, T 
     }
 }
 
+// Batched rectangle copies (pxh_copy_batch): element t belongs to the last job
+// whose start is <= t (jobs with no elements share the next job's start).
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_batch(const pxh_copy_job *__restrict__ jobs, int n, long total) {
+    for (long t = long(blockIdx.x) * 256 + threadIdx.x; t < total; t += long(gridDim.x) * 256) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (jobs[mid].start <= t) lo = mid;
+            else hi = mid;
+        }
+        const pxh_copy_job &j = jobs[lo];
+        const long e = t - j.start;
+        const long r = e / j.cols, c = e - r * j.cols;
+        static_cast<T *>(j.dst)[r * j.dst_ld + c] = static_cast<const T *>(j.src)[r * j.src_ld + c];
+    }
+}
+
 }  // namespace pxh

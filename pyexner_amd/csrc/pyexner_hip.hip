@@ -1160,6 +1160,17 @@ int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const px
                     old ? dt_partial : nullptr, static_cast<hipStream_t>(stream));
 }
 
+int pxh_copy_batch(int32_t dtype, const pxh_copy_job *jobs, int32_t n, int64_t total, void *stream) {
+    if ((dtype != PXH_F32 && dtype != PXH_F64) || n < 0 || total < 0 || (n > 0 && !jobs) || (total > 0 && n == 0))
+        return PXH_E_ARG;
+    if (total == 0) return PXH_OK;
+    const unsigned nb = unsigned(total / 256 + 1 < 4096 ? total / 256 + 1 : 4096);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (dtype == PXH_F64) hipLaunchKernelGGL(k_copy_batch<double>, dim3(nb), dim3(256), 0, st, jobs, int(n), long(total));
+    else hipLaunchKernelGGL(k_copy_batch<float>, dim3(nb), dim3(256), 0, st, jobs, int(n), long(total));
+    return launch_status();
+}
+
 int64_t pxh_dt_workspace_size(const pxh_grid *g) {
     if (!grid_ok(g)) return PXH_E_ARG;
     const int64_t esz = g->dtype == PXH_F64 ? 8 : 4;

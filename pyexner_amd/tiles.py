@@ -50,6 +50,9 @@ _OPPOSITE = {_hip.HALO_W: _hip.HALO_E, _hip.HALO_E: _hip.HALO_W, _hip.HALO_N: _h
 _STEP = (1 << _hip.G_H) | (1 << _hip.G_HUV) | (1 << _hip.G_ZB) | (1 << _hip.G_G)
 _HYDRO = (1 << _hip.G_H) | (1 << _hip.G_HUV)
 _BED = (1 << _hip.G_ZB) | (1 << _hip.G_G)
+# the fields of each group in the order pxh_halo_pack takes them
+_GROUP_FIELDS = ((_hip.G_H, ("h",)), (_hip.G_HUV, ("hu", "hv")), (_hip.G_Z, ("z",)), (_hip.G_ZB, ("z_b", "z_b_lo")),
+                 (_hip.G_N, ("n",)), (_hip.G_NB, ("n_b",)), (_hip.G_G, ("G",)))
 
 
 def split(n, parts):
@@ -152,9 +155,10 @@ class TiledExner:
     """Roe-Exner SSPRK2 on a par = (parNy, parNx) decomposition of an
     initialised single-domain state (init_fn applied), every tile on this device."""
 
-    def __init__(self, backend, state, mask, boundaries, par, cfl, fused=None):
+    def __init__(self, backend, state, mask, boundaries, par, cfl, fused=None, direct=True):
         """fused: the fused-stage path (three ghost rings); default when the
-        ring fill is frame-local."""
+        ring fill is frame-local.  direct: exchange ghost strips by batched
+        device copies (default) rather than pack/unpack per side."""
         self.be = backend
         self.cfl = float(cfl)
         self.par = tuple(par)
@@ -177,16 +181,75 @@ class TiledExner:
                                      self.cols, device, values, self.hw) for ty in range(pny) for tx in range(pnx)}
         self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
         self.dt_ready = False
+        self.direct = bool(direct)
+        self._jobs = {}
 
     def _neighbour(self, pos, side):
         ty, tx = pos
         d = {_hip.HALO_W: (0, -1), _hip.HALO_E: (0, 1), _hip.HALO_N: (-1, 0), _hip.HALO_S: (1, 0)}[side]
         return self.tiles[(ty + d[0], tx + d[1])]
 
+    def _copy_jobs(self, which, groups, phase):
+        """The strip copies of one exchange phase as a device table of
+        pxh_copy_job (built once per buffer set, groups and phase): every
+        ghost strip of the phase from its neighbour's owned strip, in the
+        order and with the depths pxh_halo_pack/unpack use."""
+        key = (which, groups, phase)
+        if key in self._jobs:
+            return self._jobs[key]
+        names = [n for bit, ns in _GROUP_FIELDS if groups & (1 << bit) for n in ns]
+        esz = 8 if self.be.dtype == torch.float64 else 4
+        jobs = []
+        for t in self.tiles.values():
+            ft = getattr(t, which)
+            ny, nx = ft.h.shape
+            hw = t.hw
+            for side in phase:
+                if not t.ring[side]:
+                    continue
+                n = self._neighbour(t.pos, side)
+                fn = getattr(n, which)
+                my, mx = fn.h.shape
+                # (dst row, dst col, src row, src col, rows, cols)
+                rect = {_hip.HALO_W: (0, 0, 0, mx - 2 * hw, ny, hw),
+                        _hip.HALO_E: (0, nx - hw, 0, hw, ny, hw),
+                        _hip.HALO_N: (0, 0, my - 2 * hw, 0, hw, nx),
+                        _hip.HALO_S: (ny - hw, 0, hw, 0, hw, nx)}[side]
+                dr, dc, sr, sc, rows, cols = rect
+                if (side in (_hip.HALO_W, _hip.HALO_E) and my != ny) or \
+                        (side in (_hip.HALO_N, _hip.HALO_S) and mx != nx):
+                    raise ValueError("neighbouring tiles disagree on the shared strip's length")
+                for name in names:
+                    a, b = getattr(ft, name), getattr(fn, name)
+                    if a is None or b is None:
+                        continue
+                    jobs.append((b.data_ptr() + (sr * mx + sc) * esz, a.data_ptr() + (dr * nx + dc) * esz,
+                                 mx, nx, rows, cols))
+        table = (_hip.CopyJob * max(len(jobs), 1))()
+        start = 0
+        for k, (src, dst, sld, dld, rows, cols) in enumerate(jobs):
+            table[k].src, table[k].dst, table[k].src_ld, table[k].dst_ld = src, dst, sld, dld
+            table[k].start, table[k].rows, table[k].cols = start, rows, cols
+            start += rows * cols
+        raw = np.frombuffer(bytes(table), dtype=np.uint8).copy()
+        entry = (torch.from_numpy(raw).to(self.tiles[(0, 0)].a.h.device), len(jobs), start)
+        self._jobs[key] = entry
+        return entry
+
     def exchange(self, which, groups):
         """Swap the ghost strips of buffer set `which` ("a", "b", "c") for the
-        field groups `groups`: west/east first, then north/south (corners)."""
+        field groups `groups`: west/east first, then north/south (corners).
+        direct (default): one batched copy per phase straight from the
+        neighbours' owned strips; else a pack and an unpack per side, the
+        form whose packed strips would travel between devices."""
         st = torch.cuda.current_stream().cuda_stream
+        if self.direct:
+            dtype = _hip.PXH_F64 if self.be.dtype == torch.float64 else _hip.PXH_F32
+            for phase in ((_hip.HALO_W, _hip.HALO_E), (_hip.HALO_N, _hip.HALO_S)):
+                table, n, total = self._copy_jobs(which, groups, phase)
+                if n:
+                    _hip.call("pxh_copy_batch", dtype, table.data_ptr(), n, total, st)
+            return
         for phase in ((_hip.HALO_W, _hip.HALO_E), (_hip.HALO_N, _hip.HALO_S)):
             for t in self.tiles.values():
                 for side in phase:

@@ -166,3 +166,35 @@ def test_tiled_graph_replays_equal_eager_steps(cuda, fused):
         assert torch.equal(graphed.dt, eager.dt), k
         for f in FIELDS_EXNER:
             assert torch.equal(graphed.gather(f), eager.gather(f)), (k, f)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("fused", [True, False])
+def test_direct_exchange_equals_pack_and_unpack(cuda, dtype, fused):
+    """The batched direct exchange (pxh_copy_batch, one launch per phase)
+    leaves every buffer set of every tile exactly as the pack/unpack exchange
+    does, for each group set the steps swap, on tiles of uneven size filled
+    with seeded random values (ghost strips included)."""
+    mesh, arr, params = cases.channel(nx=71, ny=45, n_bumps=8, seed=21)
+    params = dict(params, backend={"fused_stage": fused})
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
+    made = [TiledExner(sc.backend, sg, mask, sc.boundaries, (3, 2), 0.5, fused=fused, direct=d)
+            for d in (True, False)]
+    g = torch.Generator().manual_seed(5)
+    for which in ("a", "b", "c"):
+        for groups in (GROUPS, (1 << _hip.G_H) | (1 << _hip.G_HUV), (1 << _hip.G_ZB) | (1 << _hip.G_G)):
+            for pos, t in made[0].tiles.items():
+                for f in FIELD_ORDER:
+                    x = getattr(getattr(t, which), f)
+                    if x is None:
+                        continue
+                    x.copy_(torch.rand(x.shape, generator=g, dtype=torch.float64).to(dtype))
+                    getattr(getattr(made[1].tiles[pos], which), f).copy_(x)
+            for tl in made:
+                tl.exchange(which, groups)
+            torch.cuda.synchronize()
+            for pos, t in made[0].tiles.items():
+                for f in FIELD_ORDER:
+                    x = getattr(getattr(t, which), f)
+                    if x is not None:
+                        assert torch.equal(x, getattr(getattr(made[1].tiles[pos], which), f)), (which, groups, pos, f)

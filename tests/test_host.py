@@ -1,6 +1,7 @@
 """Host-side logic (no GPU): the C-ABI library's exports, the composed ring
 fill against the oracle's sequential handlers, the sediment table, polygon
 rasterisation and the input front end."""
+import ctypes
 import os
 import re
 import subprocess
@@ -464,3 +465,17 @@ def test_dt_workspace_holds_the_seam_columns():
     cols = -(-nx // 60) * 2 * 5 * ny
     assert f64 - f32 == cols * 4
     assert f32 > cols * 4
+
+
+def test_copy_batch_checks_its_arguments():
+    """pxh_copy_batch (batched tile strips on one device) rejects a bad dtype,
+    a negative count or total, a missing table and elements without jobs, and
+    does nothing for an empty batch (host-only checks: no job runs)."""
+    lib = _hip.load()
+    assert lib.pxh_copy_batch(7, None, 0, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_copy_batch(_hip.PXH_F32, None, -1, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_copy_batch(_hip.PXH_F32, None, 1, 10, None) == _hip.PXH_E_ARG
+    assert lib.pxh_copy_batch(_hip.PXH_F32, None, 0, 10, None) == _hip.PXH_E_ARG
+    assert lib.pxh_copy_batch(_hip.PXH_F64, None, 0, -1, None) == _hip.PXH_E_ARG
+    assert lib.pxh_copy_batch(_hip.PXH_F64, None, 0, 0, None) == _hip.PXH_OK
+    assert ctypes.sizeof(_hip.CopyJob) == 48
