@@ -119,8 +119,10 @@ def cpu_baseline(n, steps):
 def tiled_run(a, n, params, torch):
     """The same domain as a 4 x 4 decomposition of (n/4)^2 owned tiles, all on
     this GPU: the fused stage per tile with three ghost rings per shared side,
-    halo strips packed and unpacked on the device once per stage, exchanges as
-    device copies, one hipGraph per step.  Returns its throughput (not the
+    one exchange per stage (ghost strips copied straight from the neighbours'
+    owned strips, one launch per phase), one hipGraph per step; with
+    --autotune the tiles' strip height is timed first (on the same case,
+    restarted from the initial state).  Returns its throughput (not the
     headline metric)."""
     from pyexner_amd.runtime.driver import setup_case
     from pyexner_amd.tiles import TiledExner
@@ -131,7 +133,17 @@ def tiled_run(a, n, params, torch):
     del arr
     mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
     state = solver.init_fn(state, mask, sc)
+    rows, tune_ms = None, {}
+    if a.autotune:
+        from pyexner_amd.autotune import tune_tile_strip_rows
+        probe = TiledExner(sc.backend, state, mask, bm, (4, 4), params["cfl"])
+        rows, tune_ms = tune_tile_strip_rows(probe)
+        del probe
+        torch.cuda.empty_cache()
+        log("tile strip rows: " + ", ".join(f"{r}: {t:.3f} ms" for r, t in tune_ms.items()) + f" -> {rows}")
     tl = TiledExner(sc.backend, state, mask, bm, (4, 4), params["cfl"])
+    if rows:
+        tl.set_strip_rows(rows)
     del state
     tl.capture()
     torch.cuda.synchronize()
@@ -149,7 +161,8 @@ def tiled_run(a, n, params, torch):
     t = tl.tiles[(1, 1)]
     return {"ms_per_step": ms, "cell_steps_per_s": n * n / (ms * 1e-3), "steps": a.tile_steps,
             "tile_owned": [n // 4, n // 4], "tile_array": [t.grid.ny, t.grid.nx], "ghost_rings": tl.hw,
-            "halo_bytes_per_step": hb,
+            "halo_bytes_per_step": hb, "strip_rows": rows or 32,
+            "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()},
             "path": ("fused stage per tile, one exchange of every evolving field per stage, dt over tiles after it"
                      if tl.fused else "two-sweep stage, exchange after each sweep, dt reduced over tiles")}
 

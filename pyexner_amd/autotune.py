@@ -95,3 +95,26 @@ def strip_rows_of(flags):
     """The strip height a grid's flags select (PXH_STRIP_ROWS; 0 = 32)."""
     r = (int(flags) >> _hip.STRIP_ROWS_SHIFT) & 0xFF
     return r or 32
+
+
+def tune_tile_strip_rows(tiled, candidates=(8, 12, 16, 24, 32), steps=3):
+    """Choose the fused stage's strip height for the tiles of a TiledExner by
+    timing `steps` SSPRK2 steps per candidate (after one warm-up step) and
+    apply it.  The tiles advance while they are timed: tune before the run
+    whose result counts, on a copy, or accept the extra steps.  Returns
+    (rows, {rows: ms per step})."""
+    times = {}
+    for rows in candidates:
+        tiled.set_strip_rows(rows)
+        tiled.ssprk2_step()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            tiled.ssprk2_step()
+        e1.record()
+        torch.cuda.synchronize()
+        times[rows] = e0.elapsed_time(e1) / steps
+    best = min(times, key=times.get)
+    tiled.set_strip_rows(best)
+    return best, times

@@ -4,11 +4,13 @@ The reference decomposes a domain over MPI ranks (`parallel/mpi_handler.py`),
 gives every rank a tile with a one-cell ghost ring, and swaps the ring with
 `make_halo_exchange` (solvers/kernels/roe_exner.py:719-803) inside the step
 (solvers/roe_exner_solver.py:92-110).  Here each tile keeps its owned block
-plus two ghost rings on every side it shares with another tile (none on the
-sides of the physical boundary), the strips are packed and unpacked on the
-device with `pxh_halo_pack` / `pxh_halo_unpack`, and the sendrecv between
-tiles is a device copy (on a multi-GPU machine it would be the RCCL
-send/recv of the same buffers).
+plus two or three ghost rings on every side it shares with another tile
+(none on the sides of the physical boundary).  Tiles on one device swap
+ghost strips by batched device copies straight from the neighbours' owned
+strips (`pxh_copy_batch`, one launch per exchange phase); with
+direct=False the strips are packed and unpacked per side with
+`pxh_halo_pack` / `pxh_halo_unpack`, the buffers a multi-GPU run would
+send and receive over RCCL.
 
 An owned cell's stage result reads the stage input up to three cells away:
 its bed update reads its neighbours' updated hydro, whose momentum
@@ -183,6 +185,19 @@ class TiledExner:
         self.dt_ready = False
         self.direct = bool(direct)
         self._jobs = {}
+
+    def set_strip_rows(self, rows):
+        """Owned rows per strip of every tile's fused stage sweep (8..128):
+        a tile is a small grid, and shorter strips give its sweep enough
+        wavefronts to fill the GPU.  Results do not depend on it."""
+        lib = _hip.load()
+        for t in self.tiles.values():
+            t.grid.flags = (int(t.grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (int(rows) << _hip.STRIP_ROWS_SHIFT)
+            ws = lib.pxh_dt_workspace_size(t.grid)
+            if ws < 0:
+                raise ValueError(f"strip height {rows} is outside 8..128")
+            if ws > t.workspace.numel():
+                t.workspace = torch.empty(int(ws), dtype=torch.uint8, device=t.workspace.device)
 
     def _neighbour(self, pos, side):
         ty, tx = pos

@@ -198,3 +198,23 @@ def test_direct_exchange_equals_pack_and_unpack(cuda, dtype, fused):
                     x = getattr(getattr(t, which), f)
                     if x is not None:
                         assert torch.equal(x, getattr(getattr(made[1].tiles[pos], which), f)), (which, groups, pos, f)
+
+
+def test_tile_strip_height_changes_no_result(cuda):
+    """The tiles' fused strip height (TiledExner.set_strip_rows, what the
+    autotuner sets) moves work between wavefronts only: float32 tiled runs
+    with 8-row and default strips are equal bit for bit."""
+    mesh, arr, params = cases.radial_dambreak(nx=130, ny=140, dx=0.05, n_bumps=8, radius=2.0)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+    runs = []
+    for rows in (None, 8):
+        tl = TiledExner(sc.backend, sg, mask, sc.boundaries, (2, 2), 0.5)
+        if rows:
+            tl.set_strip_rows(rows)
+        for _ in range(20):
+            tl.ssprk2_step()
+        torch.cuda.synchronize()
+        runs.append((tl.dt.clone(), {f: tl.gather(f) for f in FIELDS_EXNER}))
+    assert torch.equal(runs[0][0], runs[1][0])
+    for f in FIELDS_EXNER:
+        assert torch.equal(runs[0][1][f], runs[1][1][f]), f
