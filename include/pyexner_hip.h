@@ -395,6 +395,27 @@ int pxh_halo_unpack(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t 
  * the sum of rows * cols over the jobs before it (job 0: 0) and total the sum
  * over all n.  No destination may overlap a source of the same call.  The
  * table lives on the device, so only n, total and dtype are checked here. */
+/* compute_dt_fn over several domains in one sweep launch and one reduction:
+ * the tiles of a decomposition on one device (compute_dt_roeexner,
+ * roe_exner_solver.py:118-122, whose MIN allreduce over ranks becomes the
+ * minimum over the domains).  *dt_out = max(cfl * min over the active edges of
+ * every domain, 1e-6).  doms: device memory, n entries; domain k holds the
+ * fields h, hu, hv, G and the nodata mask of a ny x nx grid with spacing dx,
+ * and its sweep blocks start at blk0 = the sum of pxh_dt_multi_blocks over the
+ * domains before it; nblocks is the sum over all n, and partial holds nblocks
+ * * 8 bytes.  flags: PXH_EXACT_F32 as in pxh_grid.flags. */
+typedef struct {
+    const void *h, *hu, *hv, *G;
+    const uint8_t *mask0;
+    int32_t ny, nx;
+    int32_t blk0;
+    int32_t reserved;
+    double dx;
+} pxh_dt_domain;
+int64_t pxh_dt_multi_blocks(int32_t ny, int32_t nx);
+int pxh_dt_multi(int32_t dtype, int32_t flags, const pxh_dt_domain *doms, int32_t n, int32_t nblocks, double cfl,
+                 void *dt_out, void *partial, void *stream);
+
 typedef struct {
     const void *src;
     void *dst;

@@ -68,6 +68,12 @@ class BCHandler(ctypes.Structure):
                 ("ny", ctypes.c_double)]
 
 
+class DtDomain(ctypes.Structure):    # pxh_dt_domain
+    _fields_ = [("h", ctypes.c_void_p), ("hu", ctypes.c_void_p), ("hv", ctypes.c_void_p), ("G", ctypes.c_void_p),
+                ("mask0", ctypes.c_void_p), ("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("blk0", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("dx", ctypes.c_double)]
+
+
 class CopyJob(ctypes.Structure):     # pxh_copy_job
     _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("src_ld", ctypes.c_int64),
                 ("dst_ld", ctypes.c_int64), ("start", ctypes.c_int64), ("rows", ctypes.c_int32),
@@ -126,6 +132,9 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_halo_unpack": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_dt_multi_blocks": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "pxh_dt_multi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_copy_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
                                       ctypes.c_void_p]),
     "pxh_ensemble_plan": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, _P(Member), ctypes.c_int32,

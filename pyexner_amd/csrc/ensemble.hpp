@@ -118,6 +118,36 @@ __global__ __launch_bounds__(SW *SWPB) void k_ens_dt_strip(F<T> s, const uint8_t
     if (threadIdx.x == 0) partial[m.part0 + local] = v;
 }
 
+// The standalone dt sweep over several domains of their own allocations
+// (pxh_dt_multi: the tiles of one device), one partial per block; domain k's
+// blocks start at doms[k].blk0.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(SW *SWPB) void k_multi_dt_strip(const pxh_dt_domain *__restrict__ doms, int n,
+                                                             T *__restrict__ partial) {
+    __shared__ T red[SWPB];
+    int lb, unused;
+    xcd_block(lb, unused);  // one-dimensional grid
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (doms[mid].blk0 <= lb) lo = mid;
+        else hi = mid;
+    }
+    const pxh_dt_domain d = doms[__builtin_amdgcn_readfirstlane(lo)];
+    const int gx = (d.nx + SO_WIDE - 1) / SO_WIDE;
+    const int local = lb - d.blk0;
+    const int by = local / gx, bx = local - by * gx;
+    const StripPos p = strip_pos_at<SO_WIDE, SR_BED>(bx, by, d.ny, d.nx);
+    F<T> s{};
+    s.h = static_cast<T *>(const_cast<void *>(d.h));
+    s.hu = static_cast<T *>(const_cast<void *>(d.hu));
+    s.hv = static_cast<T *>(const_cast<void *>(d.hv));
+    s.G = static_cast<T *>(const_cast<void *>(d.G));
+    T v = dt_strip_run<T, FAST>(p, s, d.mask0, d.ny, d.nx, static_cast<T>(d.dx));
+    v = wave_block_min(v, red);
+    if (threadIdx.x == 0) partial[lb] = v;
+}
+
 // One block per member: dts[i] = max(cfl * min of its minima, 1e-6)
 // (compute_dt_roeexner, roe_exner_solver.py:118-122).  step: the minima of a
 // fused SSPRK2 step (stage + frame/seam blocks); else of the standalone dt sweep.

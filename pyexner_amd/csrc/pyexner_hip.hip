@@ -1160,6 +1160,38 @@ int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const px
                     old ? dt_partial : nullptr, static_cast<hipStream_t>(stream));
 }
 
+int64_t pxh_dt_multi_blocks(int32_t ny, int32_t nx) {
+    if (ny < 2 || nx < 2) return PXH_E_ARG;
+    const dim3 sg = strip_grid<SO_WIDE, SR_BED>(ny, nx);
+    return int64_t(sg.x) * sg.y;
+}
+
+int pxh_dt_multi(int32_t dtype, int32_t flags, const pxh_dt_domain *doms, int32_t n, int32_t nblocks, double cfl,
+                 void *dt_out, void *partial, void *stream) {
+    if ((dtype != PXH_F32 && dtype != PXH_F64) || n <= 0 || nblocks <= 0 || !doms || !dt_out || !partial)
+        return PXH_E_ARG;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const dim3 sb(SW * SWPB);
+    if (dtype == PXH_F64) {
+        hipLaunchKernelGGL((k_multi_dt_strip<double, false>), dim3(nblocks), sb, 0, st, doms, int(n),
+                           static_cast<double *>(partial));
+        if (int e = launch_status()) return e;
+        hipLaunchKernelGGL(k_dt_final<double>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const double *>(partial),
+                           int(nblocks), cfl, 1, static_cast<double *>(dt_out));
+    } else {
+        if (flags & PXH_EXACT_F32)
+            hipLaunchKernelGGL((k_multi_dt_strip<float, false>), dim3(nblocks), sb, 0, st, doms, int(n),
+                               static_cast<float *>(partial));
+        else
+            hipLaunchKernelGGL((k_multi_dt_strip<float, true>), dim3(nblocks), sb, 0, st, doms, int(n),
+                               static_cast<float *>(partial));
+        if (int e = launch_status()) return e;
+        hipLaunchKernelGGL(k_dt_final<float>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const float *>(partial),
+                           int(nblocks), static_cast<float>(cfl), 1, static_cast<float *>(dt_out));
+    }
+    return launch_status();
+}
+
 int pxh_copy_batch(int32_t dtype, const pxh_copy_job *jobs, int32_t n, int64_t total, void *stream) {
     if ((dtype != PXH_F32 && dtype != PXH_F64) || n < 0 || total < 0 || (n > 0 && !jobs) || (total > 0 && n == 0))
         return PXH_E_ARG;

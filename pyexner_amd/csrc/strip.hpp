@@ -326,7 +326,7 @@ template <int AX>
 __device__ __forceinline__ void rx_upwind_coef(float l1, float l2, float l3, float lE1, float lE3, float a1,
                                                float a2, float a3, float b1, float vt, float ct, float P[3],
                                                float M[3]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float zero = 0.0f;
     const float f1 = l1 * a1 - b1, f2 = l2 * a2, f3 = l3 * a3 + b1;
     const float e1 = lE1 * a1, e3 = lE3 * a3;
@@ -352,7 +352,7 @@ __device__ __forceinline__ void rx_upwind_coef(float l1, float l2, float l3, flo
 template <int AX>
 __device__ __forceinline__ void rx_edge_fast(const HD<float> &I, const HD<float> &J, float dx, float P[3],
                                              float M[3]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     using Mf = Mth<float, true>;
     const float T_ = float(DRY_TOL), g = float(GRAV), V = float(VEL_TOL), zero = 0.0f;
     const float hi = I.hf, hj = J.hf;
@@ -881,7 +881,7 @@ __device__ __forceinline__ bool dd_active(const DD<T> &a, const DD<T> &b) {
 // it returns false and the caller evaluates the double form; below, max|lambda|
 // stays within 3e-6 of the double form.
 __device__ __forceinline__ bool get_lambda_f32(float ut, float at, float ct, float &lmin, float &lmax) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     using Mf = Mth<float, true>;
     const float ct2 = ct * ct;
     const float P = (1.0f + at) * ct2 + ut * ut * 0.33333333333333333f;

@@ -185,6 +185,7 @@ class TiledExner:
         self.dt_ready = False
         self.direct = bool(direct)
         self._jobs = {}
+        self._dt = None
 
     def set_strip_rows(self, rows):
         """Owned rows per strip of every tile's fused stage sweep (8..128):
@@ -278,11 +279,36 @@ class TiledExner:
                         _hip.call("pxh_halo_unpack", t.grid, t.fields(which), t.hw, groups, side, src.data_ptr(),
                                   st)
 
+    def _dt_table(self):
+        """The tiles' buffer set a as a device table of pxh_dt_domain (built
+        once) and the partial minima buffer of pxh_dt_multi."""
+        if self._dt is None:
+            lib = _hip.load()
+            tiles = list(self.tiles.values())
+            table = (_hip.DtDomain * len(tiles))()
+            blk = 0
+            for k, t in enumerate(tiles):
+                d = table[k]
+                d.h, d.hu, d.hv, d.G = (getattr(t.a, f).data_ptr() for f in ("h", "hu", "hv", "G"))
+                d.mask0, d.ny, d.nx, d.blk0, d.dx = t.m0, t.grid.ny, t.grid.nx, blk, t.grid.dx
+                nb = lib.pxh_dt_multi_blocks(t.grid.ny, t.grid.nx)
+                if nb <= 0:
+                    raise ValueError("invalid tile geometry")
+                blk += nb
+            device = self.dt.device
+            raw = np.frombuffer(bytes(table), dtype=np.uint8).copy()
+            self._dt = (torch.from_numpy(raw).to(device), len(tiles), blk,
+                        torch.empty(blk * 8, dtype=torch.uint8, device=device))
+        return self._dt
+
     def compute_dt(self):
+        """compute_dt_fn over every tile (the MIN allreduce): one sweep launch
+        over all tiles and one reduction (pxh_dt_multi)."""
         st = torch.cuda.current_stream().cuda_stream
-        for t in self.tiles.values():
-            _hip.call("pxh_roe_exner_dt", t.grid, t.fa, t.m0, self.cfl, t.dt.data_ptr(), t.workspace.data_ptr(), st)
-        self.dt.copy_(torch.stack([t.dt for t in self.tiles.values()]).min())
+        table, n, nblocks, partial = self._dt_table()
+        g = self.be.grid
+        _hip.call("pxh_dt_multi", g.dtype, g.flags, table.data_ptr(), n, nblocks, self.cfl, self.dt.data_ptr(),
+                  partial.data_ptr(), st)
         return self.dt
 
     def _sweeps(self, hyd_in, hyd_out, bed_in, old, fill_bed):

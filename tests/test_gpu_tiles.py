@@ -92,10 +92,8 @@ def test_halo_strips_pack_and_unpack(cuda, dtype, lo, hw):
 @pytest.mark.parametrize("which,par", [("channel", (2, 2)), ("radial", (3, 2)), ("wetdry", (2, 3))])
 def test_tiled_run_equals_single_domain(cuda, dtype, which, par, fused):
     """float64: every owned cell of a tiled SSPRK2 run, and every dt, equal the
-    single-domain two-sweep run bit for bit.  float32 (hardware math, fp
-    contraction per inlined copy of the edge, so an edge evaluated at the start
-    of a strip can round differently from the same edge inside one, and tiles
-    start strips at other rows): the float32 parity bar, h, hu, G within 1e-5
+    single-domain run of the same stage path bit for bit.  float32 (hardware
+    math): the float32 parity bar, h, hu, G within 1e-5
     of max |field|, hv within 1e-5 of max |hu|, the compensated bed within 1e-4
     of max |z_b - z_b(0)|, dt within 1e-6."""
     if which == "channel":

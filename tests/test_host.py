@@ -479,3 +479,21 @@ def test_copy_batch_checks_its_arguments():
     assert lib.pxh_copy_batch(_hip.PXH_F64, None, 0, -1, None) == _hip.PXH_E_ARG
     assert lib.pxh_copy_batch(_hip.PXH_F64, None, 0, 0, None) == _hip.PXH_OK
     assert ctypes.sizeof(_hip.CopyJob) == 48
+
+
+def test_dt_multi_checks_its_arguments():
+    """pxh_dt_multi (compute_dt over the tiles of one device) rejects a bad
+    dtype, an empty domain table or block count and missing buffers; the block
+    count of a domain is its dt sweep's (host-only checks: no sweep runs)."""
+    lib = _hip.load()
+    assert ctypes.sizeof(_hip.DtDomain) == 64
+    assert lib.pxh_dt_multi_blocks(1, 10) == _hip.PXH_E_ARG
+    nb = lib.pxh_dt_multi_blocks(2054, 2054)
+    assert nb == -(-2054 // 62) * -(-(-(-2054 // 16)) // 2)
+    p = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    assert lib.pxh_dt_multi(5, 0, p, 1, nb, 0.5, p, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_multi(_hip.PXH_F32, 0, None, 1, nb, 0.5, p, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_multi(_hip.PXH_F32, 0, p, 0, nb, 0.5, p, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_multi(_hip.PXH_F32, 0, p, 1, 0, 0.5, p, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_multi(_hip.PXH_F64, 0, p, 1, nb, 0.5, None, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_multi(_hip.PXH_F64, 0, p, 1, nb, 0.5, p, None, None) == _hip.PXH_E_ARG
