@@ -395,6 +395,36 @@ int pxh_halo_unpack(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t 
  * the sum of rows * cols over the jobs before it (job 0: 0) and total the sum
  * over all n.  No destination may overlap a source of the same call.  The
  * table lives on the device, so only n, total and dtype are checked here. */
+/* The fused stage's two sweeps over several domains, one launch each: the
+ * tiles of a decomposition that share one device (the reference runs one
+ * tile per rank and has no such form).  Domain k is what
+ * pxh_roe_exner_stage_sweep and pxh_roe_exner_frame_sweep take for one tile:
+ * the stage input `in`, `out`, and for the second SSPRK2 stage (second = 1)
+ * `old`; the frame sweep reads hyd = out and bed = in and writes out (first
+ * stage) or the combine into old (second).  sblk0 / fblk0: the domain's first
+ * stage / frame block, the running sums of pxh_multi_stage_blocks /
+ * pxh_multi_frame_blocks over the domains before it; nblocks: the sum over
+ * all n of the sweep launched.  lo = 1 when the fields carry z_b_lo (float32
+ * compensated bed).  Every domain must be valid for the single-domain entry
+ * with grid (ny, nx, dtype, flags, dx): the table lives on the device, so
+ * only the scalars are checked here.  No dt is reduced (flags, sed, dt_dev
+ * and PXH_SWEEP_SKIP_G in sweep_flags as in the single-domain entries). */
+typedef struct {
+    pxh_fields in, out, old;
+    const uint8_t *mask0;
+    int32_t ny, nx;
+    int32_t sblk0, fblk0;
+    double dx;
+} pxh_sweep_domain;
+int64_t pxh_multi_stage_blocks(const pxh_grid *grid);
+int64_t pxh_multi_frame_blocks(int32_t ny, int32_t nx);
+int pxh_roe_exner_stage_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,
+                                    int32_t nblocks, int32_t second, int32_t lo, const pxh_sediments *sed,
+                                    const void *dt_dev, int sweep_flags, void *stream);
+int pxh_roe_exner_frame_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,
+                                    int32_t nblocks, int32_t second, const pxh_sediments *sed, const void *dt_dev,
+                                    int sweep_flags, void *stream);
+
 /* compute_dt_fn over several domains in one sweep launch and one reduction:
  * the tiles of a decomposition on one device (compute_dt_roeexner,
  * roe_exner_solver.py:118-122, whose MIN allreduce over ranks becomes the

@@ -68,6 +68,12 @@ class BCHandler(ctypes.Structure):
                 ("ny", ctypes.c_double)]
 
 
+class SweepDomain(ctypes.Structure):  # pxh_sweep_domain ("in" is a Python keyword)
+    _fields_ = [("in_", Fields), ("out", Fields), ("old", Fields), ("mask0", ctypes.c_void_p),
+                ("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("sblk0", ctypes.c_int32), ("fblk0", ctypes.c_int32),
+                ("dx", ctypes.c_double)]
+
+
 class DtDomain(ctypes.Structure):    # pxh_dt_domain
     _fields_ = [("h", ctypes.c_void_p), ("hu", ctypes.c_void_p), ("hv", ctypes.c_void_p), ("G", ctypes.c_void_p),
                 ("mask0", ctypes.c_void_p), ("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("blk0", ctypes.c_int32),
@@ -132,6 +138,14 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_halo_unpack": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_multi_stage_blocks": (ctypes.c_int64, [_P(Grid)]),
+    "pxh_multi_frame_blocks": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "pxh_roe_exner_stage_sweep_multi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(Sediments),
+                                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "pxh_roe_exner_frame_sweep_multi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                                       ctypes.c_int32, ctypes.c_int32, _P(Sediments), ctypes.c_void_p,
+                                                       ctypes.c_int, ctypes.c_void_p]),
     "pxh_dt_multi_blocks": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
     "pxh_dt_multi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                     ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -118,6 +118,73 @@ __global__ __launch_bounds__(SW *SWPB) void k_ens_dt_strip(F<T> s, const uint8_t
     if (threadIdx.x == 0) partial[m.part0 + local] = v;
 }
 
+// Field pointers of a pxh_fields on the device.
+template <typename T>
+__device__ __forceinline__ F<T> dev_fields(const pxh_fields &f) {
+    F<T> r;
+    r.h = static_cast<T *>(f.h);
+    r.hu = static_cast<T *>(f.hu);
+    r.hv = static_cast<T *>(f.hv);
+    r.z = static_cast<T *>(f.z);
+    r.z_b = static_cast<T *>(f.z_b);
+    r.n = static_cast<T *>(f.n);
+    r.n_b = static_cast<T *>(f.n_b);
+    r.G = static_cast<T *>(f.G);
+    r.z_b_lo = sizeof(T) == 4 ? static_cast<T *>(f.z_b_lo) : nullptr;
+    return r;
+}
+
+// The domain of a multi-domain launch whose block range holds blk.
+template <int pxh_sweep_domain::*START>
+__device__ __forceinline__ int find_domain(const pxh_sweep_domain *__restrict__ doms, int n, int blk) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (doms[mid].*START <= blk) lo = mid;
+        else hi = mid;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// The fused stage sweep over several domains (pxh_roe_exner_stage_sweep_multi).
+template <typename T, bool FAST, int MODE, bool LO, bool MPM>
+__global__ __launch_bounds__(SW *FWPB) void k_multi_stage(const pxh_sweep_domain *__restrict__ doms, int n, int sr,
+                                                          const T *__restrict__ dtp, SedK sd, int write_G) {
+    int lb, unused;
+    xcd_block(lb, unused);  // one-dimensional grid
+    const pxh_sweep_domain &d = doms[find_domain<&pxh_sweep_domain::sblk0>(doms, n, lb)];
+    const int ny = d.ny, nx = d.nx;
+    const int sgx = (nx + FO - 1) / FO;
+    const int local = lb - d.sblk0;
+    const int by = local / sgx, bx = local - by * sgx;
+    const StagePos p = stage_pos_at(bx, by, ny, nx, sr);
+    const F<T> none{};
+    if constexpr (MODE == 0)
+        stage_strip_run<T, FAST, 0, false, LO, MPM>(p, dev_fields<T>(d.in), dev_fields<T>(d.out), none, none,
+                                                    d.mask0, dtp, ny, nx, static_cast<T>(d.dx), sd, write_G,
+                                                    nullptr);
+    else
+        stage_strip_run<T, FAST, 1, false, LO, MPM>(p, dev_fields<T>(d.in), none, dev_fields<T>(d.out),
+                                                    dev_fields<T>(d.old), d.mask0, dtp, ny, nx,
+                                                    static_cast<T>(d.dx), sd, write_G, nullptr);
+}
+
+// The frame sweep over several domains (pxh_roe_exner_frame_sweep_multi):
+// hyd = out, bed = in.
+template <typename T, bool FAST, int MODE>
+__global__ __launch_bounds__(256) void k_multi_band_bed(const pxh_sweep_domain *__restrict__ doms, int n,
+                                                        const T *__restrict__ dtp, SedK sd, int write_G) {
+    const pxh_sweep_domain &d = doms[find_domain<&pxh_sweep_domain::fblk0>(doms, n, blockIdx.x)];
+    const long t = long(blockIdx.x - d.fblk0) * 256 + threadIdx.x;
+    if (t < frame_cells(d.ny, d.nx)) {
+        const F<T> none{};
+        const F<T> out = dev_fields<T>(d.out);
+        band_bed_cell<T, FAST, MODE>(t, out, dev_fields<T>(d.in), MODE == 0 ? out : none,
+                                     MODE == 1 ? dev_fields<T>(d.old) : none, d.mask0, dtp, d.ny, d.nx,
+                                     static_cast<T>(d.dx), sd, write_G);
+    }
+}
+
 // The standalone dt sweep over several domains of their own allocations
 // (pxh_dt_multi: the tiles of one device), one partial per block; domain k's
 // blocks start at doms[k].blk0.

@@ -1013,6 +1013,38 @@ using namespace pxh;
 
 #define DISPATCH(g, fn, ...) ((g)->dtype == PXH_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
+// The multi-domain fused sweeps (pxh_roe_exner_stage_sweep_multi / _frame_sweep_multi).
+template <typename T, bool FAST>
+static void multi_stage_t(const pxh_sweep_domain *doms, int n, int nblocks, int sr, bool second, bool lo,
+                          const T *dtp, const SedK &sd, int write_G, hipStream_t st) {
+#define PXH_MS(MODE_, LO_, MPM_)                                                                            \
+    hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE_, LO_, MPM_>), dim3(nblocks), dim3(SW * FWPB), 0, st, doms, n, \
+                       sr, dtp, sd, write_G)
+#define PXH_MS_MODE(LO_, MPM_)         \
+    if (second) PXH_MS(1, LO_, MPM_); \
+    else PXH_MS(0, LO_, MPM_)
+    if constexpr (sizeof(T) == 4) {
+        if (lo) {
+            if (sd.mpm) { PXH_MS_MODE(true, true); }
+            else { PXH_MS_MODE(true, false); }
+            return;
+        }
+    }
+    if (sd.mpm) { PXH_MS_MODE(false, true); }
+    else { PXH_MS_MODE(false, false); }
+#undef PXH_MS_MODE
+#undef PXH_MS
+}
+
+template <typename T, bool FAST>
+static void multi_frame_t(const pxh_sweep_domain *doms, int n, int nblocks, bool second, const T *dtp,
+                          const SedK &sd, int write_G, hipStream_t st) {
+    if (second)
+        hipLaunchKernelGGL((k_multi_band_bed<T, FAST, 1>), dim3(nblocks), dim3(256), 0, st, doms, n, dtp, sd, write_G);
+    else
+        hipLaunchKernelGGL((k_multi_band_bed<T, FAST, 0>), dim3(nblocks), dim3(256), 0, st, doms, n, dtp, sd, write_G);
+}
+
 extern "C" {
 
 int pxh_version(void) { return 1; }
@@ -1158,6 +1190,61 @@ int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const px
     const int write_G = (flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
     return DISPATCH(g, rx_frame_sweep, g, hyd, bed, old ? nullptr : out, old, m0, sd, dt_dev, write_G, cfl,
                     old ? dt_partial : nullptr, static_cast<hipStream_t>(stream));
+}
+
+int64_t pxh_multi_stage_blocks(const pxh_grid *g) {
+    if (!grid_ok(g)) return PXH_E_ARG;
+    if (!stage_fits(g)) return PXH_E_RANGE;
+    const dim3 fg = stage_grid(g->ny, g->nx, strip_rows(g->flags));
+    return int64_t(fg.x) * fg.y;
+}
+
+int64_t pxh_multi_frame_blocks(int32_t ny, int32_t nx) {
+    if (ny < 2 || nx < 2) return PXH_E_ARG;
+    return (frame_cells(ny, nx) + 255) / 256;
+}
+
+int pxh_roe_exner_stage_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,
+                                    int32_t nblocks, int32_t second, int32_t lo, const pxh_sediments *sd,
+                                    const void *dt_dev, int sweep_flags, void *stream) {
+    const int rows = PXH_STRIP_ROWS(flags);
+    if ((dtype != PXH_F32 && dtype != PXH_F64) || !doms || n <= 0 || nblocks <= 0 || !sd || !dt_dev ||
+        (rows != 0 && (rows < 8 || rows > 128)))
+        return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    const int write_G = (sweep_flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int sr = strip_rows(flags);
+    if (dtype == PXH_F64)
+        multi_stage_t<double, false>(doms, n, nblocks, sr, second, false, static_cast<const double *>(dt_dev),
+                                     sed_k(*sd), write_G, st);
+    else if (flags & PXH_EXACT_F32)
+        multi_stage_t<float, false>(doms, n, nblocks, sr, second, lo, static_cast<const float *>(dt_dev), sed_k(*sd),
+                                    write_G, st);
+    else
+        multi_stage_t<float, true>(doms, n, nblocks, sr, second, lo, static_cast<const float *>(dt_dev), sed_k(*sd),
+                                   write_G, st);
+    return launch_status();
+}
+
+int pxh_roe_exner_frame_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,
+                                    int32_t nblocks, int32_t second, const pxh_sediments *sd, const void *dt_dev,
+                                    int sweep_flags, void *stream) {
+    if ((dtype != PXH_F32 && dtype != PXH_F64) || !doms || n <= 0 || nblocks <= 0 || !sd || !dt_dev)
+        return PXH_E_ARG;
+    if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
+    const int write_G = (sweep_flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (dtype == PXH_F64)
+        multi_frame_t<double, false>(doms, n, nblocks, second, static_cast<const double *>(dt_dev), sed_k(*sd), write_G,
+                                     st);
+    else if (flags & PXH_EXACT_F32)
+        multi_frame_t<float, false>(doms, n, nblocks, second, static_cast<const float *>(dt_dev), sed_k(*sd), write_G,
+                                    st);
+    else
+        multi_frame_t<float, true>(doms, n, nblocks, second, static_cast<const float *>(dt_dev), sed_k(*sd), write_G,
+                                   st);
+    return launch_status();
 }
 
 int64_t pxh_dt_multi_blocks(int32_t ny, int32_t nx) {

@@ -157,10 +157,12 @@ class TiledExner:
     """Roe-Exner SSPRK2 on a par = (parNy, parNx) decomposition of an
     initialised single-domain state (init_fn applied), every tile on this device."""
 
-    def __init__(self, backend, state, mask, boundaries, par, cfl, fused=None, direct=True):
+    def __init__(self, backend, state, mask, boundaries, par, cfl, fused=None, direct=True, multi=True):
         """fused: the fused-stage path (three ghost rings); default when the
         ring fill is frame-local.  direct: exchange ghost strips by batched
-        device copies (default) rather than pack/unpack per side."""
+        device copies (default) rather than pack/unpack per side.  multi: run
+        every tile's fused sweeps in one launch each (default) rather than
+        one launch per tile."""
         self.be = backend
         self.cfl = float(cfl)
         self.par = tuple(par)
@@ -184,14 +186,17 @@ class TiledExner:
         self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
         self.dt_ready = False
         self.direct = bool(direct)
+        self.multi = bool(multi) and self.fused
         self._jobs = {}
         self._dt = None
+        self._sweep_tabs = {}
 
     def set_strip_rows(self, rows):
         """Owned rows per strip of every tile's fused stage sweep (8..128):
         a tile is a small grid, and shorter strips give its sweep enough
         wavefronts to fill the GPU.  Results do not depend on it."""
         lib = _hip.load()
+        self._sweep_tabs = {}  # the stage block counts follow the strip height
         for t in self.tiles.values():
             t.grid.flags = (int(t.grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (int(rows) << _hip.STRIP_ROWS_SHIFT)
             ws = lib.pxh_dt_workspace_size(t.grid)
@@ -279,6 +284,32 @@ class TiledExner:
                         _hip.call("pxh_halo_unpack", t.grid, t.fields(which), t.hw, groups, side, src.data_ptr(),
                                   st)
 
+    def _sweep_table(self, src, dst, old):
+        """The tiles as a device table of pxh_sweep_domain for one stage
+        (src -> dst, combined into old when given) and the stage and frame
+        block counts of the multi-domain sweeps (built once per stage kind and
+        strip height)."""
+        key = (src, dst, old)
+        if key not in self._sweep_tabs:
+            lib = _hip.load()
+            tiles = list(self.tiles.values())
+            table = (_hip.SweepDomain * len(tiles))()
+            sb = fb = 0
+            for k, t in enumerate(tiles):
+                d = table[k]
+                d.in_, d.out = t.fields(src), t.fields(dst)
+                if old is not None:
+                    d.old = t.fields(old)
+                d.mask0, d.ny, d.nx, d.sblk0, d.fblk0, d.dx = t.m0, t.grid.ny, t.grid.nx, sb, fb, t.grid.dx
+                ns, nf = lib.pxh_multi_stage_blocks(t.grid), lib.pxh_multi_frame_blocks(t.grid.ny, t.grid.nx)
+                if ns <= 0 or nf <= 0:
+                    raise ValueError("a tile the fused stage cannot sweep")
+                sb += ns
+                fb += nf
+            raw = np.frombuffer(bytes(table), dtype=np.uint8).copy()
+            self._sweep_tabs[key] = (torch.from_numpy(raw).to(self.dt.device), len(tiles), sb, fb)
+        return self._sweep_tabs[key]
+
     def _dt_table(self):
         """The tiles' buffer set a as a device table of pxh_dt_domain (built
         once) and the partial minima buffer of pxh_dt_multi."""
@@ -335,6 +366,25 @@ class TiledExner:
         dtp = self.dt.data_ptr()
         sed = self.be.sed
         skip = _hip.SWEEP_SKIP_G if old is None else 0
+        if self.multi:
+            # every tile's sweep in one launch each; the ring fills stay per tile
+            tab, n, nsb, nfb = self._sweep_table(src, dst, old)
+            g = self.tiles[(0, 0)].grid
+            lo = int(self.tiles[(0, 0)].a.z_b_lo is not None)
+            second = int(old is not None)
+            _hip.call("pxh_roe_exner_stage_sweep_multi", g.dtype, g.flags, tab.data_ptr(), n, nsb, second, lo, sed,
+                      dtp, skip, st)
+            for t in self.tiles.values():
+                if t.n_entries:
+                    _hip.call("pxh_bc_apply", t.grid, t.fields(dst), t.programs["hy"], st)
+            _hip.call("pxh_roe_exner_frame_sweep_multi", g.dtype, g.flags, tab.data_ptr(), n, nfb, second, sed, dtp,
+                      skip, st)
+            for t in self.tiles.values():
+                if t.n_entries:
+                    _hip.call("pxh_bc_apply", t.grid, t.fields(old if old is not None else dst),
+                              t.programs["bed" if old is not None else "bed1"], st)
+            self.exchange(old if old is not None else dst, _STEP)
+            return
         for t in self.tiles.values():
             fs, fd = t.fields(src), t.fields(dst)
             fo = t.fields(old) if old is not None else None

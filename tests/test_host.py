@@ -497,3 +497,29 @@ def test_dt_multi_checks_its_arguments():
     assert lib.pxh_dt_multi(_hip.PXH_F32, 0, p, 1, 0, 0.5, p, p, None) == _hip.PXH_E_ARG
     assert lib.pxh_dt_multi(_hip.PXH_F64, 0, p, 1, nb, 0.5, None, p, None) == _hip.PXH_E_ARG
     assert lib.pxh_dt_multi(_hip.PXH_F64, 0, p, 1, nb, 0.5, p, None, None) == _hip.PXH_E_ARG
+
+
+def test_multi_domain_sweeps_check_their_arguments():
+    """The multi-domain fused sweeps (tiles on one device) size their launches
+    with the single-domain geometry and reject bad scalars before any launch
+    (host-only checks: no sweep runs)."""
+    lib = _hip.load()
+    assert ctypes.sizeof(_hip.SweepDomain) == 248
+    sh = _hip.STRIP_ROWS_SHIFT
+    assert lib.pxh_multi_stage_blocks(_hip.Grid(2054, 2054, _hip.PXH_F32, 12 << sh, 0.05)) == 35 * 172
+    assert lib.pxh_multi_stage_blocks(_hip.Grid(2054, 2054, _hip.PXH_F32, 0, 0.05)) == 35 * 65
+    assert lib.pxh_multi_stage_blocks(_hip.Grid(2054, 2054, _hip.PXH_F32, 4 << sh, 0.05)) == _hip.PXH_E_ARG
+    assert lib.pxh_multi_frame_blocks(2054, 2054) == -(-(2 * 2 * 2054 + 2 * 2 * 2050) // 256)
+    assert lib.pxh_multi_frame_blocks(1, 5) == _hip.PXH_E_ARG
+    sed = _hip.Sediments()
+    p = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    assert lib.pxh_roe_exner_stage_sweep_multi(9, 0, p, 1, 8, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, None, 1, 8, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 0, 8, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 200 << sh, p, 1, 8, 0, 0, sed, p, 0, None) == \
+        _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 1, 8, 0, 0, sed, None, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_frame_sweep_multi(_hip.PXH_F64, 0, p, 1, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_frame_sweep_multi(_hip.PXH_F64, 0, None, 1, 8, 0, sed, p, 0, None) == _hip.PXH_E_ARG
+    sed.mpm, sed.n_sed = 1, 0  # MPM without a sediment class
+    assert lib.pxh_roe_exner_frame_sweep_multi(_hip.PXH_F64, 0, p, 1, 8, 0, sed, p, 0, None) == _hip.PXH_E_RANGE
