@@ -83,6 +83,8 @@ def parse():
     p.add_argument("--adj-every", type=int, default=20, help="checkpoint interval of the gradient run")
     p.add_argument("--accuracy", type=int, default=1,
                    help="also run the timed steps in float64 and report the float32 result's deviation (1) or not (0)")
+    p.add_argument("--tile-rows", type=int, default=0,
+                   help="strip height of the tiled run's fused stage (0: tuned with --autotune 1, else 32)")
     p.add_argument("--autotune", type=int, default=1,
                    help="choose the fused stage's strip height by timing the candidates (1) or keep 32 (0)")
     return p.parse_args()
@@ -133,8 +135,8 @@ def tiled_run(a, n, params, torch):
     del arr
     mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
     state = solver.init_fn(state, mask, sc)
-    rows, tune_ms = None, {}
-    if a.autotune:
+    rows, tune_ms = (a.tile_rows or None), {}
+    if a.autotune and not rows:
         from pyexner_amd.autotune import tune_tile_strip_rows
         probe = TiledExner(sc.backend, state, mask, bm, (4, 4), params["cfl"])
         rows, tune_ms = tune_tile_strip_rows(probe)
