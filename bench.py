@@ -339,8 +339,9 @@ def adjoint_run(a, torch):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (every run here uses one "
-              "MI355X: see the sharding design in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported: the path runs on one "
+              "MI355X (a decomposition across GPUs would give each GPU one tile of pyexner_amd.tiles)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from pyexner_amd import _hip
