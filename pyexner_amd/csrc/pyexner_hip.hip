@@ -398,9 +398,9 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
 //     partial[0, nf);
 //   frame sweep: the frame's bed update from the filled hyd and in's bed
 //     (MODE 0 into out, MODE 1 combined into old), and with partial the dt
-//     minima of the frame's edges and of the edges across strip seams in
-//     partial[nf, nf + nbd) followed by the final reduction of all nf + nbd
-//     minima into *dtp.
+//     minima of the frame's edges and of the edges across strip seams, each
+//     folded with a share of the stage sweep's minima, in partial[nf, nf +
+//     nbd), followed by the final reduction of those nbd minima into *dtp.
 template <typename T, bool FAST>
 static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, const F<T> *old, const uint8_t *m0,
                          const pxh_sediments *sd, const T *dtp, int write_G, T *part, hipStream_t st) {
@@ -445,9 +445,9 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
     const long nwork = band_dt_work(ny, nx, sr);
     const int nbd = int((nwork + 255) / 256);
     hipLaunchKernelGGL((k_rx_band_dt<T, FAST>), dim3(nbd), dim3(256), 0, st, *old, m0, seam_cols<T>(g, part), ny, nx,
-                       dx, sr, nband, nwork, part + nf);
+                       dx, sr, nband, nwork, part, nf, part + nf);
     if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, nf + nbd, static_cast<T>(cfl), 1, dtp);
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part + nf, nbd, static_cast<T>(cfl), 1, dtp);
     return launch_status();
 }
 

@@ -595,12 +595,17 @@ static __host__ __device__ inline long band_dt_work(int ny, int nx, int sr) {
     return frame_cells(ny, nx) + fused_seam_edges(ny, nx, sr);
 }
 
+// One partial per block of the frame and seam edges, folded with a share of
+// the nprev partial minima `prev` of the stage sweep (grid-strided), so the
+// final reduction reads this kernel's partials only.
 template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_rx_band_dt(F<T> s, const uint8_t *__restrict__ m0, const T *__restrict__ xs,
                                                      int ny, int nx, T dx, int sr, long nband, long nwork,
-                                                     T *__restrict__ partial) {
+                                                     const T *__restrict__ prev, int nprev, T *__restrict__ partial) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    block256_min_to(band_dt_cell<T, FAST>(t, nband, nwork, s, m0, xs, ny, nx, dx, sr), partial + blockIdx.x);
+    T v = band_dt_cell<T, FAST>(t, nband, nwork, s, m0, xs, ny, nx, dx, sr);
+    for (long k = t; k < nprev; k += long(gridDim.x) * 256) v = pmin(v, prev[k]);
+    block256_min_to(v, partial + blockIdx.x);
 }
 
 }  // namespace pxh
