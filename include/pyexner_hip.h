@@ -207,7 +207,11 @@ int pxh_exner_solve_2D(const pxh_grid *grid, const pxh_fields *in, const pxh_fie
  * return dt_dev holds compute_dt_fn(a) of the new a (reduced inside the last
  * sweep), so consecutive calls pass dt_ready = 1.  a is updated in place; b
  * and c are scratch whose z, n, n_b may share a's (static fields); the evolving
- * fields of a, b, c must not alias.  workspace as for pxh_roe_exner_dt. */
+ * fields of a, b, c must not alias.  workspace as for pxh_roe_exner_dt.
+ * The ring is averaged like every other cell (SSPRK2.py:40): with bed values
+ * that are not fixed points of the average (PXH_BV_AFFINE_T, PXH_BV_OFFSET)
+ * the call runs two whole steps with their full fills, the combine and a dt
+ * pass instead of the fused sweeps (slower, same result as the reference). */
 int pxh_roe_exner_ssprk2(const pxh_grid *grid, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
                          const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc, double cfl,
                          void *dt_dev, void *workspace, int dt_ready, void *stream);
@@ -248,10 +252,16 @@ int pxh_dt_finalize(const pxh_grid *grid, const void *dt_partial, double cfl, vo
  *   frame sweep: compute_G + exner_solve_2D of the frame's cells from the
  *     filled hyd.{h,hu,hv} and bed.{z_b,z_b_lo,G} (the stage input):
  *     old == NULL: into out.{z_b,z_b_lo,G}; old != NULL: combined into old as
- *     above, and with dt_partial (the one the stage sweep filled) the minima of
- *     the frame's edges and of the edges across strip seams, and the final
- *     reduction: dt_dev = compute_dt_fn(old), so it holds
- *     the stage's dt on entry and the next step's on return.
+ *     above, and with dt_partial the minima of the frame's edges and of the
+ *     edges across strip seams, and the final reduction: dt_dev =
+ *     compute_dt_fn(old), so it holds the stage's dt on entry and the next
+ *     step's on return.
+ *   Pairing: the final reduction folds in the strip minima and seam records
+ *     that the stage sweep left in dt_partial.  A frame sweep given dt_partial
+ *     is valid only after a second-stage sweep (old != NULL) that was given
+ *     the same dt_partial, the same grid (strip height included) and the same
+ *     old, with nothing else writing dt_partial in between; otherwise it
+ *     reduces stale minima.  Pass dt_partial to both calls or to neither.
  * flags: PXH_SWEEP_SKIP_G leaves out.G unwritten in MPM mode (the next stage
  * recomputes G; the first SSPRK2 stage uses it).  dt_partial holds
  * pxh_dt_workspace_size(grid) bytes. */

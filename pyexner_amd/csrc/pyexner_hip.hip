@@ -505,6 +505,20 @@ static int seam_dt(const pxh_grid *g, const F<T> &s, const uint8_t *m0, const T 
     return launch_status();
 }
 
+// True when the program holds ring values that are not fixed points of the
+// SSPRK2 average: a bed that moves with time (PXH_BV_AFFINE_T: Berthon,
+// NormalFlowDepth) or an offset of a cell the step changes (PXH_BV_OFFSET:
+// SteepFall).  The reference averages the ring like every other cell
+// (integrators/SSPRK2.py:40), so such a ring after a step is
+// 0.5 a_ring + 0.5 fill(c)_ring, not the fill of the averaged state.
+static bool ring_values_averaged(const pxh_bc *bc) {
+    if (!bc || bc->n_entries <= 0) return false;
+    const int n = bc->n_values < PXH_MAX_VALUES ? bc->n_values : PXH_MAX_VALUES;
+    for (int k = 0; k < n; ++k)
+        if (bc->values[k].kind == PXH_BV_AFFINE_T || bc->values[k].kind == PXH_BV_OFFSET) return true;
+    return false;
+}
+
 // One SSPRK2 step (integrators/SSPRK2.py:32-40) on three buffer sets:
 // dt from a (unless dt_ready); b = S(a); c = S(b) with the combine
 // a = 0.5 a + 0.5 c fused into the second bed sweep, which also reduces the
@@ -514,6 +528,10 @@ static int seam_dt(const pxh_grid *g, const F<T> &s, const uint8_t *m0, const T 
 // copies G together with h, hu, hv from one source (Reflective, Periodic) or
 // leaves it to compute_G of the filled h, hu, hv (Transmissive), and compute_G
 // depends on the velocity only through |u|^2, so the fill cannot change it.
+// The final (z_b, G) fill of the averaged state equals the average of the
+// filled states only for copies and constants; a program with bed values
+// that move (ring_values_averaged) takes two whole steps, each with its full
+// fill, then the combine and the dt of the combined state.
 template <typename T, bool FAST>
 static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields *b_, const pxh_fields *c_,
                        const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl, void *dt, void *ws,
@@ -532,6 +550,18 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
         hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, int(dg.x * dg.y),
                            static_cast<T>(cfl), 1, dtp);
         if (int e = launch_status()) return e;
+    }
+    if (ring_values_averaged(bc)) {
+        if (int e = rx_step_t<T, FAST>(g, a_, b_, m0, sd, bc, dtp, st)) return e;
+        if (int e = rx_step_t<T, FAST>(g, b_, c_, m0, sd, bc, dtp, st)) return e;
+        const long n = long(ny) * nx;
+        hipLaunchKernelGGL(k_combine<T>, dim3((n + 255) / 256), dim3(256), 0, st, a, c, a, n);
+        if (int e = launch_status()) return e;
+        hipLaunchKernelGGL((k_rx_dt_strip<T, FAST>), dg, sb, 0, st, a, m0, ny, nx, dx, part);
+        if (int e = launch_status()) return e;
+        hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, part, int(dg.x * dg.y),
+                           static_cast<T>(cfl), 1, dtp);
+        return launch_status();
     }
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (sd->mpm ? 0u : (1u << PXH_G_G));

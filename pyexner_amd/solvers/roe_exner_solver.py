@@ -49,6 +49,16 @@ def sediment_table(seds, constant, np_dtype):
     return sd
 
 
+def workspace_bytes(grid):
+    """pxh_dt_workspace_size of the grid at the largest strip count any strip
+    height gives (8 rows), or of its own height if that needs more."""
+    lib = _hip.load()
+    g8 = _hip.Grid(grid.ny, grid.nx, grid.dtype, (int(grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT))
+                   | (8 << _hip.STRIP_ROWS_SHIFT), grid.dx)
+    a, b = lib.pxh_dt_workspace_size(grid), lib.pxh_dt_workspace_size(g8)
+    return a if a < 0 else max(a, b)
+
+
 class ExnerBackend:
     """Host-side description of one domain: grid, sediment table, ring-fill
     programs and the dt workspace.  Built once by config_fn."""
@@ -81,22 +91,28 @@ class ExnerBackend:
         self.fused_stage = bool(fused_stage)
         if self.bc_step is not None and not self.fused_stage:
             self.bc_step.flags &= ~_hip.BC_FRAME
-        ws = _hip.load().pxh_dt_workspace_size(self.grid)
+        # The dt workspace is sized once for the shortest strips (8 rows: the
+        # most strips and seams), so set_strip_rows never moves it and a
+        # hipGraph captured before a retune still points at live memory.
+        ws = workspace_bytes(self.grid)
         if ws < 0:
             raise ValueError("invalid grid")
         self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
         self.device = device
 
     def set_strip_rows(self, rows):
-        """Owned rows per strip of the fused stage sweep (pyexner_amd.autotune);
-        the dt workspace follows the strip count."""
-        flags = (int(self.grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (int(rows) << _hip.STRIP_ROWS_SHIFT)
+        """Owned rows per strip of the fused stage sweep (pyexner_amd.autotune).
+        A hipGraph captured earlier keeps the launch shape it was captured
+        with; its buffers stay valid (the workspace does not move)."""
+        rows = int(rows)
+        if not 8 <= rows <= 128:
+            raise ValueError(f"strip height {rows} is outside 8..128")
+        flags = (int(self.grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (rows << _hip.STRIP_ROWS_SHIFT)
         self.grid.flags = flags
         ws = _hip.load().pxh_dt_workspace_size(self.grid)
-        if ws < 0:
-            raise ValueError(f"strip height {rows} is outside 8..128")
-        if ws > self.workspace.numel():
-            self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=self.device)
+        if ws < 0 or ws > self.workspace.numel():
+            raise ValueError(f"strip height {rows}: dt workspace of {ws} bytes exceeds the {self.workspace.numel()} "
+                             "allocated")
 
     # raw entry points on preallocated buffers (no allocation) -----------------
     def init_into(self, fs, mask0_ptr, stream):

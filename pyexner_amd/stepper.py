@@ -72,6 +72,11 @@ class ExnerStepper:
         buffers return to their starting roles)."""
         if integrator != "SSPRK2" and nsteps % 2:
             raise ValueError("Forward-Euler capture needs an even number of steps")
+        bm = self.be.boundaries
+        if bm is not None and bm.time_dependent:
+            # pxh_bc (and its time) is a kernel argument: every replay would
+            # fill a moving bed at the time of capture
+            raise NotImplementedError("hipGraph capture of a ring fill whose values move with time")
         fn = self.ssprk2_step if integrator == "SSPRK2" else self.fe_step
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())

@@ -42,6 +42,7 @@ import torch
 
 from . import _hip
 from .boundaries import ENTRY_DTYPE, set_values
+from .solvers.roe_exner_solver import workspace_bytes
 from .state import fields_struct
 
 HW = 2  # ghost rings on a shared side, two-sweep path
@@ -137,7 +138,9 @@ class Tile:
                 bc.handlers[k].kind, bc.handlers[k].nx, bc.handlers[k].ny = kind, nxn, nyn
             set_values(bc, values)
             self.programs[key] = bc
-        ws = _hip.load().pxh_dt_workspace_size(self.grid)
+        ws = workspace_bytes(self.grid)  # sized for any strip height: it never moves
+        if ws < 0:
+            raise ValueError("invalid tile geometry")
         self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
         self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
         lib = _hip.load()
@@ -194,16 +197,19 @@ class TiledExner:
     def set_strip_rows(self, rows):
         """Owned rows per strip of every tile's fused stage sweep (8..128):
         a tile is a small grid, and shorter strips give its sweep enough
-        wavefronts to fill the GPU.  Results do not depend on it."""
+        wavefronts to fill the GPU.  Results do not depend on it.
+        A hipGraph captured earlier keeps the launch shape it was captured
+        with: its sweep tables (keyed by strip height) and workspaces stay
+        allocated."""
         lib = _hip.load()
-        self._sweep_tabs = {}  # the stage block counts follow the strip height
+        rows = int(rows)
+        if not 8 <= rows <= 128:
+            raise ValueError(f"strip height {rows} is outside 8..128")
         for t in self.tiles.values():
-            t.grid.flags = (int(t.grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (int(rows) << _hip.STRIP_ROWS_SHIFT)
+            t.grid.flags = (int(t.grid.flags) & ~(0xFF << _hip.STRIP_ROWS_SHIFT)) | (rows << _hip.STRIP_ROWS_SHIFT)
             ws = lib.pxh_dt_workspace_size(t.grid)
-            if ws < 0:
-                raise ValueError(f"strip height {rows} is outside 8..128")
-            if ws > t.workspace.numel():
-                t.workspace = torch.empty(int(ws), dtype=torch.uint8, device=t.workspace.device)
+            if ws < 0 or ws > t.workspace.numel():
+                raise ValueError(f"strip height {rows}: dt workspace of {ws} bytes exceeds the allocation")
 
     def _neighbour(self, pos, side):
         ty, tx = pos
@@ -289,7 +295,7 @@ class TiledExner:
         (src -> dst, combined into old when given) and the stage and frame
         block counts of the multi-domain sweeps (built once per stage kind and
         strip height)."""
-        key = (src, dst, old)
+        key = (src, dst, old, int(next(iter(self.tiles.values())).grid.flags))
         if key not in self._sweep_tabs:
             lib = _hip.load()
             tiles = list(self.tiles.values())
