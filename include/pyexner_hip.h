@@ -216,6 +216,21 @@ int pxh_roe_exner_ssprk2(const pxh_grid *grid, const pxh_fields *a, const pxh_fi
                          const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc, double cfl,
                          void *dt_dev, void *workspace, int dt_ready, void *stream);
 
+/* pxh_roe_exner_ssprk2 without the workspace argument, for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The dt
+ * workspace lives in a static device array of PXH_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends; grids whose pxh_dt_workspace_size exceeds
+ * it are refused with PXH_E_RANGE (float32 up to about 8192 x 8192 at any
+ * strip height).  The call allocates nothing; the caller still passes a, b, c
+ * and dt_dev as for pxh_roe_exner_ssprk2, whose results it gives bit for bit.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (concurrent streams, threads) race and must be serialised by the caller. */
+#define PXH_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int pxh_roe_exner_ssprk2_noalloc(const pxh_grid *grid, const pxh_fields *a, const pxh_fields *b,
+                                 const pxh_fields *c, const uint8_t *mask0, const pxh_sediments *sed,
+                                 const pxh_bc *bc, double cfl, void *dt_dev, int dt_ready, void *stream);
+
 /* The two fused sweeps a step is built from, one launch each, for callers that
  * sequence the step themselves (ring fills or halo exchanges in between):
  *   hydro sweep: roe_solve_2D + momentum_corrections (roe_exner.py:535-612, 14-62):

@@ -25,6 +25,7 @@ BC_VALUE = 0x80  # pxh_bc_entry.nchain flag: a value entry, chain[0] indexes pxh
 BV_CONST, BV_AFFINE_T, BV_OFFSET, BV_OUTFLUX = 1, 2, 3, 4
 MAX_VALUES = 32
 N_GROUPS = 7
+NOALLOC_SCRATCH_BYTES = 64 << 20  # PXH_NOALLOC_SCRATCH_BYTES
 HALO_W, HALO_N, HALO_E, HALO_S = 1, 2, 4, 8  # pxh_halo_* sides
 SWEEP_SKIP_G = 1  # flags of the stage/frame sweeps: leave out.G unwritten in MPM mode
 G_H, G_HUV, G_Z, G_ZB, G_N, G_NB, G_G = range(7)
@@ -120,6 +121,9 @@ _SIGS = {
     "pxh_roe_exner_ssprk2": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
                                             _P(Sediments), _P(BC), ctypes.c_double, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "pxh_roe_exner_ssprk2_noalloc": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
+                                                    _P(Sediments), _P(BC), ctypes.c_double, ctypes.c_void_p,
+                                                    ctypes.c_int, ctypes.c_void_p]),
     "pxh_roe_exner_hydro_sweep": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_roe_exner_bed_sweep": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), _P(Fields),

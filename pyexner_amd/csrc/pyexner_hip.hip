@@ -7,6 +7,7 @@
 //   -> G -> Exner flux + bed update -> ring fill (z_b, G)
 // and the CFL dt is a two-level min reduction into a device scalar.
 #include <initializer_list>
+#include <mutex>
 
 #include "common.hpp"
 #include "flux_roe.hpp"
@@ -1075,6 +1076,49 @@ static void multi_frame_t(const pxh_sweep_domain *doms, int n, int nblocks, bool
         hipLaunchKernelGGL((k_multi_band_bed<T, FAST, 0>), dim3(nblocks), dim3(256), 0, st, doms, n, dtp, sd, write_G);
 }
 
+// ---------------------------------------------------------------------------
+// The dt workspace of pxh_roe_exner_ssprk2_noalloc: a static device array,
+// one copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (in host code it names a host-side stand-in): it
+// passes the device address hipGetSymbolAddress returns, looked up once per
+// device with that device current and kept in scratch_dev[].  Every write of
+// the workspace precedes its reads within one call (each block stores its
+// partial minimum and seam records before the reductions read them), so it
+// needs no clearing between calls.  Calls that overlap on one GPU share it
+// and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PXH_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static void *scratch_dev[SCRATCH_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+// The device address of `scratch` on the device that owns `st` (the current
+// device for the null stream).
+static int scratch_of(hipStream_t st, void **out) {
+    int dev = 0;
+    hipError_t e = st ? hipStreamGetDevice(st, &dev) : hipGetDevice(&dev);
+    if (e != hipSuccess) return int(e);
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return PXH_E_RANGE;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        int cur = 0;
+        if ((e = hipGetDevice(&cur)) != hipSuccess) return int(e);
+        if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return int(e);
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) {
+            const hipError_t e2 = hipSetDevice(cur);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e != hipSuccess) return int(e);
+        if (!p) return PXH_E_ARG;
+        scratch_dev[dev] = p;
+    }
+    *out = scratch_dev[dev];
+    return PXH_OK;
+}
+
 extern "C" {
 
 int pxh_version(void) { return 1; }
@@ -1343,6 +1387,18 @@ int pxh_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_field
     if (!bc_ok(bc)) return PXH_E_ARG;
     return DISPATCH(g, rx_ssprk2, g, a, b, c, m0, sd, bc, cfl, dt_dev, ws, dt_ready,
                     static_cast<hipStream_t>(stream));
+}
+
+int pxh_roe_exner_ssprk2_noalloc(const pxh_grid *g, const pxh_fields *a, const pxh_fields *b, const pxh_fields *c,
+                                 const uint8_t *m0, const pxh_sediments *sd, const pxh_bc *bc, double cfl,
+                                 void *dt_dev, int dt_ready, void *stream) {
+    if (!grid_ok(g)) return PXH_E_ARG;
+    const int64_t need = pxh_dt_workspace_size(g);
+    if (need < 0) return PXH_E_ARG;
+    if (need > PXH_NOALLOC_SCRATCH_BYTES) return PXH_E_RANGE;
+    void *ws = nullptr;
+    if (int e = scratch_of(static_cast<hipStream_t>(stream), &ws)) return e;
+    return pxh_roe_exner_ssprk2(g, a, b, c, m0, sd, bc, cfl, dt_dev, ws, dt_ready, stream);
 }
 
 int pxh_roe_exner_dt(const pxh_grid *g, const pxh_fields *s, const uint8_t *m0, double cfl, void *dt_out,
