@@ -154,7 +154,7 @@ __global__ __launch_bounds__(SW *FWPB) void k_multi_stage(const pxh_sweep_domain
     xcd_block(lb, unused);  // one-dimensional grid
     const pxh_sweep_domain &d = doms[find_domain<&pxh_sweep_domain::sblk0>(doms, n, lb)];
     const int ny = d.ny, nx = d.nx;
-    const int sgx = (nx + FO - 1) / FO;
+    const int sgx = stage_gx(nx);
     const int local = lb - d.sblk0;
     const int by = local / sgx, bx = local - by * sgx;
     const StagePos p = stage_pos_at(bx, by, ny, nx, sr);

@@ -196,10 +196,11 @@ __device__ __forceinline__ HD<T> hd_derive(const HR<T> &q) {
     const bool wet = d.hf > T_;
     if constexpr (FAST) {
         using M = Mth<float, true>;
-        const float ih = M::rcp(d.hf);
+        const float r = M::rsq(d.hf);  // 1 / sqrt(hf): gives 1 / hf and sqrt(hf)
+        const float ih = r * r;
         d.u = wet ? d.hu * ih : zero;
         d.v = wet ? d.hv * ih : zero;
-        d.s = M::sqrt(d.hf);
+        d.s = d.hf * r;
         d.e = d.s * 3.1320919526731650f;  // sqrt(9.81)
     } else {
         d.u = wet ? d.hu / d.hf : zero;
@@ -321,14 +322,15 @@ __device__ __forceinline__ void rx_edge_exact(const HD<T> &I, const HD<T> &J, T 
 // column of P (r1 = (1, l1, vt), r2 = (0, 0, ct), r3 = (1, l3, vt)), so the
 // coefficients going to each side are summed first and the two vectors are
 // assembled once.  Mathematically the same split as rx_upwind, with a
-// different summation order (float32 hardware path only).
+// different summation order (float32 hardware path only).  The products and
+// sums are written as explicit fused multiply-adds so that the subcritical
+// path below rounds exactly as this one does.
 template <int AX>
 __device__ __forceinline__ void rx_upwind_coef(float l1, float l2, float l3, float lE1, float lE3, float a1,
                                                float a2, float a3, float b1, float vt, float ct, float P[3],
                                                float M[3]) {
-#pragma clang fp contract(on)
     const float zero = 0.0f;
-    const float f1 = l1 * a1 - b1, f2 = l2 * a2, f3 = l3 * a3 + b1;
+    const float f1 = __builtin_fmaf(l1, a1, -b1), f2 = l2 * a2, f3 = __builtin_fmaf(l3, a3, b1);
     const float e1 = lE1 * a1, e3 = lE3 * a3;
     // a NaN speed sends its wave to neither side, as in rx_upwind
     const float p1 = (l1 > zero ? f1 : zero) + (lE1 > zero ? e1 : zero);
@@ -337,9 +339,9 @@ __device__ __forceinline__ void rx_upwind_coef(float l1, float l2, float l3, flo
     const float m3 = (l3 <= zero ? f3 : zero) + (lE3 <= zero ? e3 : zero);
     const float p2 = l2 > zero ? f2 : zero;
     const float m2 = l2 <= zero ? f2 : zero;
-    float P1 = p1 * l1 + p3 * l3, M1 = m1 * l1 + m3 * l3;
+    float P1 = __builtin_fmaf(p1, l1, p3 * l3), M1 = __builtin_fmaf(m3, l3, m1 * l1);
     const float P0 = p1 + p3, M0 = m1 + m3;
-    float P2 = P0 * vt + p2 * ct, M2 = M0 * vt + m2 * ct;
+    float P2 = __builtin_fmaf(P0, vt, p2 * ct), M2 = __builtin_fmaf(M0, vt, m2 * ct);
     unrot<AX>(P1, P2);
     unrot<AX>(M1, M2);
     P[0] = P0; P[1] = P1; P[2] = P2;
@@ -348,10 +350,17 @@ __device__ __forceinline__ void rx_upwind_coef(float l1, float l2, float l3, flo
 
 // The same edge in float32 hardware arithmetic: one reciprocal for the Roe
 // average, one reciprocal square root for the celerity and its inverse, and
-// h~^(4/3) as exp2(4/3 log2 h~).
+// h~^(-4/3) as exp2(-4/3 log2 h~).
+//
+// Subcritical edges (l1 <= 0 < l3) without an entropy fix (no transonic
+// rarefaction in wave 1 or 3) send wave 1 to M and wave 3 to P whatever the
+// data, and carry no entropy parts: when every lane whose result is used
+// (care) has such an edge, the wavefront takes the short form, which gives
+// rx_upwind_coef's values bit for bit (the parts it drops are exact zeros).
+// The choice is per wavefront, so it never changes a result.
 template <int AX>
 __device__ __forceinline__ void rx_edge_fast(const HD<float> &I, const HD<float> &J, float dx, float P[3],
-                                             float M[3]) {
+                                             float M[3], bool care = true) {
 #pragma clang fp contract(on)
     using Mf = Mth<float, true>;
     const float T_ = float(DRY_TOL), g = float(GRAV), V = float(VEL_TOL), zero = 0.0f;
@@ -368,20 +377,10 @@ __device__ __forceinline__ void rx_edge_fast(const HD<float> &I, const HD<float>
     const float ut = (uhi * si + uhj * sj) * iden;
     const float vt = (vhi * si + vhj * sj) * iden;
     const float l1r = ut - ct, l2 = ut, l3r = ut + ct;
-    float ei = uhi - I.e, ej = uhj - J.e;
-    float lE1 = zero, l1 = l1r;
-    if ((ei < zero) && (ej > zero)) {
-        const float t = ei * (ej - l1r) * Mf::rcp(ej - ei);
-        lE1 = l1r - t;
-        l1 = t;
-    }
-    ei = uhi + I.e;
-    ej = uhj + J.e;
-    float lE3 = zero, l3 = l3r;
-    if ((ei < zero) && (ej > zero)) {
-        lE3 = l3r - ej * (l3r - ei) * Mf::rcp(ej - ei);
-        l3 = ej * (l3r - ei) * Mf::rcp((ej - ei) + V);
-    }
+    const float ei1 = uhi - I.e, ej1 = uhj - J.e;
+    const float ei3 = uhi + I.e, ej3 = uhj + J.e;
+    const bool hh1 = (ei1 < zero) && (ej1 > zero);
+    const bool hh3 = (ei3 < zero) && (ej3 > zero);
     const float dh = hj - hi;
     const float dhu = hj * uhj - hi * uhi;
     const float dhv = hj * vhj - hi * vhi;
@@ -393,17 +392,42 @@ __device__ __forceinline__ void rx_edge_fast(const HD<float> &I, const HD<float>
     float src = (-g * ht) * dz;
     const float nt = 0.5f * (I.n + J.n);
     if (nt != zero) {
-        const float h43 = Mf::exp2(1.3333333333333333f * Mf::log2(ht));
-        const float sf = (nt * nt * Mf::sqrt(ut * ut + vt * vt) * ut) * Mf::rcp(pmax(T_, h43));
+        // 1 / max(DRY_TOL, h~^(4/3)) = min(1 / DRY_TOL, h~^(-4/3))
+        const float ih43 = pmin(1.0f / T_, Mf::exp2(-1.3333333333333333f * Mf::log2(ht)));
+        const float sf = (nt * nt * Mf::sqrt(ut * ut + vt * vt) * ut) * ih43;
         src = src - g * ht * sf * dx;
     }
     const float b1 = src * ih2;
+    const bool plain = !hh1 && !hh3 && !(l1r > zero) && (l3r > zero);
+    if (__builtin_amdgcn_ballot_w64(care && !plain) == 0) {
+        const float f1 = __builtin_fmaf(l1r, a1, -b1), f3 = __builtin_fmaf(l3r, a3, b1), f2 = l2 * a2;
+        const float p2 = l2 > zero ? f2 : zero;
+        const float m2 = l2 <= zero ? f2 : zero;
+        float P1 = f3 * l3r, M1 = f1 * l1r;
+        float P2 = __builtin_fmaf(f3, vt, p2 * ct), M2 = __builtin_fmaf(f1, vt, m2 * ct);
+        unrot<AX>(P1, P2);
+        unrot<AX>(M1, M2);
+        P[0] = f3; P[1] = P1; P[2] = P2;
+        M[0] = f1; M[1] = M1; M[2] = M2;
+        return;
+    }
+    float lE1 = zero, l1 = l1r;
+    if (hh1) {
+        const float t = ei1 * (ej1 - l1r) * Mf::rcp(ej1 - ei1);
+        lE1 = l1r - t;
+        l1 = t;
+    }
+    float lE3 = zero, l3 = l3r;
+    if (hh3) {
+        lE3 = l3r - ej3 * (l3r - ei3) * Mf::rcp(ej3 - ei3);
+        l3 = ej3 * (l3r - ei3) * Mf::rcp((ej3 - ei3) + V);
+    }
     rx_upwind_coef<AX>(l1, l2, l3, lE1, lE3, a1, a2, a3, b1, vt, ct, P, M);
 }
 
 template <int AX, typename T, bool FAST>
-__device__ __forceinline__ void rx_edge_d(const HD<T> &I, const HD<T> &J, T dx, T P[3], T M[3]) {
-    if constexpr (FAST) rx_edge_fast<AX>(I, J, dx, P, M);
+__device__ __forceinline__ void rx_edge_d(const HD<T> &I, const HD<T> &J, T dx, T P[3], T M[3], bool care = true) {
+    if constexpr (FAST) rx_edge_fast<AX>(I, J, dx, P, M, care);
     else rx_edge_exact<AX>(I, J, dx, P, M);
 }
 
@@ -415,7 +439,7 @@ template <int AX, typename T, bool FAST>
 __device__ __forceinline__ void edge_or_zero(bool active, const HD<T> &I, const HD<T> &J, T dx, T P[3], T M[3]) {
     const T zero = T(0);
     if constexpr (FAST) {
-        rx_edge_d<AX, T, FAST>(I, J, dx, P, M);
+        rx_edge_d<AX, T, FAST>(I, J, dx, P, M, active);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             P[k] = active ? P[k] : zero;
@@ -733,10 +757,11 @@ __device__ __forceinline__ ED<T> ed_derive(const ER<T> &q, const SedK &sd) {
     const T hf = wet ? d.h : C<T>(DRY_FLOOR);
     if constexpr (FAST) {
         using Mf = Mth<float, true>;
-        const float ih = Mf::rcp(hf);
+        const float r = Mf::rsq(hf);  // 1 / sqrt(hf): gives 1 / hf and sqrt(hf)
+        const float ih = r * r;
         d.u = wet ? d.hu * ih : zero;
         d.v = wet ? d.hv * ih : zero;
-        d.s = Mf::sqrt(hf);
+        d.s = hf * r;
         d.G = sd.mpm ? mpm_G_fast(d.h, d.u * d.u + d.v * d.v, nod, sd) : const_G<T>(d.h, q.G, nod);
     } else {
         d.G = sd.mpm ? mpm_G<T>(d.h, d.hu, d.hv, nod, sd) : const_G<T>(d.h, q.G, nod);
@@ -788,7 +813,13 @@ __device__ __forceinline__ T ex_edge_d(const ED<T> &I, const ED<T> &J) {
     const T dz = J.zbf - I.zbf;
     const T dq = gt * umj * uhj - gt * umi * uhi;
     bool up;  // lambda_4 >= 0
-    if (d_abs(dz) > S_) {
+    if constexpr (FAST) {
+        // the signs of the quotients, without dividing: dz is nonzero in the
+        // first form (|dz| > SED_TOL), s_i + s_j > 0 in the second; a zero
+        // numerator gives lambda_4 = 0 >= 0 and a NaN one lambda_4 = NaN < 0
+        const float num = uhi * I.s + uhj * J.s;
+        up = d_abs(dz) > S_ ? (dq == zero || (dz > zero ? dq > zero : dq < zero)) : num >= zero;
+    } else if (d_abs(dz) > S_) {
         up = Mx::div(dq, dz) >= zero;
     } else {
         up = Mx::div(uhi * I.s + uhj * J.s, I.s + J.s) >= zero;
@@ -839,10 +870,11 @@ __device__ __forceinline__ DD<T> dd_make(T h, T hu, T hv, T G, int nod) {
     d.hf = pmax(h, C<T>(DRY_TOL));
     if constexpr (FAST) {
         using Mf = Mth<float, true>;
-        const float ih = Mf::rcp(d.hf);
+        const float r = Mf::rsq(d.hf);  // 1 / sqrt(hf): gives 1 / hf and sqrt(hf)
+        const float ih = r * r;
         d.u = hu * ih;
         d.v = hv * ih;
-        d.s = Mf::sqrt(d.hf);
+        d.s = d.hf * r;
     } else {
         d.u = hu / d.hf;
         d.v = hv / d.hf;
@@ -924,9 +956,11 @@ __device__ __forceinline__ T dt_edge_d(const DD<T> &I, const DD<T> &J, T dx) {
     T ut, ct, at;
     if constexpr (FAST) {
         using Mf = Mth<float, true>;
-        ut = (uhi * I.s + uhj * J.s) * Mf::rcp(I.s + J.s);
+        const float sum = I.s + J.s, prod = I.s * J.s;
+        const float R = Mf::rcp(sum * prod);  // 1 / (s_i + s_j) = R s_i s_j, 1 / (s_i s_j) = R (s_i + s_j)
+        ut = (uhi * I.s + uhj * J.s) * (R * prod);
         ct = Mf::sqrt(g * ht);
-        at = gt * S * Mf::rcp(I.s * J.s);
+        at = gt * S * (R * sum);
     } else {
         ut = (uhi * I.s + uhj * J.s) / (I.s + J.s);
         ct = d_sqrt(g * ht);
