@@ -42,6 +42,14 @@ ALG_BYTES = {
 }
 PER_STEP = {"stage_sweep": 1, "stage_sweep_combine_dt": 1, "compute_dt": 0}
 HBM_PEAK_GBS = 8000.0
+# The same kernels under the byte model of the hot-path survey (fields 4 B,
+# mask 1 B; G recomputed, not stored; plain float32 bed): stage 1 reads h hu hv
+# z_b z n_b + mask and writes h hu hv z_b (41 B); stage 2 + combine also reads
+# u^n's h hu hv z_b (57 B); 98 B per SSPRK2 cell-step.  The roofline fraction
+# is quoted on it; the bytes this build moves (G stored because the reference's
+# SSPRK2 averages it and the next dt reads the average, compensated bed) are
+# ALG_BYTES above.
+SURVEY_BYTES = {"stage_sweep": 41, "stage_sweep_combine_dt": 57, "compute_dt": 13}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ssprk2_radial_dambreak_8192_fp32.txt")
@@ -85,6 +93,10 @@ def parse():
                    help="also run the timed steps in float64 and report the float32 result's deviation (1) or not (0)")
     p.add_argument("--tile-rows", type=int, default=0,
                    help="strip height of the tiled run's fused stage (0: tuned with --autotune 1, else 32)")
+    p.add_argument("--active-steps", type=int, default=20,
+                   help="timed steps of an 8192^2 channel in which every cell moves and carries sediment (0: skip)")
+    p.add_argument("--driver-steps", type=int, default=1000,
+                   help="about this many steps of run_driver's SSPRK2 loop on the 2048x512 channel (0: skip)")
     p.add_argument("--autotune", type=int, default=1,
                    help="choose the fused stage's strip height by timing the candidates (1) or keep 32 (0)")
     return p.parse_args()
@@ -239,6 +251,88 @@ def ensemble_run(a, torch):
             "strip_rows": rows, "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}}
 
 
+def active_run(a, n, torch):
+    """The same step on an n x n channel (dx 0.02 m, h 0.1 m, u 0.5 m/s,
+    256 seeded bumps, MPM sediment, Transmissive in/out, Reflective walls):
+    every cell moves and carries sediment, so no still-water shortcut
+    applies.  hipGraph replay, autotuned strip height."""
+    from pyexner_amd import cases
+    from pyexner_amd.autotune import tune_strip_rows
+    from pyexner_amd.runtime.driver import setup_case
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = cases.channel(nx=n, ny=n, dx=0.02, n_bumps=256, seed=3)
+    state, bm, solver, sc, integ, ic = setup_case(mesh, arr, params, dtype=torch.float32)
+    del arr
+    mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
+    state = solver.init_fn(state, mask, sc)
+    rows, _ = tune_strip_rows(sc.backend, state, mask, params["cfl"], steps=3, use_cache=False)
+    st = ExnerStepper(sc.backend, state, mask, params["cfl"])
+    del state
+    moving = float((st.a.hu != 0).float().mean())
+    st.capture(1, "SSPRK2")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.active_steps):
+        st.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.active_steps
+    sediment = float((st.a.G > 0).float().mean())
+    if not torch.isfinite(st.a.h).all():
+        raise RuntimeError("non-finite depth in the active-flow run")
+    log(f"active flow {n}^2: {ms:.3f} ms/step, {n * n / (ms * 1e-3) / 1e9:.2f} G cell-steps/s "
+        f"(moving {moving:.3f}, G > 0 in {sediment:.3f} of the cells)")
+    return {"workload": f"channel {n}x{n}, dx 0.02, h 0.1, u 0.5, 256 bumps", "ms_per_step": ms,
+            "cell_steps_per_s": n * n / (ms * 1e-3), "steps": a.active_steps, "strip_rows": rows,
+            "moving_fraction": moving, "sediment_fraction": sediment}
+
+
+def driver_run(a, torch):
+    """run_driver's SSPRK2 loop (integrators.ssprk2.run_fn_SSPRK2) on the
+    2048 x 512 channel, float32, about a.driver_steps steps in 5 output
+    windows without io: the ExnerStepper loop (hipGraph steps, dt read back
+    one step behind), the functional loop (step_fn twice and the combine per
+    step, dt read every step), and the stepper's bare graph replays."""
+    from pyexner_amd import cases
+    from pyexner_amd.integrators import ssprk2
+    from pyexner_amd.runtime.driver import run_case, setup_case
+    from pyexner_amd.stepper import ExnerStepper
+    mesh, arr, params = cases.channel()
+    ny, nx = mesh.local_X.shape
+    state, bm, solver, sc, integ, ic = setup_case(mesh, arr, params, dtype=torch.float32)
+    mask = solver.mask_fn(state, sc.mpi_handler, bm.boundary_mask)
+    state = solver.init_fn(state, mask, sc)
+    dt0 = float(solver.compute_dt_fn(state, params["cfl"], mask, sc))
+    end = round(a.driver_steps * dt0, 3)
+    out = {"workload": f"channel {nx}x{ny}, dx 0.02, SSPRK2, end_time {end} s, out_freq {end / 5:.4f} s"}
+    for key, opts in (("stepper_loop", {}), ("functional_loop", {"stepper_loop": False})):
+        p = dict(params, integrator="SSPRK2", end_time=end, out_freq=end / 5, backend=opts)
+        run_case(mesh, arr, dict(p, end_time=end / 20, out_freq=end / 20), dtype=torch.float32)  # warm-up
+        run_case(mesh, arr, p, dtype=torch.float32)
+        r = dict(ssprk2.last_run)
+        out[key] = {"iters": r["iters"], "wall_s": r["wall"], "cell_steps_per_s": nx * ny * r["iters"] / r["wall"],
+                    "ms_per_step": r["wall"] * 1e3 / r["iters"]}
+        log(f"run_driver {key}: {r['iters']} steps in {r['wall']:.3f} s, "
+            f"{out[key]['ms_per_step']:.4f} ms/step")
+    from pyexner_amd.autotune import tune_strip_rows
+    rows, _ = tune_strip_rows(sc.backend, state, mask, params["cfl"])  # the height run_fn chose (cached)
+    out["strip_rows"] = rows
+    st = ExnerStepper(sc.backend, state, mask, params["cfl"])
+    st.capture(1, "SSPRK2")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(200):
+        st.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 200
+    out["graph_replay_ms_per_step"] = ms
+    log(f"stepper graph replay on the same grid: {ms:.4f} ms/step")
+    return out
+
+
 def accuracy_run(a, n, f32_end, torch):
     """The timed run's float32 state against the same a.steps SSPRK2 steps in
     float64 (which matches the float64 oracle to 1e-12): max-norm and L1
@@ -325,6 +419,23 @@ def adjoint_run(a, torch):
     torch.cuda.synchronize()
     ms_fwd = e0.elapsed_time(e1) / a.adj_steps
     ms_bwd = tb * 1e3 / a.adj_steps
+    # the float32 forward step of the same case (the dtype the calibration
+    # config names), strip height tuned, hipGraph replay
+    from pyexner_amd.autotune import tune_strip_rows
+    from pyexner_amd.runtime.driver import setup_case
+    s32, bm32, sol32, sc32, _, _ = setup_case(mesh, arr, params, dtype=torch.float32)
+    m32 = sol32.mask_fn(s32, sc32.mpi_handler, bm32.boundary_mask)
+    s32 = sol32.init_fn(s32, m32, sc32)
+    rows32, _ = tune_strip_rows(sc32.backend, s32, m32, run.cfl, steps=5, use_cache=False)
+    st32 = ExnerStepper(sc32.backend, s32, m32, run.cfl)
+    st32.capture(1, "SSPRK2")
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(a.adj_steps):
+        st32.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fwd32 = e0.elapsed_time(e1) / a.adj_steps
     if not torch.isfinite(g).all():
         raise RuntimeError("non-finite bed gradient")
     log(f"adjoint: {a.adj_steps} steps 1024x256 f64: forward {tf * 1e3:.1f} ms, backward {tb * 1e3:.1f} ms; "
@@ -333,6 +444,8 @@ def adjoint_run(a, torch):
             "dtype": "f64", "steps": a.adj_steps, "checkpoint_every": a.adj_every,
             "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "plain_forward_ms_per_step": ms_fwd,
             "backward_ms_per_step": ms_bwd, "adjoint_step_vs_forward_step": ms_bwd / ms_fwd,
+            "f32_forward_ms_per_step": ms_fwd32, "f32_forward_strip_rows": rows32,
+            "adjoint_step_vs_f32_forward_step": ms_bwd / ms_fwd32,
             "loss": loss, "grad_zb0_norm": float(g.norm()), "grad_d50": gd, "grad_porosity": gp}
 
 
@@ -466,12 +579,17 @@ def main():
     step_us = sum(x.elapsed_time(y) for x, y in steps_span) * 1e3 / len(steps_span)
     log(f"  instrumented step      {step_us:9.1f} us  (frame sweeps, ring fills and events included)")
     dom = max(kern, key=lambda k: kern[k]["us"] * PER_STEP[k])
-    achieved = kern[dom]["alg_GBs"]
+    achieved = round(SURVEY_BYTES[dom] * n * n / (kern[dom]["us"] * 1e-6) / 1e9, 1)
     traffic = measured_traffic(n, dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "alg_bytes_per_cell": ALG_BYTES[dom], "cells_per_launch": n * n}
+                "alg_bytes_per_cell": SURVEY_BYTES[dom], "cells_per_launch": n * n,
+                "byte_model": "hot-path survey: G recomputed, plain float32 bed"}
+    roofline_stored = {"achieved": kern[dom]["alg_GBs"], "frac": round(kern[dom]["alg_GBs"] / HBM_PEAK_GBS, 4),
+                       "alg_bytes_per_cell": ALG_BYTES[dom],
+                       "byte_model": "bytes this build moves: G stored, compensated bed z_b + z_b_lo"}
     step_bytes = sum(ALG_BYTES[k] * c for k, c in PER_STEP.items())
+    step_survey = sum(SURVEY_BYTES[k] * c for k, c in PER_STEP.items())
 
     tile = None
     if a.tile_steps > 0 and n % 4 == 0:
@@ -480,6 +598,15 @@ def main():
     if a.ens_members > 0 and a.ens_steps > 0:
         torch.cuda.empty_cache()
         ens = ensemble_run(a, torch)
+
+    active = None
+    if a.active_steps > 0:
+        torch.cuda.empty_cache()
+        active = active_run(a, n, torch)
+    drv = None
+    if a.driver_steps > 0:
+        torch.cuda.empty_cache()
+        drv = driver_run(a, torch)
 
     acc = None
     if f32_end is not None:
@@ -515,10 +642,15 @@ def main():
                    "steps_from_initial_state": a.steps, "hipgraph": use_graph,
                    "stage": "fused (frame-local ring)", "strip_rows": rows,
                    "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}, "bed": "float32 z_b + z_b_lo (compensated)", "math": "float32 hardware rcp/rsq/log2/exp2",
-                   "step_alg_bytes_per_cell": step_bytes, "step_alg_GBs": round(step_bytes * value / 1e9, 1)},
+                   "step_alg_bytes_per_cell": step_bytes, "step_alg_GBs": round(step_bytes * value / 1e9, 1),
+                   "step_survey_bytes_per_cell": step_survey,
+                   "step_survey_frac": round(step_survey * value / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roofline,
+        "roofline_stored_bytes": roofline_stored,
         "kernels": kern,
         "cpu_baseline": cpu,
+        "active_flow": active,
+        "run_driver_channel": drv,
         "tiles_4x4": tile,
         "ensemble": ens,
         "adjoint": adj,

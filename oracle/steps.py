@@ -7,6 +7,8 @@
 * step_roe / dt_roe  - solvers/roe_solver.py:40-60 (as intended, see roe_solver.py:30,40-41)
 * fe_step / ssprk2_step - integrators/forwardeuler.py:37-49, integrators/SSPRK2.py:32-40
 * run_forward_euler  - integrators/forwardeuler.py:51-117 (output-window loop with probe steps)
+* run_ssprk2         - integrators/SSPRK2.py:53-97 (output-window loop; the closing step is an
+                       SSPRK2 step of the clipped dt, not the single step_fn call at :87)
 """
 from dataclasses import fields
 
@@ -147,3 +149,34 @@ def run_forward_euler(prob, s, end_time, out_freq, on_output=None):
         time = dtype(time + dt)
         iters += 1
     return s, float(time), iters
+
+
+def run_ssprk2(prob, s, end_time, out_freq, on_output=None):
+    """SSPRK2.py:53-97 with the device while-loop (cond_fn :23-28, body_fn
+    :32-49) run on the host.  The window-closing step (:79-87) takes the CFL
+    dt clipped to the output time and is a whole SSPRK2 step; a clipped step
+    is reported to on_output (the reference writes no output).  Time is
+    accumulated in Python float."""
+    time, dt, iters = 0.0, 0.0, 0
+    while time < end_time - TIMESTEP_TOL:
+        while True:
+            target = min((int(np.int32(time / out_freq)) + 1) * out_freq, end_time)
+            if not (time + dt < target - TIMESTEP_TOL):
+                break
+            dt = float(prob.dt(s))
+            s1 = prob.step(s, dt, time)
+            s = combine_half(s, prob.step(s1, dt, time))
+            time = time + dt
+            iters += 1
+        target = min((int(time / out_freq) + 1) * out_freq, end_time)
+        dt = float(prob.dt(s))
+        clipped = time + dt >= target - TIMESTEP_TOL
+        if clipped:
+            dt = target - time
+        s1 = prob.step(s, dt, time)
+        s = combine_half(s, prob.step(s1, dt, time))
+        if clipped and on_output is not None:
+            on_output(target, s)
+        time = time + dt
+        iters += 1
+    return s, time, iters

@@ -19,7 +19,10 @@ import torch
 
 from . import _hip
 
-CANDIDATES = (16, 24, 32, 48, 64)
+# 8 and 12 rows give grids of about a million cells or fewer enough
+# wavefronts to fill the 1024 SIMDs (2048 x 512 has 35 strip columns); taller
+# strips recompute fewer halo rows on large grids
+CANDIDATES = (8, 12, 16, 24, 32, 48, 64)
 _CACHE = {}
 
 

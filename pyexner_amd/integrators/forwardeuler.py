@@ -11,7 +11,8 @@ import time as _time
 import torch
 
 from ..constants import TIMESTEP_TOL
-from .loop import DtReader, next_target
+from ..solvers.roe_exner_solver import ExnerBackend
+from .loop import DtReader, next_target, run_exner_stepper
 from .registry import IntegratorBundle, IntegratorConfig, register_integrator_bundle
 
 last_run = {}
@@ -28,6 +29,13 @@ def run_fn_forwardeuler(state, config, io=None, mesh=None, b_mask=None):
     state = sb.init_fn(state, mask, sc)
     if io is not None:
         io.write_state(state, mesh, mask[0])
+    be = getattr(sc, "backend", None)
+    if isinstance(be, ExnerBackend) and getattr(be, "stepper_loop", True):
+        a = _time.perf_counter()
+        state, iters, time = run_exner_stepper(state, mask, be, config, io, mesh, "Forward Euler",
+                                                 autotune=getattr(be, "autotune", None))
+        last_run.update(time=time, iters=iters, wall=_time.perf_counter() - a, path="stepper")
+        return state
     rd = DtReader(state.h.dtype, state.h.device)
     time, dt, iters = 0.0, 0.0, 0
     a = _time.perf_counter()
@@ -51,7 +59,7 @@ def run_fn_forwardeuler(state, config, io=None, mesh=None, b_mask=None):
         time = time + dt
         iters += 1
     torch.cuda.synchronize()
-    last_run.update(time=time, iters=iters, dt=dt, wall=_time.perf_counter() - a)
+    last_run.update(time=time, iters=iters, dt=dt, wall=_time.perf_counter() - a, path="functional")
     return state
 
 

@@ -33,3 +33,177 @@ class DtReader:
 def next_target(time, out_freq, end_time):
     """forwardeuler.py:33-34: min((int(time/out_freq) + 1) * out_freq, end_time)."""
     return min((int(np.int32(time / out_freq)) + 1) * out_freq, end_time)
+
+
+class DtRing:
+    """The dt each step of an ExnerStepper leaves on the device (the fused
+    reduction's next dt), published to the host without stalling the step
+    stream: a 4-byte copy into a device ring slot on the step stream (part of
+    the step's hipGraph), then a copy of that slot into pinned host memory on
+    a side stream.  A slot is reused only after its host copy has finished."""
+
+    def __init__(self, dt_dev, n=8):
+        self.dt = dt_dev
+        self.n = n
+        self.dev = torch.zeros(n, dtype=dt_dev.dtype, device=dt_dev.device)
+        self.host = torch.zeros(n, dtype=dt_dev.dtype, pin_memory=True)
+        self.side = torch.cuda.Stream(device=dt_dev.device)
+        self.ready = [torch.cuda.Event() for _ in range(n)]
+        self.done = [torch.cuda.Event() for _ in range(n)]
+        self.used = [False] * n
+
+    def before(self, k):
+        """Call on the step stream before the step whose dt goes to slot k:
+        the slot's previous host copy must be complete."""
+        s = k % self.n
+        if self.used[s]:
+            torch.cuda.current_stream().wait_event(self.done[s])
+
+    def stage(self, k):
+        """The device part of publishing: copy dt into slot k (captured into a
+        step's graph, or issued eagerly after the step)."""
+        self.dev[k % self.n].copy_(self.dt)
+
+    def publish(self, k):
+        """The host part, after slot k was staged on the step stream."""
+        s = k % self.n
+        self.ready[s].record(torch.cuda.current_stream())
+        with torch.cuda.stream(self.side):
+            self.side.wait_event(self.ready[s])
+            self.host[s].copy_(self.dev[s], non_blocking=True)
+            self.done[s].record(self.side)
+        self.used[s] = True
+
+    def get(self, k):
+        """The dt published from slot k (waits for its copy)."""
+        s = k % self.n
+        self.done[s].synchronize()
+        return float(self.host[s])
+
+
+def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs=True, autotune=None):
+    """The window loop of integrators/SSPRK2.py:53-97 (or forwardeuler.py:51-117)
+    on an ExnerStepper: preallocated buffers, the fused SSPRK2 entry with its
+    next dt, the autotuned strip height, and each regular step replayed from a
+    hipGraph.
+
+    The loop keeps the reference's decisions: a step is taken with the CFL dt
+    while time + (the previous step's dt) < next output time - TIMESTEP_TOL,
+    then one window-closing step whose dt is clipped to the output time
+    (SSPRK2.py:82-87; Forward Euler writes a probe step of the clipped
+    length instead and commits the CFL step, forwardeuler.py:91-117).  That
+    decision for step j needs the dts of the steps before it only, and the
+    dt of step j - 1 comes from step j - 2's fused reduction, so the host
+    decides step j + 1 while step j runs: it waits only for the step before,
+    and the GPU always has the next step queued.  Only the window-closing
+    step waits for its own input (its dt is the clipped CFL dt of the state
+    before it).  Returns (state, iters, time)."""
+    from ..autotune import tune_strip_rows
+    from ..constants import TIMESTEP_TOL
+    from ..stepper import ExnerStepper
+    fe = integrator == "Forward Euler"
+    if autotune is None:
+        autotune = state.h.numel() >= (1 << 20)
+    if autotune and not fe:
+        tune_strip_rows(backend, state, mask, config.cfl)
+    st = ExnerStepper(backend, state, mask, config.cfl)
+    bm = backend.boundaries
+    moving = bm is not None and bm.time_dependent
+    ring = DtRing(st.dt)
+    use_graphs = graphs and not moving and not fe
+    g = None
+    if use_graphs:
+        st.compute_dt()
+        keep = st.a.clone()
+        keep_dt = st.dt.clone()
+        # one eager step on a side stream before capture (library set-up),
+        # then the state and dt restored
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            st.ssprk2_step()
+            ring.stage(0)
+        torch.cuda.current_stream().wait_stream(s)
+        for name in st.a.field_names():
+            getattr(st.a, name).copy_(getattr(keep, name))
+        st.dt.copy_(keep_dt)
+        # graph k: one SSPRK2 step and the copy of its next dt into slot k
+        g = []
+        for k in range(ring.n):
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr):
+                st.ssprk2_step()
+                ring.stage(k)
+            g.append(gr)
+        torch.cuda.synchronize()
+        del keep, keep_dt
+
+    def fe_dt(j):
+        # Forward Euler: the step's dt is compute_dt_fn of its input
+        st.compute_dt()
+        ring.before(j)
+        ring.stage(j)
+        ring.publish(j)
+
+    def step(j, t):
+        """Launch step j (its dt is in st.dt); publish the dt it leaves."""
+        if fe:
+            st.be.step_into(st.fa, st.fb, st.m0, st.dt.data_ptr(), st._stream(), time=t)
+            st.a, st.b = st.b, st.a
+            st.fa, st.fb = st.fb, st.fa
+            return
+        ring.before(j)
+        if g is not None:
+            g[j % ring.n].replay()
+        else:
+            st.ssprk2_step(time=t)
+            ring.stage(j)
+        ring.publish(j)
+
+    if fe:
+        fe_dt(0)
+    else:
+        st.compute_dt()
+        ring.before(0)
+        ring.stage(0)
+        ring.publish(0)
+    # slot j holds the dt of step j + 1 (SSPRK2: left by step j; FE: computed
+    # before step j + 1)
+    time, d_prev, j = 0.0, 0.0, 0
+    end, of = config.end_time, config.out_freq
+    while time < end - TIMESTEP_TOL:
+        while time + d_prev < next_target(time, of, end) - TIMESTEP_TOL:
+            j += 1
+            if fe and j > 1:
+                fe_dt(j - 1)
+            d = ring.get(j - 1)  # the dt of step j (waits for step j - 1 only)
+            step(j, time)
+            time, d_prev = time + d, d
+        # the window-closing step: the CFL dt of the current state, clipped
+        target = next_target(time, of, end)
+        if fe:
+            fe_dt(j)
+        d = ring.get(j)
+        j += 1
+        if time + d >= target - TIMESTEP_TOL:
+            dc = target - time
+            if fe:
+                if io is not None:
+                    probe = st.b
+                    keep_dt = st.dt.clone()
+                    st.dt.fill_(dc)
+                    st.be.step_into(st.fa, st.fb, st.m0, st.dt.data_ptr(), st._stream(), time=time)
+                    io.write_state(probe, mesh, mask[0])
+                    st.dt.copy_(keep_dt)
+                step(j, time)
+            else:
+                st.dt.fill_(dc)
+                step(j, time)
+                d = dc
+                if io is not None:
+                    io.write_state(st.state, mesh, mask[0])
+        else:
+            step(j, time)
+        time, d_prev = time + d, d
+    torch.cuda.synchronize()
+    return st.state, j, time

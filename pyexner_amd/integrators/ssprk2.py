@@ -6,15 +6,18 @@ calls the bundles with the wrong arity at :33,:36-37).  The step is its
 documented intent (SSPRK2.py:32-40): one dt from u^n, u1 = S(u^n),
 u2 = S(u1), u^{n+1} = 0.5 u^n + 0.5 u2 over every field.  The window loop
 follows SSPRK2.py:53-97 with one change: the window-closing step, clipped to
-the output time (SSPRK2.py:222-225), is an SSPRK2 step rather than the single
-step_fn call at SSPRK2.py:227.
+the output time (SSPRK2.py:82-85), is an SSPRK2 step rather than the single
+step_fn call at SSPRK2.py:87.  The state is written at t = 0 and after each
+window-closing step that reaches its output time, as the Forward-Euler
+run_fn writes it (forwardeuler.py:68-117; the reference's SSPRK2 writes none).
 """
 import time as _time
 
 import torch
 
 from ..constants import TIMESTEP_TOL
-from .loop import DtReader, next_target
+from ..solvers.roe_exner_solver import ExnerBackend
+from .loop import DtReader, next_target, run_exner_stepper
 from .registry import IntegratorBundle, IntegratorConfig, register_integrator_bundle
 
 last_run = {}
@@ -54,6 +57,13 @@ def run_fn_SSPRK2(state, config, io=None, mesh=None, b_mask=None):
     state = sb.init_fn(state, mask, sc)
     if io is not None:
         io.write_state(state, mesh, mask[0])
+    be = getattr(sc, "backend", None)
+    if isinstance(be, ExnerBackend) and getattr(be, "stepper_loop", True):
+        a = _time.perf_counter()
+        state, iters, time = run_exner_stepper(state, mask, be, config, io, mesh, "SSPRK2",
+                                                 autotune=getattr(be, "autotune", None))
+        last_run.update(time=time, iters=iters, wall=_time.perf_counter() - a, path="stepper")
+        return state
     rd = DtReader(state.h.dtype, state.h.device)
     time, dt, iters = 0.0, 0.0, 0
     a = _time.perf_counter()
@@ -78,7 +88,7 @@ def run_fn_SSPRK2(state, config, io=None, mesh=None, b_mask=None):
         time = time + dt
         iters += 1
     torch.cuda.synchronize()
-    last_run.update(time=time, iters=iters, dt=dt, wall=_time.perf_counter() - a)
+    last_run.update(time=time, iters=iters, dt=dt, wall=_time.perf_counter() - a, path="functional")
     return state
 
 

@@ -174,13 +174,18 @@ def get_mask(state, mpi_handler=None, b_mask=None):
 
 def config_fn_roeexner(state, mpi_handler, boundaries, dx, params):
     """roe_exner_solver.py:41-56.  Backend options (not in the reference YAML):
-    ``backend: {compensated_bed: bool, exact_f32: bool, fused_stage: bool}``."""
+    ``backend: {compensated_bed: bool, exact_f32: bool, fused_stage: bool,
+    stepper_loop: bool, autotune: bool}``."""
     grass = params["erosion"]["grass_factor"]
     constant = not isinstance(grass, str)
     opts = params.get("backend", {}) or {}
     backend = ExnerBackend(state.h.shape, state.h.dtype, dx, state.seds, constant, boundaries,
                            device=state.h.device, compensated_bed=opts.get("compensated_bed"),
                            exact_f32=opts.get("exact_f32", False), fused_stage=opts.get("fused_stage", True))
+    # the integrators' run_fn steps an ExnerStepper (preallocated buffers, the
+    # fused SSPRK2 entry, hipGraph replay) unless stepper_loop is false
+    backend.stepper_loop = bool(opts.get("stepper_loop", True))
+    backend.autotune = opts.get("autotune")
     return SolverConfig(mpi_handler=mpi_handler, boundaries=boundaries, dx=dx, halo_exchange=identity_halo,
                         compute_G="constant" if constant else "MPM", compute_n="constant" if constant else "MPM",
                         backend=backend)

@@ -61,3 +61,46 @@ def test_run_driver_writes_output_windows(cuda, tmp_path, integrator):
     alone, _ = run_case(mesh, {k: np.asarray(v, dtype=np.float64) for k, v in arr.items()}, p, dtype=torch.float32)
     for name in ("h", "hu", "hv", "z_b", "G"):
         assert torch.equal(getattr(alone, name), getattr(state, name)), name
+
+
+class _Collect:
+    """An io object that keeps host copies of the states it is given."""
+
+    def __init__(self):
+        self.states = []
+
+    def write_state(self, state, mesh, mask0):
+        self.states.append(state.to_host())
+
+
+@pytest.mark.parametrize("integrator", ["SSPRK2", "Forward Euler"])
+def test_run_fn_stepper_loop_matches_oracle_f64(cuda, integrator):
+    """The integrators' run_fn on an ExnerStepper (preallocated buffers, the
+    fused SSPRK2 entry with hipGraph replay, the dt read back one step behind)
+    against the oracle's window loops (oracle.steps.run_ssprk2 /
+    run_forward_euler), float64: the iteration count, the final time, every
+    output window's state (SSPRK2: after the clipped closing step; Forward
+    Euler: the probe step) and the final state within 1e-12 per field."""
+    from oracle import steps as osteps
+    from pyexner_amd.integrators import forwardeuler, ssprk2
+    from tests.helpers import FIELDS_EXNER, compare, oracle_exner, rel_linf
+    mesh, arr, params = cases.channel(nx=130, ny=40, n_bumps=6, seed=17)
+    end, of = 0.25, 0.1
+    p = dict(params, integrator=integrator, end_time=end, out_freq=of)
+    so, prob = oracle_exner(mesh, arr, p)
+    outs = []
+    run = osteps.run_ssprk2 if integrator == "SSPRK2" else osteps.run_forward_euler
+    so, t_o, it_o = run(prob, so, end, of, lambda t, s: outs.append((t, s)))
+    io = _Collect()
+    sg, _ = run_case(mesh, arr, p, dtype=torch.float64, io=io)
+    mod = ssprk2 if integrator == "SSPRK2" else forwardeuler
+    assert mod.last_run["path"] == "stepper"
+    assert mod.last_run["iters"] == it_o, (mod.last_run, it_o)
+    assert abs(mod.last_run["time"] - float(t_o)) <= 1e-12 * end
+    assert len(io.states) == len(outs) + 1  # t = 0 and one per window
+    for (t, s), got in zip(outs, io.states[1:]):
+        errs = {f: rel_linf(getattr(got, f), getattr(s, f)) for f in ("h", "hu", "hv", "z_b")}
+        assert max(errs.values()) <= 1e-12, (t, errs)
+    errs = compare(sg, so, FIELDS_EXNER)
+    print(f"run_fn {integrator}: {it_o} steps", " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    assert max(errs.values()) <= 1e-12
