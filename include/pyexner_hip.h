@@ -123,6 +123,7 @@ typedef struct {
 #define PXH_BV_OFFSET 3
 #define PXH_BV_OUTFLUX 4
 #define PXH_MAX_VALUES 32
+#define PXH_MAX_LEVELS 16
 typedef struct {
     int32_t kind;                   /* PXH_BV_* */
     int32_t reserved;
@@ -166,6 +167,17 @@ typedef struct {
     int32_t n_values;
     int32_t reserved;
     pxh_bc_value values[PXH_MAX_VALUES];
+    /* Dependency levels: with n_levels > 1 the entries are ordered by level and
+     * level L is entries [level_start[L], level_start[L+1]), applied one level
+     * after another (level_start[n_levels] = n_entries; group_start and
+     * PXH_BC_GROUPED are then not used).  Every entry reads its source as it
+     * was before the whole program, as when the handlers of
+     * domain/boundary_registry.py:183-187 run one after another and each reads
+     * its sources before writing: an entry that reads a cell which another
+     * entry writes sits in an earlier level than that writer.  n_levels 0 or
+     * 1: one level (no entry reads a cell the program writes). */
+    int32_t n_levels;
+    int32_t level_start[PXH_MAX_LEVELS + 1];
 } pxh_bc;
 
 /* ---- library ------------------------------------------------------------ */
@@ -180,8 +192,14 @@ int pxh_roe_exner_init(const pxh_grid *grid, const pxh_fields *s, const uint8_t 
 
 /* One Forward-Euler step_fn: out = S(in, dt).  out->h, hu, hv, z_b, G must not
  * alias in's; z, n, n_b are static and may be shared.  dt is read from the
- * device scalar dt_dev (dtype of the grid).  bc must hold an idempotent program
- * (sources never targets) covering groups h, hu/hv, z_b and G. */
+ * device scalar dt_dev (dtype of the grid).  bc must hold a program of one
+ * dependency level (sources never targets) covering groups h, hu/hv, z_b and
+ * G: the step fills (h, hu, hv) after the hydro update and (z_b, G) after the
+ * bed update, which equals the reference's two fills of every group only for
+ * such programs.  A program of several levels returns PXH_E_UNSUPPORTED
+ * (likewise pxh_roe_exner_ssprk2, pxh_roe_step and the ensemble); its step is
+ * the single-kernel entries below sequenced as roe_exner_solver.py:80-116
+ * does, with pxh_bc_apply of every group at both fills. */
 int pxh_roe_exner_step(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
                        const uint8_t *mask0, const pxh_sediments *sed, const pxh_bc *bc,
                        const void *dt_dev, void *stream);

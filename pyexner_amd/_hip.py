@@ -24,6 +24,7 @@ BC_SRC_SORTED = 4  # pxh_bc.flags: within a group, entries ordered by src (adjoi
 BC_VALUE = 0x80  # pxh_bc_entry.nchain flag: a value entry, chain[0] indexes pxh_bc.values
 BV_CONST, BV_AFFINE_T, BV_OFFSET, BV_OUTFLUX = 1, 2, 3, 4
 MAX_VALUES = 32
+MAX_LEVELS = 16  # PXH_MAX_LEVELS: dependency levels of a ring-fill program
 N_GROUPS = 7
 NOALLOC_SCRATCH_BYTES = 64 << 20  # PXH_NOALLOC_SCRATCH_BYTES
 HALO_W, HALO_N, HALO_E, HALO_S = 1, 2, 4, 8  # pxh_halo_* sides
@@ -98,7 +99,8 @@ class BC(ctypes.Structure):
                 ("group_start", ctypes.c_int32 * (N_GROUPS + 1)),
                 ("handlers", BCHandler * MAX_HANDLERS),
                 ("time", ctypes.c_double), ("n_values", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("values", BCValue * MAX_VALUES)]
+                ("values", BCValue * MAX_VALUES), ("n_levels", ctypes.c_int32),
+                ("level_start", ctypes.c_int32 * (MAX_LEVELS + 1))]
 
 
 assert ctypes.sizeof(BCEntry) == 16

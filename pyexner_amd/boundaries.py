@@ -286,7 +286,9 @@ def compose(handlers, shape):
 
     Returns (entries: structured numpy array of pxh_bc_entry, idempotent: bool,
     values: list of (kind, a, b, c) pxh_bc_value specs the value entries
-    index).  Each (group, target) ends up as one entry: a copy of a source
+    index, level_start: the entries' dependency levels, entries[level_start[L]:
+    level_start[L + 1]] being level L; the entries are ordered by level, then
+    group).  Each (group, target) ends up as one entry: a copy of a source
     cell through a chain of momentum transforms, or a value (a constant, an
     affine function of time, an offset of a source cell, or the outflux depth
     of a source cell).  A program is idempotent when no entry reads a cell
@@ -376,6 +378,12 @@ def compose(handlers, shape):
             idempotent = False
     if len(values) > _hip.MAX_VALUES:
         raise NotImplementedError(f"more than {_hip.MAX_VALUES} boundary values")
+    level = dependency_levels(rows, values)
+    if level and max(level) + 1 > _hip.MAX_LEVELS:
+        raise NotImplementedError(f"a ring fill of more than {_hip.MAX_LEVELS} dependency levels")
+    order = sorted(range(len(rows)), key=lambda q: (level[q], rows[q][2]))
+    rows = [rows[q] for q in order]
+    level = [level[q] for q in order]
     ent = np.zeros(len(rows), dtype=ENTRY_DTYPE)
     for q, (t, s, grp, ch, vi) in enumerate(rows):
         ent[q]["target"], ent[q]["src"], ent[q]["group"] = t, s, grp
@@ -386,7 +394,42 @@ def compose(handlers, shape):
             ent[q]["nchain"] = _hip.BC_VALUE
             ent[q]["chain"][0] = vi
     assert ent.itemsize == 16
-    return ent, idempotent, values
+    n_levels = (max(level) + 1) if level else 1
+    level_start = [int(v) for v in np.searchsorted(np.asarray(level, dtype=np.int64), np.arange(n_levels + 1))]
+    return ent, n_levels == 1, values, level_start
+
+
+def dependency_levels(rows, values):
+    """The level of each composed entry (t, src, group, chain, value index):
+    an entry that reads a (cell, group) which another entry writes runs in an
+    earlier level than that writer, so that every entry reads its source as
+    it was before the program, as the sequential handlers of
+    boundary_registry.py:183-187 read theirs (each handler gathers its sources
+    before it writes).  Copies read their source's group, an offset value its
+    source's group, an outflux depth its source's h, hu and hv."""
+    writer = {(r[0], r[2]): q for q, r in enumerate(rows)}
+    reads = []
+    for t, src, grp, ch, vi in rows:
+        if vi is None:
+            reads.append(((src, grp),))
+        elif values[vi][0] == _hip.BV_OFFSET:
+            reads.append(((src, grp),))
+        elif values[vi][0] == _hip.BV_OUTFLUX:
+            reads.append(((src, G.G_H), (src, G.G_HUV)))
+        else:
+            reads.append(())
+    level = [0] * len(rows)
+    for _ in range(len(rows) + 1):
+        changed = False
+        for q, rd in enumerate(reads):
+            for key in rd:
+                w = writer.get(key)
+                if w is not None and w != q and level[w] < level[q] + 1:
+                    level[w] = level[q] + 1
+                    changed = True
+        if not changed:
+            return level
+    raise NotImplementedError("a ring fill whose entries read each other's targets in a cycle")
 
 
 def frame_local(entries, shape):
@@ -438,7 +481,8 @@ class BoundaryManager:
         self.boundary_mask = np.any(np.stack(masks), axis=0) if masks else np.zeros(shape, dtype=bool)
         if len(self.boundary_handlers) > _hip.MAX_HANDLERS:
             raise NotImplementedError(f"more than {_hip.MAX_HANDLERS} boundary handlers")
-        entries, self.idempotent, self.values = compose(self.boundary_handlers, shape)
+        entries, self.idempotent, self.values, self.level_start = compose(self.boundary_handlers, shape)
+        self.n_levels = len(self.level_start) - 1
         self.time_dependent = any(v[0] == _hip.BV_AFFINE_T for v in self.values)
         self.entries = entries
         self.n_entries = len(entries)
@@ -447,8 +491,11 @@ class BoundaryManager:
         self._entries_dev = torch.from_numpy(entries.view(np.uint8).copy()).to(device) if len(entries) else None
         self.groups_present = sorted({int(g) for g in entries["group"]}) if len(entries) else []
         grp = entries["group"].astype(np.int64)
-        assert np.all(np.diff(grp) >= 0), "compose() emits the entries group by group"
-        self.group_start = [int(v) for v in np.searchsorted(grp, np.arange(_hip.N_GROUPS + 1))]
+        if self.n_levels == 1:
+            assert np.all(np.diff(grp) >= 0), "compose() emits one level's entries group by group"
+            self.group_start = [int(v) for v in np.searchsorted(grp, np.arange(_hip.N_GROUPS + 1))]
+        else:
+            self.group_start = None
 
     def program(self, groups=None):
         """pxh_bc struct restricted to the given groups (default: all)."""
@@ -460,9 +507,15 @@ class BoundaryManager:
         for g_ in (range(7) if groups is None else groups):
             mask |= 1 << g_
         bc.group_mask = mask
-        bc.flags = (_hip.BC_FRAME if self.frame_local else 0) | _hip.BC_GROUPED
-        for g_ in range(_hip.N_GROUPS + 1):
-            bc.group_start[g_] = self.group_start[g_]
+        bc.flags = _hip.BC_FRAME if self.frame_local else 0
+        if self.n_levels == 1:
+            bc.flags |= _hip.BC_GROUPED
+            for g_ in range(_hip.N_GROUPS + 1):
+                bc.group_start[g_] = self.group_start[g_]
+        else:
+            bc.n_levels = self.n_levels
+            for k, v in enumerate(self.level_start):
+                bc.level_start[k] = v
         for k, hd in enumerate(self.boundary_handlers):
             bc.handlers[k].kind = hd.kind
             bc.handlers[k].nx = float(hd.normal[0])
@@ -475,6 +528,8 @@ class BoundaryManager:
         sorted by (group, src), stably, so that the targets filled from one
         source are adjacent and their adjoints are summed in a fixed order
         (PXH_BC_SRC_SORTED)."""
+        if self.n_levels > 1:
+            raise NotImplementedError("the adjoint ring fill needs a program of one dependency level")
         if not hasattr(self, "_adj_entries_dev"):
             ent = self.entries
             order = np.lexsort((ent["src"], ent["group"])) if len(ent) else np.zeros(0, dtype=np.int64)

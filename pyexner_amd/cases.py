@@ -266,3 +266,49 @@ def nodata_island(nx=96, ny=40, dx=0.02, seed=13):
     z[r0:r0 + ny // 4, c0:c0 + nx // 6] = np.nan
     arrays = dict(arrays, z=z)
     return mesh, arrays, params
+
+
+def berthon_convergence(dh=0.05, end_time=10.0, out_freq=1.0, integrator="Forward Euler"):
+    """The reference's Berthon convergence case as its own inputs give it
+    (tests/runtime/2D_const_flux/2D_build_num_sol.py:8-56 and
+    dh_*/input_*.yaml): x = 0..7 and y = 1..0 in steps of dh, stored as
+    float32 file coordinates; q = 1, A = alpha = beta = 0.005, gamma = 1,
+    constant G = 0.005; Reflective bottom and top polygons two rows of cell
+    centres thick (y within dh of 0 and of 1), Berthon bed and ConstantInflux
+    at the inlet column, NormalFlowDepth at the outlet.  The thick polygons
+    make a ring fill that reads cells it writes: the handlers apply one
+    after another (boundary_registry.py:183-187), which the composed program
+    keeps with two dependency levels."""
+    from .mesh import mesh_from_coords
+    x = np.arange(0.0, 7.0 + dh, dh)
+    y = np.arange(1.0, 0.0 - dh, -dh)
+    X, _ = np.meshgrid(x, y, indexing="xy")
+    A = alpha = beta = 0.005
+    gamma, q0, g = 1.0, 1.0, 9.81
+    u = ((alpha * X + beta) / A) ** (1.0 / 3.0)
+    q = q0 * np.ones_like(X)
+    h = q / u
+    zb = -(u ** 3 + 2 * g * q) / (2 * g * u) + gamma
+    arrays = {"h": h, "hu": h * u, "hv": np.zeros_like(h), "z": np.zeros_like(h), "z_b": zb,
+              "n": np.zeros_like(h)}
+    mesh = mesh_from_coords(x.astype(np.float32), y.astype(np.float32))
+    d = float(dh)
+    top, right = 1.0, 7.0 + d
+    prof = [1.0, 0.005, 0.005, 0.005, 1]
+    bounds = {
+        "bottom": {"type": "Reflective", "polygon": [[0.0, -d], [right, -d], [right, d], [0.0, d]],
+                   "values": [0, 0, 0, 0], "normal": [0.0, -1.0]},
+        "inlet_bed": {"type": "Berthon", "polygon": [[0.0, 0.0], [d, 0.0], [d, top], [0.0, top]],
+                      "values": prof + [d / 2], "normal": [-1.0, 0.0]},
+        "inlet_flux": {"type": "ConstantInflux", "polygon": [[0.0, 0.0], [d, 0.0], [d, top], [0.0, top]],
+                       "values": [1.0, 1.0, 0.0], "normal": [-1.0, 0.0]},
+        "outlet_bed": {"type": "NormalFlowDepth",
+                       "polygon": [[7.0 - d, 0.0], [right, 0.0], [right, top + d], [7.0 - d, top + d]],
+                       "values": prof + [7.0 - d / 2], "normal": [1.0, 0.0]},
+        "top": {"type": "Reflective", "polygon": [[0.0, top + d], [right, top + d], [right, top - d], [0.0, top - d]],
+                "values": [0, 0, 0, 0], "normal": [0.0, 1.0]},
+    }
+    params = {"flux_scheme": "Roe Exner", "integrator": integrator, "cfl": 0.5, "end_time": end_time,
+              "out_freq": out_freq, "parNx": 1, "parNy": 1, "erosion": {"grass_factor": 0.005},
+              "boundaries": bounds}
+    return mesh, arrays, params

@@ -50,9 +50,12 @@ inline AdjWs ws_carve(int ny, int nx, void *ws) {
 
 inline bool adj_grid_ok(const pxh_grid *g) { return grid_ok(g) && g->dtype == PXH_F64; }
 
-// The outflux depth (a Newton solve from the source cell) has no adjoint here.
+// The outflux depth (a Newton solve from the source cell) has no adjoint here,
+// nor has a program of several dependency levels (the transposed fill runs as
+// one level).
 inline bool bc_adj_ok(const pxh_bc *bc) {
     if (!bc) return true;
+    if (bc->n_levels > 1) return false;
     for (int k = 0; k < bc->n_values; ++k)
         if (bc->values[k].kind == PXH_BV_OUTFLUX) return false;
     return true;

@@ -227,7 +227,12 @@ static inline bool bc_ok(const pxh_bc *bc) {
     if (bc->n_handlers < 0 || bc->n_handlers > PXH_MAX_HANDLERS || bc->n_entries < 0) return false;
     if (bc->n_values < 0 || bc->n_values > PXH_MAX_VALUES) return false;
     if (bc->n_entries > 0 && !bc->entries) return false;
-    if (bc->flags & PXH_BC_GROUPED) {
+    if (bc->n_levels < 0 || bc->n_levels > PXH_MAX_LEVELS) return false;
+    if (bc->n_levels > 1) {
+        if (bc->level_start[0] != 0 || bc->level_start[bc->n_levels] != bc->n_entries) return false;
+        for (int q = 0; q < bc->n_levels; ++q)
+            if (bc->level_start[q + 1] < bc->level_start[q]) return false;
+    } else if (bc->flags & PXH_BC_GROUPED) {
         if (bc->group_start[0] != 0 || bc->group_start[PXH_N_GROUPS] != bc->n_entries) return false;
         for (int q = 0; q < PXH_N_GROUPS; ++q)
             if (bc->group_start[q + 1] < bc->group_start[q]) return false;

@@ -369,6 +369,20 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
     pxh_bc b = *bc;
     b.group_mask &= groups;
     if (!b.group_mask) return PXH_OK;
+    if (bc->n_levels > 1) {
+        // one launch per dependency level, in order: each level reads only
+        // values no earlier level wrote (k_bc skips the unselected groups)
+        for (int L = 0; L < bc->n_levels; ++L) {
+            BcSel sel{};
+            sel.n = 1;
+            sel.start[0] = bc->level_start[L];
+            sel.off[1] = bc->level_start[L + 1] - bc->level_start[L];
+            if (sel.off[1] <= 0) continue;
+            hipLaunchKernelGGL(k_bc<T>, dim3((sel.off[1] + 255) / 256), dim3(256), 0, st, s, bc->entries, sel, b);
+            if (int e = launch_status()) return e;
+        }
+        return PXH_OK;
+    }
     BcSel sel{};
     if (bc->flags & PXH_BC_GROUPED) {
         for (int q = 0; q < PXH_N_GROUPS; ++q) {
@@ -1155,6 +1169,7 @@ int pxh_roe_exner_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields
         return PXH_E_ARG;
     if (!lo_ok(g, {in, out}) || (in->z_b_lo && in->z_b_lo == out->z_b_lo)) return PXH_E_ARG;
     if (!bc_ok(bc)) return PXH_E_ARG;
+    if (bc && bc->n_levels > 1) return PXH_E_UNSUPPORTED;
     return DISPATCH(g, rx_step, g, in, out, m0, sd, bc, dt_dev, static_cast<hipStream_t>(stream));
 }
 
@@ -1385,6 +1400,7 @@ int pxh_roe_exner_ssprk2(const pxh_grid *g, const pxh_fields *a, const pxh_field
     if (!lo_ok(g, {a, b, c}) || (a->z_b_lo && (a->z_b_lo == b->z_b_lo || a->z_b_lo == c->z_b_lo || b->z_b_lo == c->z_b_lo)))
         return PXH_E_ARG;
     if (!bc_ok(bc)) return PXH_E_ARG;
+    if (bc && bc->n_levels > 1) return PXH_E_UNSUPPORTED;
     return DISPATCH(g, rx_ssprk2, g, a, b, c, m0, sd, bc, cfl, dt_dev, ws, dt_ready,
                     static_cast<hipStream_t>(stream));
 }
@@ -1427,6 +1443,7 @@ int pxh_roe_step(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out,
     if (!grid_ok(g) || !fields_ok(in, false) || !fields_ok(out, false) || !dt_dev) return PXH_E_ARG;
     if (in->h == out->h || in->hu == out->hu || in->hv == out->hv) return PXH_E_ARG;
     if (!bc_ok(bc)) return PXH_E_ARG;
+    if (bc && bc->n_levels > 1) return PXH_E_UNSUPPORTED;
     return DISPATCH(g, roe_step, g, in, out, bc, dt_dev, static_cast<hipStream_t>(stream));
 }
 
@@ -1496,7 +1513,7 @@ int pxh_ensemble_ssprk2(const void *plan, const void *plan_dev, const pxh_fields
         !workspace)
         return PXH_E_ARG;
     if (!bc_ok(bc)) return PXH_E_ARG;
-    if (bc && bc->n_entries > 0 && !(bc->flags & PXH_BC_FRAME)) return PXH_E_UNSUPPORTED;
+    if (bc && bc->n_entries > 0 && (!(bc->flags & PXH_BC_FRAME) || bc->n_levels > 1)) return PXH_E_UNSUPPORTED;
     const pxh_grid g{1, 1, h->dtype, h->flags, 1.0};
     if (!lo_ok(&g, {a, b, c})) return PXH_E_ARG;
     const void *pa[] = {a->h, a->hu, a->hv, a->z_b, a->G}, *pb[] = {b->h, b->hu, b->hv, b->z_b, b->G},

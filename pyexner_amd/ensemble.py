@@ -43,6 +43,8 @@ class Ensemble:
                 raise NotImplementedError("ensemble members need frame-local rings (PXH_BC_FRAME)")
             if bm.values:
                 raise NotImplementedError("ensemble members take copy and reflection boundaries only")
+            if bm.n_levels > 1:
+                raise NotImplementedError("ensemble members take ring fills of one dependency level")
             hd = [(h.kind, float(h.normal[0]), float(h.normal[1])) for h in bm.boundary_handlers]
             if handlers is None:
                 handlers = hd

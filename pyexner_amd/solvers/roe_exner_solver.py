@@ -72,25 +72,31 @@ class ExnerBackend:
         self.np_dtype = np.float32 if dtype == torch.float32 else np.float64
         self.dx = float(dx)
         f32 = dtype == torch.float32
-        # float32 keeps the bed as z_b + z_b_lo unless told otherwise
+        # a ring fill of several dependency levels (a handler overwriting a
+        # cell an earlier one read) runs the stage as the reference sequences
+        # it, through the single-kernel entries (_seq_stage)
+        self.sequential = boundaries is not None and boundaries.n_levels > 1
+        # float32 keeps the bed as z_b + z_b_lo unless told otherwise (the
+        # single-kernel Exner update of the sequential stage keeps a plain bed)
         self.compensated_bed = f32 if compensated_bed is None else (bool(compensated_bed) and f32)
+        if self.sequential:
+            self.compensated_bed = False
         self.exact_f32 = bool(exact_f32) and f32
         flags = _hip.EXACT_F32 if self.exact_f32 else 0
         self.grid = _hip.Grid(self.shape[0], self.shape[1], dtype_code(dtype), flags, self.dx)
         self.constant = constant
         self.sed = sediment_table(seds, constant, self.np_dtype)
         self.boundaries = boundaries
-        if boundaries is not None and boundaries.n_entries and not boundaries.idempotent:
-            raise NotImplementedError("boundary polygons whose ring fill reads cells another handler writes "
-                                      "are not supported by the fused ring fill")
         self.bc_all = boundaries.program() if boundaries is not None else None
         self.bc_step = boundaries.program(self.STEP_GROUPS) if boundaries is not None else None
         # The step entries fuse the hydro and bed updates of a stage when the
         # program is frame-local (pxh_bc.flags); fused_stage=False keeps the
         # two sweeps with the ring fill between them.
-        self.fused_stage = bool(fused_stage)
+        self.fused_stage = bool(fused_stage) and not self.sequential
         if self.bc_step is not None and not self.fused_stage:
             self.bc_step.flags &= ~_hip.BC_FRAME
+        # the sequential stage's copy of the filled bed
+        self._bed = torch.empty(self.shape, dtype=dtype, device=device) if self.sequential else None
         # The dt workspace is sized once for the shortest strips (8 rows: the
         # most strips and seams), so set_strip_rows never moves it and a
         # hipGraph captured before a retune still points at live memory.
@@ -120,12 +126,37 @@ class ExnerBackend:
         if self.bc_all is not None and self.bc_all.n_entries:
             _hip.call("pxh_bc_apply", self.grid, fs, self.bc_all, stream)
 
-    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream, time=0.0):
+    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream, time=0.0, states=None):
         """One stage; time: the simulation time the ring fill sees (value
-        boundaries whose bed moves with time)."""
+        boundaries whose bed moves with time).  states = (in, out) state
+        objects, needed by the sequential stage."""
+        if self.sequential:
+            self._seq_stage(states[0], states[1], mask0_ptr, dt_ptr, stream, time)
+            return
         if self.bc_step is not None:
             self.bc_step.time = float(time)
         _hip.call("pxh_roe_exner_step", self.grid, fin, fout, mask0_ptr, self.sed, self.bc_step, dt_ptr, stream)
+
+    def _seq_stage(self, sin, sout, mask0_ptr, dt_ptr, stream, time):
+        """step_fn (roe_exner_solver.py:80-116) kernel by kernel with the ring
+        fill of every group after the hydro update and after the bed update,
+        as the reference applies its handlers: for a program of several
+        dependency levels two fills differ from one.  sout must not share
+        evolving fields with sin."""
+        for f in ("z_b", "G"):
+            getattr(sout, f).copy_(getattr(sin, f))  # the stage state carries the input bed
+        fin, fout = fields_struct(sin), fields_struct(sout)
+        g = self.grid
+        self.bc_all.time = float(time)
+        _hip.call("pxh_roe_solve_2D", g, fin, fout, mask0_ptr, dt_ptr, stream)
+        _hip.call("pxh_momentum_corrections", g, fin, fout, mask0_ptr, stream)
+        _hip.call("pxh_bc_apply", g, fout, self.bc_all, stream)
+        _hip.call("pxh_compute_G", g, fout, fout, mask0_ptr, self.sed, stream)
+        self._bed.copy_(sout.z_b)  # exner_solve_2D reads the filled bed and writes sout's
+        fb = fields_struct(sout)
+        fb.z_b = self._bed.data_ptr()
+        _hip.call("pxh_exner_solve_2D", g, fb, fout, mask0_ptr, dt_ptr, stream)
+        _hip.call("pxh_bc_apply", g, fout, self.bc_all, stream)
 
     def dt_into(self, fs, mask0_ptr, cfl, dt_ptr, stream):
         _hip.call("pxh_roe_exner_dt", self.grid, fs, mask0_ptr, float(cfl), dt_ptr, self.workspace.data_ptr(),
@@ -134,10 +165,20 @@ class ExnerBackend:
     def combine_into(self, fa, fb, fo, stream):
         _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
 
-    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False, time=0.0):
+    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False, time=0.0, states=None):
         """dt (unless dt_ready), two stages and the combine; a is advanced in
         place and dt_ptr receives the dt of the new a.  Both stages' ring
-        fills see `time`, the step's start time (SSPRK2.py:34-35)."""
+        fills see `time`, the step's start time (SSPRK2.py:34-35).  states =
+        (a, b, c), needed by the sequential stage."""
+        if self.sequential:
+            a, b, c = states
+            if not dt_ready:
+                self.dt_into(fa, mask0_ptr, cfl, dt_ptr, stream)
+            self._seq_stage(a, b, mask0_ptr, dt_ptr, stream, time)
+            self._seq_stage(b, c, mask0_ptr, dt_ptr, stream, time)
+            self.combine_into(fa, fc, fa, stream)
+            self.dt_into(fa, mask0_ptr, cfl, dt_ptr, stream)
+            return
         if self.bc_step is not None:
             self.bc_step.time = float(time)
         _hip.call("pxh_roe_exner_ssprk2", self.grid, fa, fb, fc, mask0_ptr, self.sed, self.bc_step, float(cfl),
@@ -220,7 +261,8 @@ def step_fn_roeexner(state, time, dt, mask, config):
     be = config.backend
     out = _new_evolving(state)
     dtt = _dt_tensor(dt, state.h.dtype, state.h.device)
-    be.step_into(fields_struct(state), fields_struct(out), _mask0_ptr(mask), dtt.data_ptr(), _stream(), time=time)
+    be.step_into(fields_struct(state), fields_struct(out), _mask0_ptr(mask), dtt.data_ptr(), _stream(), time=time,
+                 states=(state, out))
     return out
 
 

@@ -5,7 +5,7 @@ while the integrators pass one (integrators/forwardeuler.py:40,59); here the
 bundle takes the Roe-Exner signatures so both integrators can drive it.
 ``init_fn`` is the identity (halo exchange only, roe_solver.py:30-38), so the
 first step also fills the static fields z and n of the ring; later steps only
-refill h, hu, hv (the static fill is idempotent).
+refill h, hu, hv (the static fill gives the same static fields every time).
 """
 import torch
 
@@ -22,9 +22,10 @@ class RoeBackend:
         self.dx = float(dx)
         self.grid = _hip.Grid(self.shape[0], self.shape[1], dtype_code(dtype), 0, self.dx)
         self.boundaries = boundaries
-        if boundaries is not None and boundaries.n_entries and not boundaries.idempotent:
-            raise NotImplementedError("boundary polygons whose ring fill reads cells another handler writes "
-                                      "are not supported by the fused ring fill")
+        if boundaries is not None and boundaries.n_levels > 1:
+            # pxh_roe_step fills once per step; a program whose handlers read
+            # cells another one writes needs the reference's repeated fills
+            raise NotImplementedError("plain-Roe stepping of a ring fill of several dependency levels")
         self.bc_all = boundaries.program() if boundaries is not None else None
         self.bc_step = boundaries.program((_hip.G_H, _hip.G_HUV)) if boundaries is not None else None
         self.bc_static = boundaries.program((_hip.G_Z, _hip.G_N)) if boundaries is not None else None

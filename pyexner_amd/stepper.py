@@ -55,14 +55,15 @@ class ExnerStepper:
         (default self.time; the stepper does not read dt back, so a caller
         with such boundaries advances self.time itself)."""
         self.be.ssprk2_into(self.fa, self.fb, self.fc, self.m0, self.cfl, self.dt.data_ptr(), self._stream(),
-                            dt_ready=self.dt_ready, time=self.time if time is None else time)
+                            dt_ready=self.dt_ready, time=self.time if time is None else time,
+                            states=(self.a, self.b, self.c))
         self.dt_ready = True
 
     def fe_step(self):
         st = self._stream()
         dtp = self.dt.data_ptr()
         self.be.dt_into(self.fa, self.m0, self.cfl, dtp, st)
-        self.be.step_into(self.fa, self.fb, self.m0, dtp, st, time=self.time)
+        self.be.step_into(self.fa, self.fb, self.m0, dtp, st, time=self.time, states=(self.a, self.b))
         self.a, self.b = self.b, self.a
         self.fa, self.fb = self.fb, self.fa
         self.dt_ready = False

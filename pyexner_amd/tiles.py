@@ -176,6 +176,8 @@ class TiledExner:
         values = boundaries.values if boundaries is not None else []
         if boundaries is not None and boundaries.time_dependent:
             raise NotImplementedError("tiles run at time 0: boundaries whose values move with time are not tiled")
+        if boundaries is not None and boundaries.n_levels > 1:
+            raise NotImplementedError("tiles take ring fills of one dependency level")
         handlers = [(hd.kind, float(hd.normal[0]), float(hd.normal[1])) for hd in
                     (boundaries.boundary_handlers if boundaries is not None else [])]
         device = state.h.device

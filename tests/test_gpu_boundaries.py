@@ -123,3 +123,76 @@ def test_nodata_island_f64_and_f32(cuda, integrator):
         assert np.max(np.abs(a - o)) <= 1e-12 * scale, k
         assert np.isfinite(b).all(), k
     assert np.max(np.abs(g32.h[data] - so.h[data])) <= 1e-5 * np.max(so.h[data])
+
+
+def test_sequential_ring_fill_matches_handlers_f64(cuda):
+    """The reference's Berthon convergence inputs (2D_const_flux/dh_0.05/
+    input_0.05.yaml: bottom and top polygons two rows of cells thick) give a
+    ring fill that reads cells it writes.  pxh_bc_apply runs it level by
+    level and equals the handlers applied one after another
+    (boundary_registry.py:183-187), bit for bit."""
+    import torch
+    from oracle import boundaries as ob
+    from pyexner_amd.boundaries import BoundaryManager
+    from pyexner_amd.state import RoeExnerState
+    mesh, arr, params = cases.berthon_convergence(0.05)
+    bm = BoundaryManager(params, mesh.local_X, mesh.local_Y)
+    assert bm.n_levels > 1
+    rng = np.random.default_rng(21)
+    shape = mesh.local_shape
+    host = {k: rng.uniform(0.1, 1.0, shape) for k in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")}
+    st = RoeExnerState.from_arrays(host, np.zeros((7, 1)), dtype=torch.float64, device="cuda")
+    bm.apply_inplace(st, time=0.37)
+    from oracle.state import ExnerState
+    exp = ob.Boundaries("Roe Exner", params["boundaries"], mesh.local_X, mesh.local_Y).apply(
+        ExnerState(seds=None, **host), 0.37)
+    for k in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G"):
+        np.testing.assert_array_equal(getattr(st, k).cpu().numpy(), getattr(exp, k), err_msg=k)
+
+
+@pytest.mark.parametrize("integ", ["Forward Euler", "SSPRK2"])
+def test_sequential_ring_fill_run_f64(cuda, integ):
+    """60-odd steps of that case (Forward Euler is its own integrator)
+    through run_case against the oracle's window loop: each stage runs the
+    single-kernel sequence with both full fills; within 1e-12 per field."""
+    import torch
+    from oracle import steps as osteps
+    from pyexner_amd.runtime.driver import run_case
+    from tests.helpers import FIELDS_EXNER, compare, oracle_exner
+    mesh, arr, params = cases.berthon_convergence(0.05, end_time=0.36, out_freq=0.12, integrator=integ)
+    so, prob = oracle_exner(mesh, arr, params)
+    run = osteps.run_forward_euler if integ == "Forward Euler" else osteps.run_ssprk2
+    so, t_o, it_o = run(prob, so, 0.36, 0.12)
+    sg, _ = run_case(mesh, arr, params, dtype=torch.float64)
+    from pyexner_amd.integrators import forwardeuler, ssprk2
+    last = forwardeuler.last_run if integ == "Forward Euler" else ssprk2.last_run
+    assert last["iters"] == it_o and it_o >= 50
+    errs = compare(sg, so, FIELDS_EXNER)
+    print(f"berthon convergence {integ} x{it_o}", " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    assert max(errs.values()) <= 1e-12
+
+
+def test_sequential_ring_fill_refused_by_fused_entries(cuda):
+    """The fused stage and SSPRK2 entries fill once per stage: they refuse a
+    program of several dependency levels rather than fill it differently
+    from the reference."""
+    import torch
+    from pyexner_amd import _hip
+    from pyexner_amd.boundaries import BoundaryManager
+    from pyexner_amd.solvers.roe_exner_solver import ExnerBackend
+    from pyexner_amd.state import RoeExnerState, fields_struct
+    mesh, arr, params = cases.berthon_convergence(0.05)
+    bm = BoundaryManager(params, mesh.local_X, mesh.local_Y)
+    be = ExnerBackend(mesh.local_shape, torch.float64, mesh.dh, None, True, bm)
+    assert be.sequential and not be.fused_stage
+    host = {k: np.ones(mesh.local_shape) for k in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")}
+    st = RoeExnerState.from_arrays(host, np.zeros((7, 1)), dtype=torch.float64, device="cuda")
+    fa, fb, fc = (fields_struct(s) for s in (st, st.clone(), st.clone()))
+    dt = torch.zeros((), dtype=torch.float64, device="cuda")
+    m0 = torch.ones(mesh.local_shape, dtype=torch.uint8, device="cuda")
+    rc = _hip.load().pxh_roe_exner_step(be.grid, fa, fb, m0.data_ptr(), be.sed, be.bc_all, dt.data_ptr(),
+                                        torch.cuda.current_stream().cuda_stream)
+    assert rc == _hip.PXH_E_UNSUPPORTED
+    rc = _hip.load().pxh_roe_exner_ssprk2(be.grid, fa, fb, fc, m0.data_ptr(), be.sed, be.bc_all, 0.5, dt.data_ptr(),
+                                          be.workspace.data_ptr(), 0, torch.cuda.current_stream().cuda_stream)
+    assert rc == _hip.PXH_E_UNSUPPORTED

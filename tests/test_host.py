@@ -116,8 +116,8 @@ def test_composed_program_equals_sequential_handlers(builder, order):
         specs = {keys[i]: specs[keys[i]] for i in order}
     bm = BoundaryManager({"flux_scheme": "Roe Exner", "boundaries": specs}, mesh.local_X, mesh.local_Y,
                          device="cpu")
-    entries, idem, values = compose(bm.boundary_handlers, mesh.local_shape)
-    assert idem and (values == []) == (builder in ("channel", "radial", "periodic"))
+    entries, idem, values, levels = compose(bm.boundary_handlers, mesh.local_shape)
+    assert idem and len(levels) == 2 and (values == []) == (builder in ("channel", "radial", "periodic"))
     oc = ob.Boundaries("Roe Exner", specs, mesh.local_X, mesh.local_Y)
     np.testing.assert_array_equal(bm.boundary_mask, oc.boundary_mask)
     st = _random_exner(mesh.local_shape, rng)
@@ -140,7 +140,7 @@ def test_composed_program_roe_reflective_and_oblique_normal():
     specs = dict(params["boundaries"])
     specs["north"] = dict(specs["north"], normal=[0.6, 0.8])  # oblique normal: rounding pinned too
     bm = BoundaryManager({"flux_scheme": "Roe", "boundaries": specs}, mesh.local_X, mesh.local_Y, device="cpu")
-    entries, idem, values = compose(bm.boundary_handlers, mesh.local_shape)
+    entries, idem, values, levels = compose(bm.boundary_handlers, mesh.local_shape)
     oc = ob.Boundaries("Roe", specs, mesh.local_X, mesh.local_Y)
     st = RoeState(**{k: rng.uniform(-1, 1, mesh.local_shape) for k in ("h", "hu", "hv", "z", "n")})
     ex = ExnerState(h=st.h, hu=st.hu, hv=st.hv, z=st.z, z_b=st.z * 0, n=st.n, n_b=st.n * 0, G=st.n * 0,
@@ -159,7 +159,60 @@ def test_non_idempotent_layout_is_detected():
     specs = {"w": {"type": "Transmissive", "polygon": poly, "values": [0], "normal": [-1.0, 0.0]}}
     bm = BoundaryManager({"flux_scheme": "Roe Exner", "boundaries": specs}, mesh.local_X, mesh.local_Y,
                          device="cpu")
-    assert not bm.idempotent
+    assert not bm.idempotent and bm.n_levels == 2
+    # the outer column reads the inner one before the inner one is overwritten
+    oc = ob.Boundaries("Roe Exner", specs, mesh.local_X, mesh.local_Y)
+    st = _random_exner(mesh.local_shape, np.random.default_rng(5))
+    got = _emulate_program(bm.entries, bm.boundary_handlers, st, values=bm.values)
+    exp = oc.apply(st)
+    for f in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G"):
+        np.testing.assert_array_equal(getattr(got, f), getattr(exp, f), err_msg=f)
+
+
+@pytest.mark.parametrize("dh", [0.1, 0.05])
+def test_sequential_ring_fill_of_the_reference_berthon_inputs(dh):
+    """The reference's Berthon convergence inputs (2D_const_flux/dh_*/input_*.yaml:
+    bottom and top polygons two rows thick) compose into a program of several
+    dependency levels that, applied level by level, equals the handlers
+    applied one after another (boundary_registry.py:183-187), bit for bit,
+    at a time where the inlet and outlet beds have moved."""
+    mesh, arr, params = cases.berthon_convergence(dh)
+    bm = BoundaryManager(params, mesh.local_X, mesh.local_Y, device="cpu")
+    assert bm.n_levels > 1 and not bm.frame_local
+    ls = bm.level_start
+    assert ls[0] == 0 and ls[-1] == bm.n_entries and all(a <= b for a, b in zip(ls, ls[1:]))
+    # a copy entry reads its source before the entry that writes that source
+    # runs: the writer sits in a later level
+    lev = np.repeat(np.arange(bm.n_levels), np.diff(ls))
+    writer = {(int(e["target"]), int(e["group"])): int(lv) for e, lv in zip(bm.entries, lev)}
+    for e, lv in zip(bm.entries, lev):
+        if e["nchain"] & _hip.BC_VALUE:
+            continue
+        w = writer.get((int(e["src"]), int(e["group"])))
+        assert w is None or w > lv
+    oc = ob.Boundaries("Roe Exner", params["boundaries"], mesh.local_X, mesh.local_Y)
+    st = _random_exner(mesh.local_shape, np.random.default_rng(11))
+    got = _emulate_program(bm.entries, bm.boundary_handlers, st, values=bm.values, time=0.41)
+    exp = oc.apply(st, 0.41)
+    for f in ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G"):
+        np.testing.assert_array_equal(getattr(got, f), getattr(exp, f), err_msg=f)
+        np.testing.assert_array_equal(np.signbit(getattr(got, f)), np.signbit(getattr(exp, f)), err_msg=f)
+
+
+def test_multi_level_program_selects_the_sequential_stage():
+    """A program of several levels turns the Roe-Exner backend to the
+    single-kernel sequence (plain bed, no fused stage), and the plain-Roe
+    backend, which fills once per step, refuses it."""
+    import torch
+    from pyexner_amd.solvers.roe_exner_solver import ExnerBackend
+    from pyexner_amd.solvers.roe_solver import RoeBackend
+    mesh, arr, params = cases.berthon_convergence(0.05)
+    bm = BoundaryManager(params, mesh.local_X, mesh.local_Y, device="cpu")
+    be = ExnerBackend(mesh.local_shape, torch.float32, mesh.dh, None, True, bm, device="cpu")
+    assert be.sequential and not be.fused_stage and not be.compensated_bed
+    assert be.bc_all.n_levels == bm.n_levels and tuple(be._bed.shape) == tuple(mesh.local_shape)
+    with pytest.raises(NotImplementedError):
+        RoeBackend(mesh.local_shape, torch.float64, mesh.dh, bm, device="cpu")
 
 
 def test_sediment_table_matches_oracle_terms_float32():
