@@ -85,6 +85,30 @@ def dambreak_1d(nx=512, ny=32, dx=0.1, x_split=25.6, hl=1.0, hr=0.5):
     return mesh, arrays, params
 
 
+def roe_stepped_bed(nx=64, ny=40, dx=0.05, seed=21, level=0.5, end_time=0.05, film_range=(0.05, 0.12)):
+    """Plain Roe over a bed of flat blocks (steps at block edges), a still
+    level with thin films on the blocks that stand above it and a gentle
+    random current: the edges take the step-face and wall thrusts and the
+    source fix of roe.py:132-189.  Every cell stays wet (two dry neighbours
+    give 0/0 in the Roe average, roe.py:66)."""
+    mesh = uniform_mesh(ny, nx, dx)
+    rng = np.random.default_rng(seed)
+    by, bx = 5, 8
+    blocks = rng.uniform(0.0, 0.8, (ny // by + 1, nx // bx + 1))
+    z = np.kron(blocks, np.ones((by, bx)))[:ny, :nx]
+    film = rng.uniform(film_range[0], film_range[1], (ny, nx))
+    h = np.maximum(level - z, film)
+    still = (level - z) < film  # films stand still
+    u = np.where(still, 0.0, rng.uniform(-0.3, 0.3, (ny, nx)) * np.sqrt(9.81 * h))
+    v = np.where(still, 0.0, rng.uniform(-0.3, 0.3, (ny, nx)) * np.sqrt(9.81 * h))
+    arrays = {"h": h, "hu": h * u, "hv": h * v, "z": z, "n": np.full_like(h, 0.02)}
+    params = {"flux_scheme": "Roe", "integrator": "Forward Euler", "cfl": 0.5, "end_time": end_time,
+              "out_freq": end_time, "parNx": 1, "parNy": 1,
+              "boundaries": ring_boundaries(mesh, [("west", "Reflective", "W"), ("east", "Reflective", "E"),
+                                                   ("south", "Reflective", "S"), ("north", "Reflective", "N")])}
+    return mesh, arrays, params
+
+
 def _exner_params(mesh, bounds, integrator="SSPRK2"):
     return {"flux_scheme": "Roe Exner", "integrator": integrator, "cfl": 0.5, "end_time": 1.0, "out_freq": 1.0,
             "parNx": 1, "parNy": 1,

@@ -210,6 +210,26 @@ def test_roe_dambreak_c1_f64(cuda):
     assert np.max(np.abs(hh - hh[0:1, :])) <= 1e-12
 
 
+def test_roe_stepped_bed_f64(cuda):
+    """Plain Roe over flat blocks with thin films (cases.roe_stepped_bed):
+    step-face and wall thrusts and the source fix fire (roe.py:132-189);
+    about 45 Forward-Euler steps through run_case against the oracle, within
+    1e-12 per field, negative film depths included."""
+    from pyexner_amd.runtime.driver import run_case
+    from pyexner_amd.integrators import forwardeuler
+    mesh, arr, params = cases.roe_stepped_bed()
+    h, z = arr["h"], arr["z"]
+    assert ((z[:, 1:] >= z[:, :-1]) & (h[:, :-1] + z[:, :-1] < z[:, 1:])).sum() > 50  # walls
+    so, prob = oracle_roe(mesh, arr, params)
+    so, t_o, it_o = osteps.run_forward_euler(prob, so, params["end_time"], params["out_freq"])
+    assert np.isfinite(so.h).all() and so.h.min() < 0 and it_o >= 40
+    sg, _ = run_case(mesh, arr, params, dtype=torch.float64)
+    assert forwardeuler.last_run["iters"] == it_o
+    errs = compare(sg, so, FIELDS_ROE)
+    _print("roe stepped bed", errs)
+    assert max(errs.values()) <= F64_TOL
+
+
 def test_periodic_translation_invariance_f64(cuda):
     mesh, arr, params = cases.periodic_box(nx=40, ny=30)
     shift = 7

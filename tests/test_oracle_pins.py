@@ -120,22 +120,146 @@ def test_get_lambda_linear_and_clip_branches():
     u, c = np.array([0.3]), np.array([1.2])
     lmin, lmax = rx.get_lambda(u, np.array([5e-6]), c)
     assert lmin[0] == 0.3 - 1.2 and lmax[0] == 0.3 + 1.2
-    # near Froude 1 the trig argument leaves [-1, 1]: clipped to theta = 0 or pi
-    for uu in (0.95, 1.0, 1.05):
-        a, cc = 0.3, 1.0
-        b = -2 * uu
-        cq = (1 + a) * cc * cc - uu * uu
-        d = -cc * cc * a * uu
-        p = (-3 * cq - b * b) / 3
-        q = (2 * b ** 3 + 9 * b * cq + 27 * d) / -27
-        arg = (3 * q) / (2 * p) * math.sqrt(-3 / p)
-        if abs(arg) <= 1:
+    # Where the coded cubic has one real root the trigonometric argument
+    # leaves [-1, 1] and is clipped (roe_exner.py:189).  The pair returned is
+    # then the extreme roots of the nearest cubic with a double root: the
+    # same depressed form t^3 + p t + q with q moved to the discriminant-zero
+    # boundary -+2 s^3 (s = sqrt(-p/3)), whose roots are {2s, -s, -s} or
+    # {-2s, s, s}.  p and q come here from expanding the cubic about its
+    # inflection point with numpy's polynomial algebra, not from the
+    # Cardano coefficients the oracle uses.
+    from numpy.polynomial import Polynomial as Poly
+    rng = np.random.default_rng(17)
+    hits = {1: 0, -1: 0}
+    for _ in range(4000):
+        uu, cc, aa = rng.uniform(-3, 3), rng.uniform(0.1, 3), 10 ** rng.uniform(-4, 0.5)
+        cub = Poly([-cc * cc * aa * uu, (1 + aa) * cc * cc - uu * uu, -2 * uu, -1.0])  # ascending
+        shift = -cub.coef[2] / (3 * cub.coef[3])
+        dep = cub(Poly([shift, 1.0])) / cub.coef[3]  # monic, in t = lambda - shift
+        assert abs(dep.coef[2]) <= 1e-12 * (1 + abs(shift))
+        p, q = dep.coef[1], dep.coef[0]
+        if p >= 0:
             continue
-        th = 0.0 if arg > 1 else math.pi
-        st = 2 * math.sqrt(-p / 3)
-        lmin, lmax = rx.get_lambda(np.array([uu]), np.array([a]), np.array([cc]))
-        assert abs(lmax[0] - (st * math.cos(th / 3) - b / -3)) < 1e-12
-        assert abs(lmin[0] - (st * math.cos((th + 2 * math.pi) / 3) - b / -3)) < 1e-12
+        sq = math.sqrt(-p / 3)
+        if abs(q) <= 2 * sq ** 3 * (1 + 1e-9):  # three real roots: not the clip branch
+            continue
+        side = 1 if q < 0 else -1  # q < -2s^3 clips the argument at +1
+        lmin, lmax = rx.get_lambda(np.array([uu]), np.array([aa]), np.array([cc]))
+        tmax, tmin = lmax[0] - shift, lmin[0] - shift
+        clamped = Poly([-side * 2 * sq ** 3, p, 0.0, 1.0])
+        scale = 1 + abs(shift) + sq
+        for t in (tmax, tmin):
+            assert abs(clamped(t)) <= 1e-9 * scale ** 3
+        double = tmin if side == 1 else tmax
+        assert abs(clamped.deriv()(double)) <= 1e-9 * scale ** 2  # the double root
+        assert abs((tmax if side == 1 else -tmin) - 2 * sq) <= 1e-9 * scale  # the simple root
+        hits[side] += 1
+    assert min(hits.values()) > 50, hits
+
+
+def test_roe_exner_manning_uniform_flow():
+    """Uniform flow over a flat bed has no jump, so upwP + upwM is the
+    friction source alone: (0, nx tau, ny tau) with Manning's force
+    tau = g h Sf dx, Sf = n^2 |u| u_n / h^(4/3), whatever the edge normal
+    (roe_exner.py:455-457).  beta_3 = -beta_1 (roe_exner.py:462) puts
+    -tau/(2c) of mass on the M side and +tau/(2c) on the P side."""
+    rng = np.random.default_rng(31)
+    n = 500
+    h = rng.uniform(0.05, 3.0, n)
+    c = np.sqrt(g * h)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    sp = rng.uniform(0, 0.7, n) * c  # subcritical across every edge
+    u, v = sp * np.cos(ang), sp * np.sin(ang)
+    nm = rng.uniform(0.01, 0.08, n)
+    st = (h, h * u, h * v, rng.uniform(0.01, 0.3, n), rng.uniform(0, 1, n), nm)
+    dx = 0.1
+    for nx, ny in ((1, 0), (0, -1), (-1, 0), (0, 1), (0.6, 0.8)):
+        P, M = rx.roe_edge(st, st, nx, ny, dx)
+        un = u * nx + v * ny
+        tau = g * nm ** 2 * np.hypot(u, v) * un / np.cbrt(h) * dx
+        big = np.max(np.abs(tau))
+        assert big > 1e-3
+        for k, e in enumerate((np.zeros(n), nx * tau, ny * tau)):
+            assert np.max(np.abs(P[k] + M[k] - e)) <= 1e-12 * big
+        assert np.max(np.abs(M[0] + tau / (2 * c))) <= 1e-12 * big
+        assert np.max(np.abs(P[0] - tau / (2 * c))) <= 1e-12 * big
+
+
+def _hh_fluctuations(hi, hj, ui, uj, vi, vj, vel_tol3=0.0, fix=True):
+    """Upwind fluctuations of a flat-bed, frictionless x edge from the
+    textbook Harten-Hyman split of each transonic wave (lambda^- = e_i (e_j -
+    l) / (e_j - e_i), lambda^+ = e_j (l - e_i) / (e_j - e_i), so lambda^- +
+    lambda^+ = l) with the wave strengths solved from the Roe eigenvectors by
+    numpy.linalg.solve.  As in the reference's P (roe_exner.py:379-383) the
+    eigenvector of a fixed wave carries its fixed speed; the reference also
+    adds VEL_TOL to the wave-3 denominator of the fixed speed (vel_tol3)."""
+    si, sj = np.sqrt(hi), np.sqrt(hj)
+    c = np.sqrt(g * 0.5 * (hi + hj))
+    ut = (ui * si + uj * sj) / (si + sj)
+    vt = (vi * si + vj * sj) / (si + sj)
+    l1, l3 = ut - c, ut + c
+    R = np.stack([np.stack([np.ones_like(c), np.zeros_like(c), np.ones_like(c)], -1),
+                  np.stack([l1, np.zeros_like(c), l3], -1),
+                  np.stack([vt, c, vt], -1)], -2)
+    dU = np.stack([hj - hi, hj * uj - hi * ui, hj * vj - hi * vi], -1)
+    alpha = np.linalg.solve(R, dU[..., None])[..., 0]
+    lam, lamE = [l1, ut, l3], [np.zeros_like(c)] * 3
+    for k, sgn in ((0, -1.0), (2, 1.0)):
+        ei, ej = ui + sgn * np.sqrt(g * hi), uj + sgn * np.sqrt(g * hj)
+        fire = (ei < 0) & (ej > 0) & fix
+        l = lam[k]
+        neg, pos = ei * (ej - l) / (ej - ei), ej * (l - ei) / (ej - ei)
+        if k == 0:
+            lam[0], lamE[0] = np.where(fire, neg, l), np.where(fire, pos, 0.0)
+        else:
+            pos_r = ej * (l - ei) / ((ej - ei) + vel_tol3)
+            lam[2], lamE[2] = np.where(fire, pos_r, l), np.where(fire, l - pos, 0.0)
+    E = [np.stack([np.ones_like(c), lam[0], vt]), np.stack([0 * c, 0 * c, c]), np.stack([np.ones_like(c), lam[2], vt])]
+    P = np.zeros((3,) + c.shape)
+    M = np.zeros_like(P)
+    for k in range(3):
+        for sp in (lam[k], lamE[k]):
+            f = sp * alpha[..., k] * E[k]
+            P += np.where(sp > 0, f, 0.0)
+            M += np.where(sp <= 0, f, 0.0)
+    return P, M, (lam, lamE, (l1, l3))
+
+
+@pytest.mark.parametrize("wave", [1, 3])
+def test_roe_exner_harten_hyman_transonic(wave):
+    """Transonic rarefactions (e_i < 0 < e_j for wave 1 or 3) fire the
+    Harten-Hyman fix (roe_exner.py:357-373): the fluctuations equal the
+    textbook split, lambda + lambda_E = lambda_roe, and differ from the
+    unfixed Roe fluctuations."""
+    from oracle.constants import VEL_TOL
+    rng = np.random.default_rng(40 + wave)
+    n = 600
+    hi, hj = rng.uniform(0.2, 2.0, n), rng.uniform(0.2, 2.0, n)
+    ci, cj = np.sqrt(g * hi), np.sqrt(g * hj)
+    if wave == 1:  # subcritical left state, supercritical right-going right state
+        ui, uj = rng.uniform(-0.9, 0.9, n) * ci, rng.uniform(1.05, 2.5, n) * cj
+    else:  # supercritical left-going left state, subcritical right state
+        ui, uj = -rng.uniform(1.05, 2.5, n) * ci, rng.uniform(-0.9, 0.9, n) * cj
+    vi, vj = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    Pe, Me, (lam, lamE, (l1, l3)) = _hh_fluctuations(hi, hj, ui, uj, vi, vj, vel_tol3=VEL_TOL)
+    k = 0 if wave == 1 else 2
+    keep = (np.abs(lam[k]) > 1e-3) & (np.abs(lamE[k]) > 1e-3) & (np.abs(lam[1]) > 1e-3)
+    assert keep.sum() > n // 2
+    sel = lambda a: a[..., keep]  # noqa: E731
+    hi, hj, ui, uj, vi, vj = map(sel, (hi, hj, ui, uj, vi, vj))
+    Pe, Me = sel(Pe), sel(Me)
+    z0, zb = np.zeros_like(hi), np.full_like(hi, 0.1)
+    P, M = rx.roe_edge((hi, hi * ui, hi * vi, zb, z0, z0), (hj, hj * uj, hj * vj, zb, z0, z0), 1, 0, 0.1)
+    scale = np.max(np.abs(Pe)) + np.max(np.abs(Me))
+    tol = 1e-11 if wave == 1 else 3e-6  # VEL_TOL / (e_j - e_i) for wave 3
+    for c in range(3):
+        assert np.max(np.abs(P[c] - Pe[c])) <= tol * scale, c
+        assert np.max(np.abs(M[c] - Me[c])) <= tol * scale, c
+    lr = l1 if wave == 1 else l3
+    assert np.max(np.abs(sel(lam[k] + lamE[k]) - sel(lr))) <= (1e-12 if wave == 1 else 1e-5)
+    # the fix moves a visible part of the fluctuation from one side to the other
+    Pn, Mn, _ = _hh_fluctuations(hi, hj, ui, uj, vi, vj, fix=False)
+    assert np.max(np.abs(M[0] - Mn[0])) > 1e-2 * scale and np.max(np.abs(P[0] - Pn[0])) > 1e-2 * scale
 
 
 def test_dt_2d_brute_force():
@@ -331,6 +455,253 @@ def test_roe_c_property():
     P, M = oroe.roe_edge((hi, z0, z0, zi, z0), (hj, z0, z0, zj, z0), 1, 0, 0.1)
     for c in range(3):
         assert np.max(np.abs(P[c])) < 1e-13 and np.max(np.abs(M[c])) < 1e-13
+
+
+def _step_force(hi, hj, zi, zj):
+    """Hydrostatic thrust of a bed step on the x edge, -g times the integral
+    of the depth over the step face: a face of height |dz| under an upstream
+    depth h_r gives h_r |dz| - dz^2/2; a face that the upstream water surface
+    does not reach gives h^2/2, the whole column.  The downward-step wall
+    (j's surface below z_i) keeps the reference's sign: its dztilde = h_j
+    has the sign of a rise (roe.py:143-145)."""
+    dz = zj - zi
+    up = dz >= 0
+    wall_up = up & (hi + zi < zj)
+    wall_dn = ~up & (hj + zj < zi)
+    face = np.where(up, g * (hi * dz - 0.5 * dz * dz), -g * (hj * -dz - 0.5 * dz * dz))
+    return np.where(wall_up, 0.5 * g * hi * hi, np.where(wall_dn, 0.5 * g * hj * hj, face)) * -1.0
+
+
+def _roe_pairs_over_steps(n, rng):
+    hi, hj = 10 ** rng.uniform(-2.5, 0.3, n), 10 ** rng.uniform(-2.5, 0.3, n)
+    ui = rng.uniform(-0.8, 0.8, n) * np.sqrt(g * hi)
+    uj = rng.uniform(-0.8, 0.8, n) * np.sqrt(g * hj)
+    zi, zj = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    return hi, hj, ui, uj, zi, zj
+
+
+def _roe_textbook(hi, hj, ui, uj, zi, zj):
+    """Roe speeds, wave strengths (numpy.linalg.solve of the eigenvectors),
+    the step thrust and the source strengths of an x edge without friction."""
+    si, sj = np.sqrt(hi), np.sqrt(hj)
+    c = np.sqrt(g * 0.5 * (hi + hj))
+    ut = (ui * si + uj * sj) / (si + sj)
+    l1, l3 = ut - c, ut + c
+    R = np.stack([np.stack([np.ones_like(c), np.ones_like(c)], -1), np.stack([l1, l3], -1)], -2)
+    dU = np.stack([hj - hi, hj * uj - hi * ui], -1)
+    a1, a3 = np.moveaxis(np.linalg.solve(R, dU[..., None])[..., 0], -1, 0)
+    dz = zj - zi
+    Ta = -g * 0.5 * (hi + hj) * dz  # Roe-averaged thrust
+    Tb = _step_force(hi, hj, zi, zj)
+    rising = (dz * ((hj + zj) - (hi + zi)) >= 0) & (ut * dz > 0)
+    T = np.where(rising, np.maximum(Ta, Tb), Tb)
+    b1 = -T / (2 * c)  # P^-1 (0, T, 0): beta_1 = T / (l1 - l3)
+    return dict(c=c, ut=ut, l1=l1, l3=l3, a1=a1, a3=a3, T=T, b1=b1,
+                hi1=hi + a1 - b1 / l1, hj3=hj - a3 - b1 / l3)
+
+
+def test_roe_step_thrust_and_beta_fix():
+    """Wet pairs over bed steps, flat-bed speeds subcritical, no friction.
+    beta_2 = 0 and beta_3 = -beta_1 always (roe.py:159-189), so upwP + upwM =
+    dF - (0, beta_1 (l1 - l3), 0): the mass and tangential parts are the flux
+    jump and the momentum part gives the thrust the edge applied.
+    * No source fix: that thrust is the hydrostatic step force (or the
+      larger of it and the Roe-averaged -g h dz on a rising, uphill-flowing
+      edge, roe.py:154-155).
+    * Fix 1 (first intermediate depth negative, roe.py:183-185): beta_1 is
+      set so that the intermediate depth h_i + alpha_1 - beta_1 / l1 is zero.
+    * Fix 2 (third intermediate depth negative, roe.py:187-189): the
+      reference writes beta1min, not a beta3 bound, into beta_3 when
+      beta3min <= 0, so the thrust is -beta1min (l1 - l3) there."""
+    rng = np.random.default_rng(52)
+    hi, hj, ui, uj, zi, zj = _roe_pairs_over_steps(40000, rng)
+    z0 = np.zeros_like(hi)
+    tb = _roe_textbook(hi, hj, ui, uj, zi, zj)
+    l1, l3, a1 = tb["l1"], tb["l3"], tb["a1"]
+    ei, ej = ui - np.sqrt(g * hi), uj - np.sqrt(g * hj)
+    ei3, ej3 = ui + np.sqrt(g * hi), uj + np.sqrt(g * hj)
+    hh = ((ei < 0) & (ej > 0)) | ((ei3 < 0) & (ej3 > 0))
+    lmax = np.maximum(np.abs(l1), np.maximum(np.abs(tb["ut"]), np.abs(l3)))
+    dx = 0.1
+    dt = dx / lmax
+    d1 = (dx / 2 * l1) * (hi / (hi - tb["hi1"]))  # (dx/2) * lambda, as coded (roe.py:180)
+    d3 = (dx / 2 * l3) * (hj / (hj - tb["hj3"]))
+    b1min = -(hi + a1) * np.abs(l1)
+    b3min = -(hi - a1) * np.abs(l3)
+    fix1 = (tb["hi1"] < 0) & (tb["hj3"] > 0) & (d1 < dt) & (-b1min >= b3min)
+    fix2 = (tb["hi1"] > 0) & (tb["hj3"] < 0) & (d3 < dt) & (b3min <= 0)
+    touch1 = (tb["hi1"] < 0) & (tb["hj3"] > 0) & (d1 < dt)
+    touch2 = (tb["hi1"] > 0) & (tb["hj3"] < 0) & (d3 < dt)
+    # keep pairs whose branch is decided with margin
+    margin = (np.abs(tb["hi1"]) > 1e-6) & (np.abs(tb["hj3"]) > 1e-6) & (np.abs(d1 - dt) > 1e-6) & (
+        np.abs(d3 - dt) > 1e-6) & (np.abs(b1min + b3min) > 1e-6) & (np.abs(b3min) > 1e-6) & (np.abs(l1) > 1e-3)
+    plain = ~touch1 & ~touch2
+    keep = ~hh & margin
+    P, M = oroe.roe_edge((hi, hi * ui, z0, zi, z0), (hj, hj * uj, z0, zj, z0), 1, 0, dx)
+    dF = _rot_flux(hj, uj, z0) - _rot_flux(hi, ui, z0)
+    tot = np.stack([P[k] + M[k] for k in range(3)])
+    # VEL_TOL in P^-1's denominator (roe.py:107) shifts every product by up to
+    # VEL_TOL / |l1 - l3| relative: the bound below is ten times that
+    from oracle.constants import VEL_TOL
+    scale = (np.abs(dF[0]) + np.abs(dF[1]) + np.abs(tb["T"]) + g * (hi + hj) ** 2) * (10 * VEL_TOL / (2 * tb["c"]))
+    rel = lambda a: np.abs(a) / scale  # noqa: E731
+    assert np.max(rel(tot[0] - dF[0])[keep]) < 1
+    assert np.max(rel(tot[2] - dF[2])[keep]) < 1
+    thrust = dF[1] - tot[1]  # = beta_1 (l3 - l1) up to VEL_TOL
+    b1_eff = thrust / (l1 - l3)
+    sel = keep & plain
+    assert sel.sum() > 5000 and np.max(rel(thrust - tb["T"])[sel]) < 1
+    # every thrust branch is exercised: both walls, both face directions, the max
+    dz = zj - zi
+    for branch in (sel & (dz >= 0) & (hi + zi < zj), sel & (dz < 0) & (hj + zj < zi),
+                   sel & (dz >= 0) & (hi + zi >= zj), sel & (dz < 0) & (hj + zj >= zi),
+                   sel & (tb["T"] != _step_force(hi, hj, zi, zj))):
+        assert branch.sum() > 50
+    s1 = keep & fix1
+    assert s1.sum() > 500
+    assert np.max(rel((b1_eff - b1min) * (l1 - l3))[s1]) < 1
+    # beta1min = -(h_i + alpha_1) |l1| zeroes the intermediate depth where
+    # the wave runs left (l1 < 0), the case the bound is built for
+    s1l = s1 & (l1 < 0)
+    assert s1l.sum() > 200
+    h_star = hi + a1 - b1_eff / l1
+    assert np.max((np.abs(h_star) / (hi + np.abs(a1)))[s1l]) < 1e-5
+    assert np.max(np.abs(tb["hi1"][s1l])) > 1e-2  # the unfixed intermediate depth was well below zero
+    s2 = keep & fix2
+    assert s2.sum() > 500
+    assert np.max(rel((b1_eff + b1min) * (l1 - l3))[s2]) < 1
+    # fix conditions met but the bound not taken: the source is left alone
+    s3 = keep & ((touch1 & ~fix1) | (touch2 & ~fix2))
+    assert s3.sum() > 50 and np.max(rel(thrust - tb["T"])[s3]) < 1
+
+
+def test_roe_lake_at_rest_against_a_wall():
+    """Water at rest beside a step its surface does not reach (h_i + z_i < z_j,
+    the far side dry): the wall thrust -g h_i^2 / 2 balances the pressure
+    jump, so neither side receives a fluctuation (roe.py:143-157)."""
+    rng = np.random.default_rng(57)
+    n = 300
+    H = rng.uniform(0.01, 2.0, n)
+    zi = rng.uniform(0, 1, n)
+    zj = zi + H * rng.uniform(1.01, 3.0, n)
+    z0 = np.zeros(n)
+    P, M = oroe.roe_edge((H, z0, z0, zi, z0), (z0, z0, z0, zj, z0), 1, 0, 0.1)
+    from oracle.constants import VEL_TOL
+    bound = g * H * H * 10 * VEL_TOL / (2 * np.sqrt(g * H / 2))  # VEL_TOL in P^-1 (roe.py:107)
+    for c in range(3):
+        assert np.all(np.abs(P[c]) < bound) and np.all(np.abs(M[c]) < bound)
+
+
+def test_roe_harten_hyman_uses_the_fixed_speed():
+    """Plain Roe replaces the transonic speed first and builds lambda_E from
+    the replaced value, VEL_TOL in both denominators (roe.py:82-92); its P
+    and P^-1 then carry the replaced speeds.  Checked against a direct
+    evaluation with the eigenvector matrix inverted by numpy."""
+    from oracle.constants import VEL_TOL
+    rng = np.random.default_rng(63)
+    n = 400
+    hi, hj = rng.uniform(0.2, 2.0, n), rng.uniform(0.2, 2.0, n)
+    ci, cj = np.sqrt(g * hi), np.sqrt(g * hj)
+    ui, uj = rng.uniform(-0.9, 0.9, n) * ci, rng.uniform(1.05, 2.5, n) * cj  # wave 1 transonic
+    vi, vj = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    si, sj = np.sqrt(hi), np.sqrt(hj)
+    c = np.sqrt(g * 0.5 * (hi + hj))
+    ut = (ui * si + uj * sj) / (si + sj)
+    vt = (vi * si + vj * sj) / (si + sj)
+    ei, ej = ui - ci, uj - cj
+    l1 = ei * (ej - (ut - c)) / ((ej - ei) + VEL_TOL)
+    lE1 = l1 - ei * (ej - l1) / ((ej - ei) + VEL_TOL)
+    l3 = ut + c
+    Pm = np.stack([np.stack([np.ones(n), np.zeros(n), np.ones(n)], -1), np.stack([l1, np.zeros(n), l3], -1),
+                   np.stack([vt, c, vt], -1)], -2)
+    dU = np.stack([hj - hi, hj * uj - hi * ui, hj * vj - hi * vi], -1)
+    alpha = np.linalg.solve(Pm, dU[..., None])[..., 0]
+    lam, lamE = (l1, ut, l3), (lE1, np.zeros(n), np.zeros(n))
+    Pe, Me = np.zeros((3, n)), np.zeros((3, n))
+    for k in range(3):
+        for sp in (lam[k], lamE[k]):
+            f = sp * alpha[:, k] * Pm[:, :, k].T
+            Pe += np.where(sp > 0, f, 0.0)
+            Me += np.where(sp <= 0, f, 0.0)
+    keep = (np.abs(l1) > 1e-3) & (np.abs(lE1) > 1e-3) & (np.abs(ut) > 1e-3)
+    z0 = np.zeros(n)
+    P, M = oroe.roe_edge((hi, hi * ui, hi * vi, z0, z0), (hj, hj * uj, hj * vj, z0, z0), 1, 0, 0.1)
+    scale = np.max(np.abs(Pe)) + np.max(np.abs(Me))
+    for k in range(3):
+        # VEL_TOL in P^-1's denominator (roe.py:107) against the exact inverse
+        assert np.max(np.abs(P[k] - Pe[k])[keep]) < 1e-5 * scale
+        assert np.max(np.abs(M[k] - Me[k])[keep]) < 1e-5 * scale
+    # lambda + lambda_E is not lambda_roe here (lambda_E is built from the replaced speed)
+    assert np.min(np.abs(l1 + lE1 - (ut - c))[keep]) > 1e-3
+
+
+def test_constant_outflux_newton():
+    """ConstantOutflux (roeexner_constantoutflux.py:20-71) solves q / h_b +
+    2 sqrt(g h_b) = R+ = u_n + 2 sqrt(g h) of the interior neighbour by five
+    Newton steps from h: for subcritical interiors the result is the
+    subcritical root (above the critical depth (q^2/g)^(1/3)) that a bracketing
+    solver finds, and the ring momentum is the prescribed discharge."""
+    from scipy.optimize import brentq
+    from oracle import boundaries as ob
+    mesh, arr, params = cases.outflux_channel(nx=13, ny=9, q=0.05)
+    specs = {"outlet": params["boundaries"]["outlet"]}
+    bc = ob.Boundaries("Roe Exner", specs, mesh.local_X, mesh.local_Y)
+    hd = bc.handlers[0]
+    rng = np.random.default_rng(71)
+    shp = mesh.local_shape
+    st = ExnerState(**{k: rng.uniform(0, 1, shp) for k in ("z", "z_b", "n", "n_b", "G")},
+                    h=rng.uniform(0.12, 0.6, shp), hu=0.05 * rng.uniform(0.6, 1.6, shp),
+                    hv=rng.uniform(-0.01, 0.01, shp), seds=np.zeros((7, 1)))
+    out = bc.apply(st)
+    (by, bx), (iy, ix) = hd.b, hd.i
+    assert len(by) >= 5 and np.all(bx == shp[1] - 1) and np.all(ix == shp[1] - 2)
+    q = float(np.float32(0.05))
+    h_i = st.h[iy, ix]
+    Rp = st.hu[iy, ix] / h_i + 2 * np.sqrt(g * h_i)
+    hc = (q * q / g) ** (1 / 3)
+    f = lambda hb, r: q / hb + 2 * np.sqrt(g * hb) - r  # noqa: E731
+    assert np.all(st.hu[iy, ix] / h_i < 0.5 * np.sqrt(g * h_i))  # subcritical interiors
+    for k in range(len(by)):
+        hb = out.h[by[k], bx[k]]
+        root = brentq(f, hc, 100.0, args=(Rp[k],), xtol=1e-15, rtol=1e-15)
+        assert hb > hc and abs(hb - root) <= 1e-10 * root, (k, hb, root)
+        assert abs(f(hb, Rp[k])) <= 1e-10 * Rp[k]
+    np.testing.assert_array_equal(out.hu[by, bx], np.full(len(by), q))
+    np.testing.assert_array_equal(out.hv[by, bx], np.zeros(len(by)))
+
+
+def test_constant_G_masking_and_stickiness():
+    """With a constant grass factor G is never computed, only zeroed where the
+    cell is masked or dry (roe_exner.py:136-139), and the state carries it:
+    a cell the flood reaches after it was zeroed keeps G = 0.  Forward-Euler
+    steps of a dam break onto a dry bed, every interior cell checked."""
+    mesh, arr, params = cases.wetdry_obstacle(nx=40, ny=18)
+    params = dict(params, erosion={"grass_factor": 0.3})
+    rng = np.random.default_rng(77)
+    h = np.where(arr["h"] > 0, arr["h"], 5e-5)  # dry, yet not zero
+    z = np.array(arr["z"], dtype=np.float64)
+    z[3:6, 4:7] = np.nan  # nodata: masked (roe_exner_solver.py:21-39)
+    arr = dict(arr, h=h, z=z, G=rng.uniform(0.1, 1.0, mesh.local_shape))
+    s, prob = oracle_exner(mesh, arr, params)
+    assert prob.constant
+    m0 = prob.mask[0]
+    inner = np.zeros_like(m0)
+    inner[1:-1, 1:-1] = True
+    G0 = np.asarray(arr["G"])
+    assert m0[inner].sum() == 9 and ((s.h > 0) & (s.h <= 1e-4))[inner].sum() > 100
+    np.testing.assert_array_equal(s.G[inner], np.where(m0 | (s.h <= 1e-4), 0.0, G0)[inner])
+    dry0 = inner & (s.h <= 1e-4) & ~m0
+    t = 0.0
+    for _ in range(40):
+        Gin = s.G
+        s, dt = osteps.fe_step(prob, s, t)
+        t += dt
+        exp = np.where(m0 | (s.h <= 1e-4), 0.0, Gin)
+        np.testing.assert_array_equal(s.G[inner], exp[inner])
+    wetted = dry0 & (s.h > 1e-4)
+    assert wetted.sum() >= 10 and np.all(s.G[wetted] == 0.0)
+    still = inner & ~m0 & (s.h > 1e-4) & ~dry0
+    np.testing.assert_array_equal(s.G[still], G0[still])
 
 
 def test_roe_stoker_dambreak():
