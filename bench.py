@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
 """Throughput of the Roe-Exner SSPRK2 step on one MI355X.
 
-Workload: Roe-Exner, SSPRK2, float32 (cubic wave speeds in float64), radial dam
-break on one 8192 x 8192 domain, dx = 0.05 m, h = 2.0 m inside r < 50 m else
+Workload: Roe-Exner, SSPRK2, float32 throughout (hardware-rate square roots,
+reciprocals and transcendentals; the cubic wave speeds solved in float32),
+radial dam break on one 8192 x 8192 domain, dx = 0.05 m, h = 2.0 m inside r < 50 m else
 0.5 m, 256 seeded Gaussian bed bumps, MPM sediment (d = 1.61 mm, 2630 kg/m^3,
 porosity 0.42), Reflective ring.  A step is one SSPRK2 step of the whole
 domain (dt reduction, two Forward-Euler stages, combine) with the state
@@ -79,8 +80,12 @@ def parse():
     p.add_argument("--kernel-steps", type=int, default=10,
                    help="SSPRK2 steps issued sweep by sweep to time each kernel inside the step")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-size", type=int, default=1024)
-    p.add_argument("--cpu-steps", type=int, default=30)
+    p.add_argument("--cpu-channel-steps", type=int, default=20,
+                   help="timed steps of the 2048 x 512 channel in the CPU baseline")
+    p.add_argument("--cpu-steps", type=int, default=3,
+                   help="timed steps of the headline grid in the CPU baseline (0: skip it)")
+    p.add_argument("--cpu-numpy-size", type=int, default=1024,
+                   help="grid of the single-thread numpy oracle timing reported beside the baseline (0: skip)")
     p.add_argument("--graph", type=int, default=1, help="capture each step into a hipGraph (1) or not (0)")
     p.add_argument("--tile-steps", type=int, default=50,
                    help="steps of the same domain as a 4x4 decomposition (0: skip)")
@@ -113,21 +118,96 @@ def case(n):
                                  radius=50.0 * scale)
 
 
-def cpu_baseline(n, steps):
-    """The float32 oracle (numpy, one thread) on the same physics at n x n."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def _torch_cpu_steps(mesh, arr, params, steps, torch, warm=True):
+    """Median seconds per SSPRK2 step of the torch CPU restatement (float32),
+    after one untimed step if warm."""
+    from tests.helpers import oracle_exner
+    from oracle import torch_cpu as tc
+    so, prob = oracle_exner(mesh, arr, params, dtype=np.float32)
+    st = tc.from_oracle_state(so, torch.float32)
+    seds = np.asarray(so.seds, dtype=np.float64)
+    del so
+    m0 = torch.as_tensor(prob.mask[0])
+    fill = tc.Fill(prob.bcs)
+    if warm:
+        st, _ = tc.ssprk2_step(st, m0, fill, seds, prob.constant, prob.dx, prob.cfl)
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        st, _ = tc.ssprk2_step(st, m0, fill, seds, prob.constant, prob.dx, prob.cfl)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), bool(torch.isfinite(st["h"]).all())
+
+
+def cpu_baseline(a, n, torch):
+    """Own CPU restatement, not the reference (its JAX CPU path is not
+    runnable here): the whole-array torch restatement of oracle/torch_cpu.py
+    in float32 on torch's intra-op pool, every thread this process may use
+    (OMP_NUM_THREADS, else all cores).  Timed on the 2048 x 512 channel
+    (median of --cpu-channel-steps steps) and on the headline grid (median of
+    --cpu-steps steps); `value` is the headline grid's rate.  The numpy
+    oracle on one thread is reported beside it for information."""
+    from pyexner_amd import cases
+    threads = min(int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count()), os.cpu_count())
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        mesh, arr, params = cases.channel()
+        cells_c2 = mesh.local_shape[0] * mesh.local_shape[1]
+        t_c2, ok2 = _torch_cpu_steps(mesh, arr, params, a.cpu_channel_steps, torch)
+        del mesh, arr
+        log(f"cpu baseline: channel 2048x512 {t_c2 * 1e3:.1f} ms/step on {threads} threads")
+        out = {"unit": "cell-steps/s", "cores": threads, "kind": "port",
+               "label": "own CPU restatement, not the reference",
+               "cpu_model": _cpu_model(), "host_cores": os.cpu_count(),
+               "channel_2048x512": {"steps": a.cpu_channel_steps, "s_per_step": t_c2,
+                                    "value": cells_c2 / t_c2, "finite": ok2}}
+        if a.cpu_steps > 0:
+            mesh, arr, params = case(n)
+            t_c3, ok3 = _torch_cpu_steps(mesh, arr, params, a.cpu_steps, torch, warm=False)
+            del mesh, arr
+            log(f"cpu baseline: radial {n}^2 {t_c3:.2f} s/step on {threads} threads")
+            out.update(value=n * n / t_c3, sample=(
+                f"torch CPU float32 restatement, {threads} threads, radial dam break {n}x{n}: median of "
+                f"its first {a.cpu_steps} SSPRK2 steps ({t_c3:.2f} s/step); channel 2048x512: median of "
+                f"{a.cpu_channel_steps} steps after one untimed"))
+            out["radial"] = {"n": n, "steps": a.cpu_steps, "s_per_step": t_c3, "finite": ok3}
+        else:
+            out.update(value=cells_c2 / t_c2, sample=(
+                f"torch CPU float32 restatement, {threads} threads, channel 2048x512, median of "
+                f"{a.cpu_channel_steps} SSPRK2 steps"))
+    finally:
+        torch.set_num_threads(prev)
+    if a.cpu_numpy_size > 0:
+        out["numpy_oracle_info"] = numpy_oracle_rate(a.cpu_numpy_size, 2)
+    return out
+
+
+def numpy_oracle_rate(n, steps):
+    """The float32 numpy oracle, one thread, on the same physics at n x n
+    (information only)."""
     from tests.helpers import oracle_exner
     from oracle import steps as osteps
     mesh, arr, params = case(n)
-    s, prob = oracle_exner(mesh, arr, params, dtype=np.float32, cubic_dtype=np.float64)
-    s, _ = osteps.ssprk2_step(prob, s)  # warm caches
+    s, prob = oracle_exner(mesh, arr, params, dtype=np.float32)
+    s, _ = osteps.ssprk2_step(prob, s)
     t0 = time.perf_counter()
     for _ in range(steps):
         s, _ = osteps.ssprk2_step(prob, s)
     el = time.perf_counter() - t0
-    return {"value": n * n * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"float32 numpy oracle, radial dam break {n}x{n} (same physics scaled), "
-                      f"{steps} SSPRK2 steps, {el:.1f} s, cpu={platform.processor() or platform.machine()}, "
-                      f"host cores={os.cpu_count()}"}
+    return {"value": n * n * steps / el, "unit": "cell-steps/s", "threads": 1,
+            "sample": f"radial dam break {n}x{n}, {steps} SSPRK2 steps, {el:.1f} s"}
 
 
 def tiled_run(a, n, params, torch):
@@ -621,8 +701,8 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        log(f"cpu baseline: oracle float32 at {a.cpu_size}^2 x {a.cpu_steps} steps")
-        cpu = cpu_baseline(a.cpu_size, a.cpu_steps)
+        log(f"cpu baseline: torch CPU restatement, channel x {a.cpu_channel_steps} and {n}^2 x {a.cpu_steps}")
+        cpu = cpu_baseline(a, n, torch)
 
     out = {
         "metric": "Roe-Exner SSPRK2 cell-steps/s at 8192^2 fp32",

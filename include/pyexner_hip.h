@@ -249,6 +249,15 @@ int pxh_roe_exner_ssprk2_noalloc(const pxh_grid *grid, const pxh_fields *a, cons
                                  const pxh_fields *c, const uint8_t *mask0, const pxh_sediments *sed,
                                  const pxh_bc *bc, double cfl, void *dt_dev, int dt_ready, void *stream);
 
+/* Failure detection, no counterpart in the reference (its only guard is the dt
+ * clamp, solvers/roe_exner_solver.py:121): the number of cells of the grid
+ * whose h, hu, hv, z_b, z_b_lo or G (null pointers skipped) is NaN or
+ * infinite, written to *count_dev (device memory, one uint64_t) in stream
+ * order.  Two launches (a zeroing kernel, then the scan); the integrators'
+ * output-window loops call it once per window and read the count back with
+ * the window's output. */
+int pxh_count_nonfinite(const pxh_grid *grid, const pxh_fields *s, void *count_dev, void *stream);
+
 /* The two fused sweeps a step is built from, one launch each, for callers that
  * sequence the step themselves (ring fills or halo exchanges in between):
  *   hydro sweep: roe_solve_2D + momentum_corrections (roe_exner.py:535-612, 14-62):

@@ -123,6 +123,7 @@ _SIGS = {
     "pxh_roe_exner_ssprk2": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
                                             _P(Sediments), _P(BC), ctypes.c_double, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "pxh_count_nonfinite": (ctypes.c_int, [_P(Grid), _P(Fields), ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_roe_exner_ssprk2_noalloc": (ctypes.c_int, [_P(Grid), _P(Fields), _P(Fields), _P(Fields), ctypes.c_void_p,
                                                     _P(Sediments), _P(BC), ctypes.c_double, ctypes.c_void_p,
                                                     ctypes.c_int, ctypes.c_void_p]),

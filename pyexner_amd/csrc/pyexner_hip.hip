@@ -254,6 +254,57 @@ __global__ __launch_bounds__(FIN_THREADS) void k_dt_final(const T *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Non-finite scan: the failure flag an output window reads back
+// ---------------------------------------------------------------------------
+constexpr int NF_THREADS = 256;
+constexpr int NF_MAX_BLOCKS = 4096;
+
+template <typename T>
+struct NfSet {
+    const T *p[6];
+    int nf;
+};
+
+__global__ void k_zero_u64(unsigned long long *x) { *x = 0ull; }
+
+// Cells with a NaN or infinite value in any of the set's fields; one atomic
+// per block that found any.
+template <typename T>
+__global__ __launch_bounds__(NF_THREADS) void k_count_nonfinite(NfSet<T> s, long ncell,
+                                                                unsigned long long *count) {
+    __shared__ unsigned red[NF_THREADS / 64];
+    unsigned cnt = 0;
+    for (long i = long(blockIdx.x) * NF_THREADS + threadIdx.x; i < ncell; i += long(gridDim.x) * NF_THREADS) {
+        bool bad = false;
+        for (int f = 0; f < s.nf; ++f) bad |= !__builtin_isfinite(s.p[f][i]);
+        cnt += bad ? 1u : 0u;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < NF_THREADS / 64; ++w) tot += red[w];
+        if (tot) atomicAdd(count, tot);
+    }
+}
+
+template <typename T>
+static int count_nonfinite_t(const pxh_grid *g, const pxh_fields *s, unsigned long long *count, hipStream_t st) {
+    NfSet<T> set{};
+    const void *cand[6] = {s->h, s->hu, s->hv, s->z_b, sizeof(T) == 4 ? s->z_b_lo : nullptr, s->G};
+    for (const void *q : cand)
+        if (q) set.p[set.nf++] = static_cast<const T *>(q);
+    const long n = long(g->ny) * g->nx;
+    const long want = (n + NF_THREADS - 1) / NF_THREADS;
+    const int nb = int(want < NF_MAX_BLOCKS ? want : NF_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, count);
+    hipLaunchKernelGGL(k_count_nonfinite<T>, dim3(nb), dim3(NF_THREADS), 0, st, set, n, count);
+    return PXH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // SSPRK2 combine and init
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1219,6 +1270,14 @@ int pxh_roe_exner_bed_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_
     if (old && (hyd->h == out->h || hyd->hu == out->hu || hyd->hv == out->hv)) return PXH_E_ARG;
     return DISPATCH(g, rx_bed_sweep, g, hyd, bed, out, old, m0, sd, dt_dev, dt_partial,
                     static_cast<hipStream_t>(stream));
+}
+
+int pxh_count_nonfinite(const pxh_grid *g, const pxh_fields *s, void *count_dev, void *stream) {
+    if (!grid_ok(g) || !s || !s->h || !s->hu || !s->hv || !count_dev) return PXH_E_ARG;
+    if (s->z_b_lo && !lo_ok(g, {s})) return PXH_E_ARG;
+    auto *cnt = static_cast<unsigned long long *>(count_dev);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return g->dtype == PXH_F64 ? count_nonfinite_t<double>(g, s, cnt, st) : count_nonfinite_t<float>(g, s, cnt, st);
 }
 
 int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void *dt_out, void *stream) {

@@ -12,7 +12,7 @@ import torch
 
 from ..constants import TIMESTEP_TOL
 from ..solvers.roe_exner_solver import ExnerBackend
-from .loop import DtReader, next_target, run_exner_stepper
+from .loop import DtReader, FiniteWatch, next_target, run_exner_stepper
 from .registry import IntegratorBundle, IntegratorConfig, register_integrator_bundle
 
 last_run = {}
@@ -37,6 +37,7 @@ def run_fn_forwardeuler(state, config, io=None, mesh=None, b_mask=None):
         last_run.update(time=time, iters=iters, wall=_time.perf_counter() - a, path="stepper")
         return state
     rd = DtReader(state.h.dtype, state.h.device)
+    watch = FiniteWatch(be, state.h.device)
     time, dt, iters = 0.0, 0.0, 0
     a = _time.perf_counter()
     while time < config.end_time - TIMESTEP_TOL:
@@ -45,19 +46,24 @@ def run_fn_forwardeuler(state, config, io=None, mesh=None, b_mask=None):
             rd.start(dt_dev)
             state = sb.step_fn(state, time, dt_dev, mask, sc)
             dt = rd.value()
+            watch.dt(dt, iters + 1, time)
             time = time + dt
             iters += 1
         dt_dev = sb.compute_dt_fn(state, config.cfl, mask, sc)
         rd.start(dt_dev)
         dt = rd.value()
+        watch.dt(dt, iters + 1, time)
         target = next_target(time, config.out_freq, config.end_time)
-        if time + dt >= target - TIMESTEP_TOL:
+        closing = time + dt >= target - TIMESTEP_TOL
+        if closing:
             probe = sb.step_fn(state, time, target - time, mask, sc)
             if io is not None:
                 io.write_state(probe, mesh, mask[0])
         state = sb.step_fn(state, time, dt_dev, mask, sc)
         time = time + dt
         iters += 1
+        if closing:
+            watch.window(state, time)
     torch.cuda.synchronize()
     last_run.update(time=time, iters=iters, dt=dt, wall=_time.perf_counter() - a, path="functional")
     return state

@@ -4,8 +4,13 @@ The dt of every step is produced on the device; the host reads it through a
 side stream while the step it feeds is already running, so the GPU is not
 left idle between steps.
 """
+import math
+
 import numpy as np
 import torch
+
+from .. import _hip
+from ..state import fields_struct
 
 
 class DtReader:
@@ -28,6 +33,40 @@ class DtReader:
     def value(self):
         self.done.synchronize()
         return float(self.host.item())
+
+
+class NonFiniteState(FloatingPointError):
+    """A NaN or infinite dt, or state values, met by an output-window loop."""
+
+
+class FiniteWatch:
+    """Failure detection the reference lacks (its only guard is the dt clamp,
+    solvers/roe_exner_solver.py:121, and a NaN dt simply ends its loop): each
+    step's dt, already read on the host, must be finite, and at the end of
+    every output window pxh_count_nonfinite scans the state (h, hu, hv, z_b,
+    G) once, read back with the window.  Either failure raises
+    NonFiniteState.  A backend with check_finite = False is not watched."""
+
+    def __init__(self, backend, device):
+        self.on = backend is not None and hasattr(backend, "grid") and getattr(backend, "check_finite", True)
+        self.grid = backend.grid if self.on else None
+        self.count = torch.zeros((), dtype=torch.int64, device=device) if self.on else None
+        self.windows = 0
+
+    def dt(self, d, step, time):
+        if self.on and not math.isfinite(d):
+            raise NonFiniteState(f"dt = {d} for step {step} at t = {time:.9g}")
+
+    def window(self, state, time):
+        if not self.on:
+            return
+        _hip.call("pxh_count_nonfinite", self.grid, fields_struct(state), self.count.data_ptr(),
+                  torch.cuda.current_stream().cuda_stream)
+        n = int(self.count.item())
+        self.windows += 1
+        if n:
+            raise NonFiniteState(f"{n} cells hold NaN or infinite values at t = {time:.9g} "
+                                 f"(end of output window {self.windows})")
 
 
 def next_target(time, out_freq, end_time):
@@ -107,6 +146,7 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
     if autotune and not fe:
         tune_strip_rows(backend, state, mask, config.cfl)
     st = ExnerStepper(backend, state, mask, config.cfl)
+    watch = FiniteWatch(backend, state.h.device)
     bm = backend.boundaries
     moving = bm is not None and bm.time_dependent
     ring = DtRing(st.dt)
@@ -177,6 +217,7 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
             if fe and j > 1:
                 fe_dt(j - 1)
             d = ring.get(j - 1)  # the dt of step j (waits for step j - 1 only)
+            watch.dt(d, j, time)
             step(j, time)
             time, d_prev = time + d, d
         # the window-closing step: the CFL dt of the current state, clipped
@@ -185,6 +226,7 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
             fe_dt(j)
         d = ring.get(j)
         j += 1
+        watch.dt(d, j, time)
         if time + d >= target - TIMESTEP_TOL:
             dc = target - time
             if fe:
@@ -203,8 +245,10 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
                 d = dc
                 if io is not None:
                     io.write_state(st.state, mesh, mask[0])
-        else:
-            step(j, time)
+            time, d_prev = time + d, d
+            watch.window(st.state, time)
+            continue
+        step(j, time)
         time, d_prev = time + d, d
     torch.cuda.synchronize()
     return st.state, j, time

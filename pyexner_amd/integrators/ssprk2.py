@@ -17,7 +17,7 @@ import torch
 
 from ..constants import TIMESTEP_TOL
 from ..solvers.roe_exner_solver import ExnerBackend
-from .loop import DtReader, next_target, run_exner_stepper
+from .loop import DtReader, FiniteWatch, next_target, run_exner_stepper
 from .registry import IntegratorBundle, IntegratorConfig, register_integrator_bundle
 
 last_run = {}
@@ -65,6 +65,7 @@ def run_fn_SSPRK2(state, config, io=None, mesh=None, b_mask=None):
         last_run.update(time=time, iters=iters, wall=_time.perf_counter() - a, path="stepper")
         return state
     rd = DtReader(state.h.dtype, state.h.device)
+    watch = FiniteWatch(be, state.h.device)
     time, dt, iters = 0.0, 0.0, 0
     a = _time.perf_counter()
     while time < config.end_time - TIMESTEP_TOL:
@@ -73,20 +74,25 @@ def run_fn_SSPRK2(state, config, io=None, mesh=None, b_mask=None):
             rd.start(dt_dev)
             state = ssprk2_step(sb, sc, state, time, dt_dev, mask)
             dt = rd.value()
+            watch.dt(dt, iters + 1, time)
             time = time + dt
             iters += 1
         target = next_target(time, config.out_freq, config.end_time)
         dt_dev = sb.compute_dt_fn(state, config.cfl, mask, sc)
         rd.start(dt_dev)
         dt = rd.value()
-        if time + dt >= target - TIMESTEP_TOL:
+        watch.dt(dt, iters + 1, time)
+        closing = time + dt >= target - TIMESTEP_TOL
+        if closing:
             dt = target - time
             dt_dev = dt
         state = ssprk2_step(sb, sc, state, time, dt_dev, mask)
-        if io is not None and time + dt >= target - TIMESTEP_TOL:
+        if io is not None and closing:
             io.write_state(state, mesh, mask[0])
         time = time + dt
         iters += 1
+        if closing:
+            watch.window(state, time)
     torch.cuda.synchronize()
     last_run.update(time=time, iters=iters, dt=dt, wall=_time.perf_counter() - a, path="functional")
     return state

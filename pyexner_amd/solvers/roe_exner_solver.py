@@ -216,7 +216,7 @@ def get_mask(state, mpi_handler=None, b_mask=None):
 def config_fn_roeexner(state, mpi_handler, boundaries, dx, params):
     """roe_exner_solver.py:41-56.  Backend options (not in the reference YAML):
     ``backend: {compensated_bed: bool, exact_f32: bool, fused_stage: bool,
-    stepper_loop: bool, autotune: bool}``."""
+    stepper_loop: bool, autotune: bool, check_finite: bool}``."""
     grass = params["erosion"]["grass_factor"]
     constant = not isinstance(grass, str)
     opts = params.get("backend", {}) or {}
@@ -227,6 +227,8 @@ def config_fn_roeexner(state, mpi_handler, boundaries, dx, params):
     # fused SSPRK2 entry, hipGraph replay) unless stepper_loop is false
     backend.stepper_loop = bool(opts.get("stepper_loop", True))
     backend.autotune = opts.get("autotune")
+    # the integrators' loops raise on a non-finite dt or state (per output window)
+    backend.check_finite = bool(opts.get("check_finite", True))
     return SolverConfig(mpi_handler=mpi_handler, boundaries=boundaries, dx=dx, halo_exchange=identity_halo,
                         compute_G="constant" if constant else "MPM", compute_n="constant" if constant else "MPM",
                         backend=backend)

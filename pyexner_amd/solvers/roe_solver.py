@@ -56,6 +56,7 @@ def get_mask(state, *args, **kwargs):
 
 def config_fn_roe(state, mpi_handler, boundaries, dx, params=None):
     backend = RoeBackend(state.h.shape, state.h.dtype, dx, boundaries, device=state.h.device)
+    backend.check_finite = bool(((params or {}).get("backend") or {}).get("check_finite", True))
     return SolverConfig(mpi_handler=mpi_handler, boundaries=boundaries, dx=dx, halo_exchange=identity_halo,
                         backend=backend)
 
