@@ -148,6 +148,28 @@ def test_tiled_run_equals_single_domain(cuda, dtype, which, par, fused):
 
 
 @pytest.mark.parametrize("fused", [True, False])
+def test_tiled_run_matches_oracle_f64(cuda, fused):
+    """A 2 x 3 tiled float64 SSPRK2 run against the numpy oracle directly
+    (not only against the single-domain HIP run): 40 steps of the channel,
+    every field within 1e-12 of its largest magnitude."""
+    from oracle import steps as osteps
+    from tests.helpers import oracle_exner
+    mesh, arr, params = cases.channel(nx=70, ny=40, n_bumps=8, seed=21)
+    params = dict(params, backend={"fused_stage": fused})
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float64)
+    tiled = TiledExner(sc.backend, sg, mask, sc.boundaries, (2, 3), 0.5, fused=fused)
+    so, prob = oracle_exner(mesh, arr, params)
+    for _ in range(40):
+        tiled.ssprk2_step()
+        so, _ = osteps.ssprk2_step(prob, so)
+    torch.cuda.synchronize()
+    for f in FIELDS_EXNER:
+        got, want = tiled.gather(f).cpu().numpy(), getattr(so, f)
+        scale = max(float(np.max(np.abs(want))), 1e-30)
+        assert float(np.max(np.abs(got - want))) <= 1e-12 * scale, f
+
+
+@pytest.mark.parametrize("fused", [True, False])
 def test_tiled_graph_replays_equal_eager_steps(cuda, fused):
     """A captured tiled step replayed equals eager tiled steps, replay after replay."""
     mesh, arr, params = cases.channel(nx=90, ny=52, n_bumps=6, seed=3)
