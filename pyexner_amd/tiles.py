@@ -46,6 +46,7 @@ from .solvers.roe_exner_solver import workspace_bytes
 from .state import fields_struct
 
 HW = 2  # ghost rings on a shared side, two-sweep path
+_ALL_FIELDS = ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G", "z_b_lo")
 HW_FUSED = 3  # ghost rings on a shared side, fused path
 _SIDES = (_hip.HALO_W, _hip.HALO_N, _hip.HALO_E, _hip.HALO_S)
 _OPPOSITE = {_hip.HALO_W: _hip.HALO_E, _hip.HALO_E: _hip.HALO_W, _hip.HALO_N: _hip.HALO_S,
@@ -121,6 +122,7 @@ class Tile:
         g = backend.grid
         self.grid = _hip.Grid(e1 - e0, f1 - f0, g.dtype, g.flags, g.dx)
         sub, start = _tile_program(entries, state.h.shape, self.owned, self.ext)
+        self.sub = sub  # local indices; TiledExner folds every tile's into one arena program
         self.n_entries = len(sub)
         self._entries = torch.from_numpy(sub.view(np.uint8).copy()).to(device) if len(sub) else None
         self.programs = {}
@@ -195,6 +197,91 @@ class TiledExner:
         self._jobs = {}
         self._dt = None
         self._sweep_tabs = {}
+        self._arena_fill(handlers, values, device)
+
+    def _arena_fill(self, handlers, values, device):
+        """Every tile's buffer sets in one arena per field and set (each tile
+        a contiguous view of it), and the tiles' ring-fill programs folded
+        into one program over the arenas: a fill of every tile is one launch
+        (per dependency group set) instead of one per tile.  A tile's fill
+        reads and writes only its own cells, so the folded program is the
+        per-tile programs side by side."""
+        tiles = list(self.tiles.values())
+        off, total = [], 0
+        for t in tiles:
+            off.append(total)
+            total += t.grid.ny * t.grid.nx
+        total += total & 1  # the arena grid below is 2 x total/2
+        self._arena_cells = total
+        static = ("z", "n", "n_b")
+        names = [n for n in _ALL_FIELDS if getattr(tiles[0].a, n) is not None]
+        self._arena = {}
+        for which in ("a", "b", "c"):
+            kw = {}
+            for name in names:
+                if which != "a" and name in static:
+                    continue
+                src0 = getattr(getattr(tiles[0], which), name)
+                arena = torch.empty(total, dtype=src0.dtype, device=device)
+                for t, o in zip(tiles, off):
+                    cur = getattr(getattr(t, which), name)
+                    view = arena[o:o + cur.numel()].view(cur.shape)
+                    view.copy_(cur)
+                    kw.setdefault(id(t), {})[name] = view
+                self._arena[(which, name)] = arena
+            for t in tiles:
+                setattr(t, which, getattr(t, which).replace(**kw[id(t)]))
+        for t in tiles:
+            # the stage buffer sets share the tile's static fields
+            stat = {n: getattr(t.a, n) for n in static}
+            t.b, t.c = t.b.replace(**stat), t.c.replace(**stat)
+            t.fa, t.fb, t.fc = (fields_struct(x) for x in (t.a, t.b, t.c))
+        self._arena_fields = {}
+        for which in ("a", "b", "c"):
+            f = _hip.Fields()
+            for name in names:
+                key = ("a", name) if name in static else (which, name)
+                setattr(f, name, self._arena[key].data_ptr())
+            self._arena_fields[which] = f
+        g = self.be.grid
+        self._arena_grid = _hip.Grid(2, total // 2, g.dtype, 0, g.dx)
+        parts = []
+        for t, o in zip(tiles, off):
+            if t.n_entries:
+                e = t.sub.copy()
+                e["target"] = (e["target"].astype(np.int64) + o).astype(np.int32)
+                e["src"] = (e["src"].astype(np.int64) + o).astype(np.int32)
+                parts.append(e)
+        self.n_fill_entries = int(sum(len(p) for p in parts))
+        self._fill_progs = {}
+        if not self.n_fill_entries:
+            return
+        allp = np.concatenate(parts)
+        allp = allp[np.argsort(allp["group"], kind="stable")]
+        grp = allp["group"].astype(np.int64)
+        start = [int(v) for v in np.searchsorted(grp, np.arange(_hip.N_GROUPS + 1))]
+        self._fill_entries = torch.from_numpy(allp.view(np.uint8).copy()).to(device)
+        for key, groups in (("hy", _HYDRO), ("bed1", (1 << _hip.G_ZB) | (0 if self.be.sed.mpm else 1 << _hip.G_G)),
+                            ("bed", _BED)):
+            bc = _hip.BC()
+            bc.entries = self._fill_entries.data_ptr()
+            bc.n_entries = len(allp)
+            bc.n_handlers = len(handlers)
+            bc.group_mask = groups
+            bc.flags = _hip.BC_GROUPED
+            for k in range(_hip.N_GROUPS + 1):
+                bc.group_start[k] = start[k]
+            for k, (kind, nxn, nyn) in enumerate(handlers):
+                bc.handlers[k].kind, bc.handlers[k].nx, bc.handlers[k].ny = kind, nxn, nyn
+            set_values(bc, values)
+            self._fill_progs[key] = bc
+
+    def _fill(self, which, key):
+        """The ring fill `key` ("hy", "bed1", "bed") of buffer set `which` of
+        every tile, in one launch over the arenas."""
+        if self.n_fill_entries:
+            _hip.call("pxh_bc_apply", self._arena_grid, self._arena_fields[which], self._fill_progs[key],
+                      torch.cuda.current_stream().cuda_stream)
 
     def set_strip_rows(self, rows):
         """Owned rows per strip of every tile's fused stage sweep (8..128):
@@ -355,15 +442,13 @@ class TiledExner:
         dtp = self.dt.data_ptr()
         for t in self.tiles.values():
             _hip.call("pxh_roe_exner_hydro_sweep", t.grid, t.fields(hyd_in), t.fields(hyd_out), t.m0, dtp, st)
-            if t.n_entries:
-                _hip.call("pxh_bc_apply", t.grid, t.fields(hyd_out), t.programs["hy"], st)
+        self._fill(hyd_out, "hy")
         self.exchange(hyd_out, _HYDRO)
         out = old if old is not None else hyd_out
         for t in self.tiles.values():
             _hip.call("pxh_roe_exner_bed_sweep", t.grid, t.fields(hyd_out), t.fields(bed_in), t.fields(out),
                       t.fields(old) if old is not None else None, t.m0, self.be.sed, dtp, None, st)
-            if t.n_entries:
-                _hip.call("pxh_bc_apply", t.grid, t.fields(out), t.programs[fill_bed], st)
+        self._fill(out, fill_bed)
         self.exchange(out, _BED)
 
     def _fused_stage(self, src, dst, old):
@@ -382,15 +467,10 @@ class TiledExner:
             second = int(old is not None)
             _hip.call("pxh_roe_exner_stage_sweep_multi", g.dtype, g.flags, tab.data_ptr(), n, nsb, second, lo, sed,
                       dtp, skip, st)
-            for t in self.tiles.values():
-                if t.n_entries:
-                    _hip.call("pxh_bc_apply", t.grid, t.fields(dst), t.programs["hy"], st)
+            self._fill(dst, "hy")
             _hip.call("pxh_roe_exner_frame_sweep_multi", g.dtype, g.flags, tab.data_ptr(), n, nfb, second, sed, dtp,
                       skip, st)
-            for t in self.tiles.values():
-                if t.n_entries:
-                    _hip.call("pxh_bc_apply", t.grid, t.fields(old if old is not None else dst),
-                              t.programs["bed" if old is not None else "bed1"], st)
+            self._fill(old if old is not None else dst, "bed" if old is not None else "bed1")
             self.exchange(old if old is not None else dst, _STEP)
             return
         for t in self.tiles.values():
