@@ -77,47 +77,38 @@ def next_target(time, out_freq, end_time):
 class DtRing:
     """The dt each step of an ExnerStepper leaves on the device (the fused
     reduction's next dt), published to the host without stalling the step
-    stream: a 4-byte copy into a device ring slot on the step stream (part of
-    the step's hipGraph), then a copy of that slot into pinned host memory on
-    a side stream.  A slot is reused only after its host copy has finished."""
+    stream: a 4-byte copy of it straight into a pinned host slot on the step
+    stream (part of the step's hipGraph), and an event after the step that
+    the host waits on before reading the slot.  The host reads slot k before
+    it queues the step that writes slot k + 1, so n slots are never
+    overwritten unread."""
 
     def __init__(self, dt_dev, n=8):
         self.dt = dt_dev
         self.n = n
-        self.dev = torch.zeros(n, dtype=dt_dev.dtype, device=dt_dev.device)
         self.host = torch.zeros(n, dtype=dt_dev.dtype, pin_memory=True)
-        self.side = torch.cuda.Stream(device=dt_dev.device)
-        self.ready = [torch.cuda.Event() for _ in range(n)]
+        self.host_v = [self.host[k] for k in range(n)]
+        self.host_np = self.host.numpy()
         self.done = [torch.cuda.Event() for _ in range(n)]
-        self.used = [False] * n
 
     def before(self, k):
-        """Call on the step stream before the step whose dt goes to slot k:
-        the slot's previous host copy must be complete."""
-        s = k % self.n
-        if self.used[s]:
-            torch.cuda.current_stream().wait_event(self.done[s])
+        """Call on the step stream before the step whose dt goes to slot k
+        (nothing to wait for: the slot was read before this step was queued)."""
 
     def stage(self, k):
-        """The device part of publishing: copy dt into slot k (captured into a
-        step's graph, or issued eagerly after the step)."""
-        self.dev[k % self.n].copy_(self.dt)
+        """The device part of publishing: copy dt into host slot k (captured
+        into a step's graph, or issued eagerly after the step)."""
+        self.host_v[k % self.n].copy_(self.dt, non_blocking=True)
 
     def publish(self, k):
         """The host part, after slot k was staged on the step stream."""
-        s = k % self.n
-        self.ready[s].record(torch.cuda.current_stream())
-        with torch.cuda.stream(self.side):
-            self.side.wait_event(self.ready[s])
-            self.host[s].copy_(self.dev[s], non_blocking=True)
-            self.done[s].record(self.side)
-        self.used[s] = True
+        self.done[k % self.n].record()
 
     def get(self, k):
-        """The dt published from slot k (waits for its copy)."""
+        """The dt published from slot k (waits for the step that staged it)."""
         s = k % self.n
         self.done[s].synchronize()
-        return float(self.host[s])
+        return float(self.host_np[s])
 
 
 def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs=True, autotune=None):
@@ -131,12 +122,13 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
     then one window-closing step whose dt is clipped to the output time
     (SSPRK2.py:82-87; Forward Euler writes a probe step of the clipped
     length instead and commits the CFL step, forwardeuler.py:91-117).  That
-    decision for step j needs the dts of the steps before it only, and the
-    dt of step j - 1 comes from step j - 2's fused reduction, so the host
-    decides step j + 1 while step j runs: it waits only for the step before,
-    and the GPU always has the next step queued.  Only the window-closing
-    step waits for its own input (its dt is the clipped CFL dt of the state
-    before it).  Returns (state, iters, time)."""
+    decision for step j needs the dts of the steps before it only, and a
+    regular step needs its own dt on the device only (the fused reduction of
+    step j - 1 leaves it there), so the host queues step j, then waits for
+    its dt (for step j - 1 to end) and decides step j + 1 while step j runs:
+    the GPU always has the next step queued.  Only the window-closing step
+    waits for its own input (its dt is the clipped CFL dt of the state before
+    it).  Returns (state, iters, time)."""
     from ..autotune import tune_strip_rows
     from ..constants import TIMESTEP_TOL
     from ..stepper import ExnerStepper
@@ -216,9 +208,12 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
             j += 1
             if fe and j > 1:
                 fe_dt(j - 1)
-            d = ring.get(j - 1)  # the dt of step j (waits for step j - 1 only)
-            watch.dt(d, j, time)
+            # a regular step needs its dt on the device only: it is queued
+            # before the host waits for that dt (which waits for step j - 1),
+            # so the GPU has step j in hand when step j - 1 ends
             step(j, time)
+            d = ring.get(j - 1)  # the dt of step j
+            watch.dt(d, j, time)
             time, d_prev = time + d, d
         # the window-closing step: the CFL dt of the current state, clipped
         target = next_target(time, of, end)
