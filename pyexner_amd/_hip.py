@@ -7,7 +7,10 @@ call fails, an exception is raised.
 import ctypes
 import os
 
-_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpyexner_hip.so")
+# PYEXNER_HIP_LIB names another build of the same sources (the host-sanitized
+# one of pyexner_amd.build.build_host_sanitized, for the tests run under it)
+_LIB_PATH = os.environ.get("PYEXNER_HIP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
+                                                              "libpyexner_hip.so")
 
 PXH_F32 = 0
 PXH_F64 = 1

@@ -13,11 +13,20 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", n) for n in ("pyexner_hip.hip", "adjoint.hip")]
 OUT = os.path.join(HERE, "lib", "libpyexner_hip.so")
+# The same library with AddressSanitizer and UndefinedBehaviorSanitizer on its
+# host code (argument checks, launch planning, the ensemble plan); the device
+# code is the ordinary -O3 build.  Test infrastructure: tests/test_host_sanitized.py
+# runs the host tests against it.
+OUT_SANITIZED = os.path.join(HERE, "lib", "host-sanitized", "libpyexner_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PYEXNER_ARCH", "gfx950")
 
 FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined"]
+FLAGS_SANITIZED = ["-Xarch_host", "-O1", "-Xarch_device", "-O3", "-fno-omit-frame-pointer", f"--offload-arch={ARCH}",
+                   "-fPIC", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
+                   "-packed-fp32-ops"] + SANITIZE + ["-Xarch_host", "-fno-sanitize-recover=undefined"]
 
 
 def _check(r, cmd):
@@ -26,14 +35,22 @@ def _check(r, cmd):
         raise RuntimeError(f"hipcc failed ({r.returncode}): {' '.join(cmd)}")
 
 
-def build(verbose=False):
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    objs, procs = [], []
-    for src in SRCS:
-        obj = os.path.join(os.path.dirname(OUT), os.path.splitext(os.path.basename(src))[0] + ".o")
-        cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, src]
-        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), cmd))
-        objs.append(obj)
+def build(verbose=False, sanitized=True):
+    """Compile both translation units (and, with sanitized, their
+    host-sanitized variants) in parallel and link the libraries."""
+    outs = [(OUT, FLAGS, [])]
+    if sanitized:
+        outs.append((OUT_SANITIZED, FLAGS_SANITIZED, SANITIZE + ["-shared-libasan"]))
+    procs, plan = [], []
+    for out, flags, _ in outs:
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        objs = []
+        for src in SRCS:
+            obj = os.path.join(os.path.dirname(out), os.path.splitext(os.path.basename(src))[0] + ".o")
+            cmd = [HIPCC] + flags + ["-c", "-o", obj, src]
+            procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), cmd))
+            objs.append(obj)
+        plan.append(objs)
     # wait for every compile before reporting a failure, so that none is left
     # running (and writing its temporary object) behind the error
     results = []
@@ -44,12 +61,13 @@ def build(verbose=False):
         _check(r, r.args)
         if verbose:
             sys.stderr.write(r.stderr)
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
-    _check(subprocess.run(cmd, capture_output=True, text=True), cmd)
-    os.replace(tmp, OUT)
-    for o in objs:
-        os.remove(o)
+    for (out, _, link), objs in zip(outs, plan):
+        tmp = out + ".tmp"
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + link + ["-o", tmp] + objs
+        _check(subprocess.run(cmd, capture_output=True, text=True), cmd)
+        os.replace(tmp, out)
+        for o in objs:
+            os.remove(o)
     return OUT
 
 
