@@ -558,6 +558,7 @@ def main():
         torch.cuda.empty_cache()
         log("strip rows: " + ", ".join(f"{r}: {t:.3f} ms" for r, t in tune_ms.items()) + f" -> {rows}")
     st = ExnerStepper(sc.backend, state, mask, params["cfl"])
+    no_nodata = st.no_nodata
     init = {f: getattr(state, f).clone() for f in ("h", "hu", "hv", "z_b", "z_b_lo", "G")}
     del state
     torch.cuda.synchronize()
@@ -605,7 +606,7 @@ def main():
     # stage-sweep launch.
     be, m0 = sc.backend, st.m0
     stream = torch.cuda.current_stream().cuda_stream
-    g = be.grid
+    g = be.grid_for(st.no_nodata)  # the grid the stepper's steps run with
     fa, fb, fc = st.fa, st.fb, st.fc
     dtp = st.dt.data_ptr()
     part = be.workspace.data_ptr()
@@ -624,6 +625,13 @@ def main():
         e.record()
         return e
 
+    # the byte models, less the mask byte the stage sweeps skip on a grid
+    # without nodata cells (PXH_NO_NODATA)
+    alg_b, survey_b = dict(ALG_BYTES), dict(SURVEY_BYTES)
+    if st.no_nodata:
+        for k in ("stage_sweep", "stage_sweep_combine_dt"):
+            alg_b[k] -= 1
+            survey_b[k] -= 1
     spans = {k: [] for k in ALG_BYTES}
     steps_span = []
     torch.cuda.synchronize()
@@ -652,24 +660,24 @@ def main():
     kern = {}
     for name, lst in spans.items():
         us = sum(x.elapsed_time(y) for x, y in lst) * 1e3 / len(lst)
-        gbs = ALG_BYTES[name] * n * n / (us * 1e-6) / 1e9
-        kern[name] = {"us": round(us, 2), "launches": len(lst), "alg_bytes_per_cell": ALG_BYTES[name],
+        gbs = alg_b[name] * n * n / (us * 1e-6) / 1e9
+        kern[name] = {"us": round(us, 2), "launches": len(lst), "alg_bytes_per_cell": alg_b[name],
                       "alg_GBs": round(gbs, 1), "per_step": PER_STEP[name]}
         log(f"  {name:22s} {us:9.1f} us  {gbs:8.1f} GB/s algorithmic  ({len(lst)} launches in the step sequence)")
     step_us = sum(x.elapsed_time(y) for x, y in steps_span) * 1e3 / len(steps_span)
     log(f"  instrumented step      {step_us:9.1f} us  (frame sweeps, ring fills and events included)")
     dom = max(kern, key=lambda k: kern[k]["us"] * PER_STEP[k])
-    achieved = round(SURVEY_BYTES[dom] * n * n / (kern[dom]["us"] * 1e-6) / 1e9, 1)
+    achieved = round(survey_b[dom] * n * n / (kern[dom]["us"] * 1e-6) / 1e9, 1)
     traffic = measured_traffic(n, dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "alg_bytes_per_cell": SURVEY_BYTES[dom], "cells_per_launch": n * n,
+                "alg_bytes_per_cell": survey_b[dom], "cells_per_launch": n * n,
                 "byte_model": "hot-path survey: G recomputed, plain float32 bed"}
     roofline_stored = {"achieved": kern[dom]["alg_GBs"], "frac": round(kern[dom]["alg_GBs"] / HBM_PEAK_GBS, 4),
-                       "alg_bytes_per_cell": ALG_BYTES[dom],
+                       "alg_bytes_per_cell": alg_b[dom],
                        "byte_model": "bytes this build moves: G stored, compensated bed z_b + z_b_lo"}
-    step_bytes = sum(ALG_BYTES[k] * c for k, c in PER_STEP.items())
-    step_survey = sum(SURVEY_BYTES[k] * c for k, c in PER_STEP.items())
+    step_bytes = sum(alg_b[k] * c for k, c in PER_STEP.items())
+    step_survey = sum(survey_b[k] * c for k, c in PER_STEP.items())
 
     tile = None
     if a.tile_steps > 0 and n % 4 == 0:
@@ -722,6 +730,7 @@ def main():
                    "steps_from_initial_state": a.steps, "hipgraph": use_graph,
                    "stage": "fused (frame-local ring)", "strip_rows": rows,
                    "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}, "bed": "float32 z_b + z_b_lo (compensated)", "math": "float32 hardware rcp/rsq/log2/exp2",
+                   "mask_loads": "skipped (no nodata cell)" if no_nodata else "per row",
                    "step_alg_bytes_per_cell": step_bytes, "step_alg_GBs": round(step_bytes * value / 1e9, 1),
                    "step_survey_bytes_per_cell": step_survey,
                    "step_survey_frac": round(step_survey * value / 1e9 / HBM_PEAK_GBS, 4)},

@@ -46,6 +46,10 @@ extern "C" {
 
 /* Grid geometry. */
 #define PXH_EXACT_F32 1  /* flags bit: float32 with IEEE division/sqrt/libm (default: hardware rcp/rsq/log2/exp2) */
+/* flags bit: the caller promises that mask0 is zero in every cell (no nodata
+ * cell; the ring covered by boundary polygons), so the fused stage sweeps do
+ * not load it.  Results are those of the same call with the mask loaded. */
+#define PXH_NO_NODATA 2
 /* flags bits 8-15: owned rows per strip of the fused stage sweep (0: 32;
  * otherwise 8..128), the launch shape pyexner_amd.autotune chooses per grid. */
 #define PXH_STRIP_ROWS_SHIFT 8
@@ -54,7 +58,7 @@ extern "C" {
 typedef struct {
     int32_t ny, nx;   /* array shape, ghost ring included */
     int32_t dtype;    /* PXH_F32 or PXH_F64 */
-    int32_t flags;    /* PXH_EXACT_F32 */
+    int32_t flags;    /* PXH_EXACT_F32, PXH_NO_NODATA, PXH_STRIP_ROWS */
     double dx;        /* cell size (dx == dy) */
 } pxh_grid;
 

@@ -38,6 +38,7 @@ GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G
 
 
 EXACT_F32 = 1
+NO_NODATA = 2  # pxh_grid.flags: mask0 is zero in every cell
 
 
 class Grid(ctypes.Structure):

@@ -133,15 +133,18 @@ struct SRow {
     T zl, G;
 };
 
-// LO: the compensated-bed low part is present; MPM: G is computed, not read.
-// Both are template parameters so that the row refill has no branches inside
-// it (a branch there makes the compiler copy the fresh mask byte behind a wait
-// for every outstanding load).
-template <typename T, bool LO, bool MPM>
+// LO: the compensated-bed low part is present; MPM: G is computed, not read;
+// NN: the grid has no nodata cell, so the mask is not loaded (one memory
+// stream of the sweep fewer: 4-5% of its time at 8192^2).  All are template
+// parameters so that the row refill has no branches inside it (a branch there
+// makes the compiler copy the fresh mask byte behind a wait for every
+// outstanding load).
+template <typename T, bool LO, bool MPM, bool NN = false>
 __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, const T *G_in, const uint8_t *m0,
                                            unsigned ro, unsigned c) {
     SRow<T> q;
-    q.h = hr_load(in, m0, ro, c);
+    if constexpr (NN) q.h = hr_load_no_nodata(in, ro, c);
+    else q.h = hr_load(in, m0, ro, c);
     q.zl = T(0);
     if constexpr (LO) q.zl = ldr(zb_lo_in, ro, c);
     q.G = T(0);
@@ -167,7 +170,7 @@ static inline long seam_col_elems(int ny, int nx) { return long((nx + FO - 1) / 
 // Everything by value: a reference to a kernel argument reaches the inlined
 // body as a generic pointer, and the fields read through it land in vector
 // registers.
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, bool NN = false>
 __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, const F<T> out, const F<T> cb,
                                              const F<T> old, const uint8_t *__restrict__ m0,
                                              const T *__restrict__ dtp, int ny, int nx, T dx, const SedK &sd,
@@ -208,12 +211,12 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
         HCarry<T> hc = hcarry_at<T, FAST>(ins, m0s, row(hr0), row(b0), hr0 > 0, ucc, dx);
         T cur_zl = zero, cur_G = zero;  // bed inputs of the current hydro row
         {
-            const SRow<T> q0 = sr_load<T, LO, MPM>(ins, ins.z_b_lo, ins.G, m0s, row(hr0), ucc);
+            const SRow<T> q0 = sr_load<T, LO, MPM, NN>(ins, ins.z_b_lo, ins.G, m0s, row(hr0), ucc);
             cur_zl = q0.zl;
             cur_G = q0.G;
         }
         // the next hydro row in flight (row k+1 at hydro row k)
-        SRow<T> q1 = sr_load<T, LO, MPM>(ins, ins.z_b_lo, ins.G, m0s, row(hr0 + 1), ucc);
+        SRow<T> q1 = sr_load<T, LO, MPM, NN>(ins, ins.z_b_lo, ins.G, m0s, row(hr0 + 1), ucc);
         // MODE 1: the previous-step row k-1 in flight at hydro row k
         OR<T> o1 = {zero, zero, zero, zero, zero, zero};
         if (MODE == 1) o1 = or_load(olds, row(b0), ucc);
@@ -234,7 +237,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
             }
             const HD<T> nxt = hd_derive<T, FAST>(q1.h);
             const T nxt_zl = q1.zl, nxt_G = q1.G;
-            q1 = sr_load<T, LO, MPM>(ins, ins.z_b_lo, ins.G, m0s, row(k + 2), ucc);  // clamped
+            q1 = sr_load<T, LO, MPM, NN>(ins, ins.z_b_lo, ins.G, m0s, row(k + 2), ucc);  // clamped
             // hydro row k
             const T zb_k = hc.cur.zb;
             const int nod_k = hc.cur.flags & 2;
@@ -340,7 +343,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
     return vmin;
 }
 
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, bool NN>
 __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                              const uint8_t *__restrict__ m0,
                                                              const T *__restrict__ dtp, int ny, int nx, int sr,
@@ -348,32 +351,40 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
                                                              T *__restrict__ partial, T *__restrict__ xs) {
     __shared__ T red[FWPB];
     const StagePos p = stage_pos(ny, nx, sr);
-    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G, xs);
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, NN>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G,
+                                                             xs);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
         if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = vmin;
     }
 }
 
-// Host launcher: picks the (LO, MPM) instance for in's compensated bed and the
-// sediment mode.  In a compensated run every set (in, out, old) has z_b_lo.
+// Host launcher: picks the (LO, MPM, NN) instance for in's compensated bed,
+// the sediment mode and the grid's PXH_NO_NODATA promise.  In a compensated
+// run every set (in, out, old) has z_b_lo.
 template <typename T, bool FAST, int MODE, bool DT>
 static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
                          const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, int sr, T dx,
-                         const SedK &sd, int write_G, T *partial, T *xs) {
+                         const SedK &sd, int write_G, T *partial, T *xs, bool no_nodata) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
-#define PXH_STAGE(LO_, MPM_)                                                                                   \
-    hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_>), grid, dim3(SW * FWPB), 0, st, in, out, cb, old, \
-                       m0, dtp, ny, nx, sr, dx, sd, write_G, partial, xs)
+#define PXH_STAGE(LO_, MPM_, NN_)                                                                              \
+    hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_, NN_>), grid, dim3(SW * FWPB), 0, st, in, out, \
+                       cb, old, m0, dtp, ny, nx, sr, dx, sd, write_G, partial, xs)
+#define PXH_STAGE_NN(LO_, MPM_)             \
+    do {                                    \
+        if (no_nodata) PXH_STAGE(LO_, MPM_, true); \
+        else PXH_STAGE(LO_, MPM_, false);   \
+    } while (0)
     if constexpr (sizeof(T) == 4) {
         if (lo) {
-            if (sd.mpm) PXH_STAGE(true, true);
-            else PXH_STAGE(true, false);
+            if (sd.mpm) PXH_STAGE_NN(true, true);
+            else PXH_STAGE_NN(true, false);
             return;
         }
     }
-    if (sd.mpm) PXH_STAGE(false, true);
-    else PXH_STAGE(false, false);
+    if (sd.mpm) PXH_STAGE_NN(false, true);
+    else PXH_STAGE_NN(false, false);
+#undef PXH_STAGE_NN
 #undef PXH_STAGE
 }
 

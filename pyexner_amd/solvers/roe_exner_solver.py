@@ -106,6 +106,19 @@ class ExnerBackend:
         self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
         self.device = device
 
+    def grid_for(self, no_nodata):
+        """The grid of a call: with no_nodata (the caller checked that mask0
+        is zero in every cell) PXH_NO_NODATA is set, and the fused stage
+        sweeps skip the mask loads."""
+        if not no_nodata:
+            return self.grid
+        g = self.grid
+        key = (int(g.flags), int(g.ny), int(g.nx))
+        if getattr(self, "_nn_key", None) != key:
+            self._nn_grid = _hip.Grid(g.ny, g.nx, g.dtype, int(g.flags) | _hip.NO_NODATA, g.dx)
+            self._nn_key = key
+        return self._nn_grid
+
     def set_strip_rows(self, rows):
         """Owned rows per strip of the fused stage sweep (pyexner_amd.autotune).
         A hipGraph captured earlier keeps the launch shape it was captured
@@ -126,16 +139,18 @@ class ExnerBackend:
         if self.bc_all is not None and self.bc_all.n_entries:
             _hip.call("pxh_bc_apply", self.grid, fs, self.bc_all, stream)
 
-    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream, time=0.0, states=None):
+    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream, time=0.0, states=None, no_nodata=False):
         """One stage; time: the simulation time the ring fill sees (value
         boundaries whose bed moves with time).  states = (in, out) state
-        objects, needed by the sequential stage."""
+        objects, needed by the sequential stage.  no_nodata: mask0 is zero in
+        every cell (grid_for)."""
         if self.sequential:
             self._seq_stage(states[0], states[1], mask0_ptr, dt_ptr, stream, time)
             return
         if self.bc_step is not None:
             self.bc_step.time = float(time)
-        _hip.call("pxh_roe_exner_step", self.grid, fin, fout, mask0_ptr, self.sed, self.bc_step, dt_ptr, stream)
+        _hip.call("pxh_roe_exner_step", self.grid_for(no_nodata), fin, fout, mask0_ptr, self.sed, self.bc_step,
+                  dt_ptr, stream)
 
     def _seq_stage(self, sin, sout, mask0_ptr, dt_ptr, stream, time):
         """step_fn (roe_exner_solver.py:80-116) kernel by kernel with the ring
@@ -165,7 +180,8 @@ class ExnerBackend:
     def combine_into(self, fa, fb, fo, stream):
         _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
 
-    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False, time=0.0, states=None):
+    def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False, time=0.0, states=None,
+                    no_nodata=False):
         """dt (unless dt_ready), two stages and the combine; a is advanced in
         place and dt_ptr receives the dt of the new a.  Both stages' ring
         fills see `time`, the step's start time (SSPRK2.py:34-35).  states =
@@ -181,8 +197,8 @@ class ExnerBackend:
             return
         if self.bc_step is not None:
             self.bc_step.time = float(time)
-        _hip.call("pxh_roe_exner_ssprk2", self.grid, fa, fb, fc, mask0_ptr, self.sed, self.bc_step, float(cfl),
-                  dt_ptr, self.workspace.data_ptr(), int(bool(dt_ready)), stream)
+        _hip.call("pxh_roe_exner_ssprk2", self.grid_for(no_nodata), fa, fb, fc, mask0_ptr, self.sed, self.bc_step,
+                  float(cfl), dt_ptr, self.workspace.data_ptr(), int(bool(dt_ready)), stream)
 
 
 def _stream():

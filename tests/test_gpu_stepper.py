@@ -233,3 +233,29 @@ def test_two_sweep_entries_and_dt_finalize_equal_ssprk2(cuda):
     errs = compare(a, so, FIELDS_EXNER)
     _print("two-sweep entries x30", errs)
     assert max(errs.values()) <= F64_TOL
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_no_nodata_flag_changes_no_result(cuda, dtype):
+    """PXH_NO_NODATA only skips loading a mask that is zero: 30 SSPRK2 steps
+    of the radial dam break with and without it are equal bit for bit
+    (float32 FAST with the compensated bed, and float64), the next dt too.
+    A grid with nodata cells does not take the flag."""
+    mesh, arr, params = cases.radial_dambreak(nx=300, ny=260, dx=0.05, n_bumps=8, radius=4.0)
+    runs = []
+    for nn in (True, False):
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
+        st = ExnerStepper(sc.backend, sg, mask, 0.5)
+        assert st.no_nodata
+        st.no_nodata = nn
+        for _ in range(30):
+            st.ssprk2_step()
+        torch.cuda.synchronize()
+        runs.append(st)
+    a, b = runs
+    for f in FIELDS_EXNER + (("z_b_lo",) if a.a.z_b_lo is not None else ()):
+        assert torch.equal(getattr(a.a, f), getattr(b.a, f)), f
+    assert torch.equal(a.dt, b.dt)
+    mesh, arr, params = cases.nodata_island()
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
+    assert not ExnerStepper(sc.backend, sg, mask, 0.5).no_nodata
