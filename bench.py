@@ -556,7 +556,8 @@ def main():
         from pyexner_amd.autotune import tune_strip_rows
         rows, tune_ms = tune_strip_rows(sc.backend, state, mask, params["cfl"], steps=3, use_cache=False)
         torch.cuda.empty_cache()
-        log("strip rows: " + ", ".join(f"{r}: {t:.3f} ms" for r, t in tune_ms.items()) + f" -> {rows}")
+        log("strip rows: " + ", ".join(f"{r}: {t:.3f} ms" for r, t in tune_ms.items()) + f" -> {rows}, "
+            f"wave caps {sc.backend.stage_waves()}")
     st = ExnerStepper(sc.backend, state, mask, params["cfl"])
     no_nodata = st.no_nodata
     init = {f: getattr(state, f).clone() for f in ("h", "hu", "hv", "z_b", "z_b_lo", "G")}
@@ -729,6 +730,7 @@ def main():
                    "integrator": "SSPRK2", "cfl": 0.5, "bumps": 256 if n == 8192 else None,
                    "steps_from_initial_state": a.steps, "hipgraph": use_graph,
                    "stage": "fused (frame-local ring)", "strip_rows": rows,
+                   "stage_waves_per_simd_cap": list(sc.backend.stage_waves()),
                    "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}, "bed": "float32 z_b + z_b_lo (compensated)", "math": "float32 hardware rcp/rsq/log2/exp2",
                    "mask_loads": "skipped (no nodata cell)" if no_nodata else "per row",
                    "step_alg_bytes_per_cell": step_bytes, "step_alg_GBs": round(step_bytes * value / 1e9, 1),

@@ -54,6 +54,15 @@ extern "C" {
  * otherwise 8..128), the launch shape pyexner_amd.autotune chooses per grid. */
 #define PXH_STRIP_ROWS_SHIFT 8
 #define PXH_STRIP_ROWS(flags) (((flags) >> PXH_STRIP_ROWS_SHIFT) & 0xff)
+/* flags bits 16-19 / 20-23: the most wavefronts per SIMD of the first /
+ * second SSPRK2 stage's fused sweep (the Forward-Euler stage counts as a
+ * first stage), 0: no cap (registers set the occupancy), else 1..8.  Another
+ * launch shape pyexner_amd.autotune chooses per grid; results do not depend
+ * on it. */
+#define PXH_STAGE1_WAVES_SHIFT 16
+#define PXH_STAGE2_WAVES_SHIFT 20
+#define PXH_STAGE1_WAVES(flags) (((flags) >> PXH_STAGE1_WAVES_SHIFT) & 0xf)
+#define PXH_STAGE2_WAVES(flags) (((flags) >> PXH_STAGE2_WAVES_SHIFT) & 0xf)
 
 typedef struct {
     int32_t ny, nx;   /* array shape, ghost ring included */

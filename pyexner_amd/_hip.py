@@ -20,6 +20,9 @@ MAX_CHAIN = 6
 
 PXH_OK, PXH_E_ARG, PXH_E_RANGE, PXH_E_UNSUPPORTED = 0, -1, -2, -3
 STRIP_ROWS_SHIFT = 8  # pxh_grid.flags bits 8-15: owned rows per fused strip (0: 32)
+# pxh_grid.flags bits 16-19 / 20-23: wavefronts per SIMD of the first / second
+# stage's fused sweep (0: no cap)
+STAGE1_WAVES_SHIFT, STAGE2_WAVES_SHIFT = 16, 20
 BC_COPY, BC_REFLECT_RX, BC_REFLECT_ROE = 0, 1, 2
 BC_FRAME = 1  # pxh_bc.flags: every target on the outer ring, every source within two cells of the edge
 BC_GROUPED = 2  # pxh_bc.flags: entries ordered by group, group_start[] valid

@@ -9,8 +9,14 @@ the actual case for each candidate height on a copy of the state and keeps
 the fastest.  Results are cached per (shape, dtype, math, sediment mode) for
 the process, and in a JSON file when PYEXNER_TUNE_CACHE names one.
 
-The strip height changes which wavefront computes a cell, never what it
-computes: every candidate gives the same result bit for bit.
+After the height, a few occupancy caps are timed (PXH_STAGE1_WAVES /
+PXH_STAGE2_WAVES: the most wavefronts per SIMD of each stage's sweep, below
+the occupancy the registers allow) and the fastest kept when it beats the
+uncapped sweep by at least 1%.
+
+The strip height and the caps change which wavefront computes a cell and
+when, never what it computes: every candidate gives the same result bit for
+bit.
 """
 import json
 import os
@@ -23,6 +29,9 @@ from . import _hip
 # wavefronts to fill the 1024 SIMDs (2048 x 512 has 35 strip columns); taller
 # strips recompute fewer halo rows on large grids
 CANDIDATES = (8, 12, 16, 24, 32, 48, 64)
+# (first, second) stage caps in wavefronts per SIMD tried after the height; the
+# registers allow 4 and 3 (float32 sweeps), so each is one or two below that
+WAVE_CAPS = ((3, 0), (2, 0), (0, 2))
 _CACHE = {}
 
 
@@ -42,6 +51,14 @@ def _load_file():
         return {}
     with open(path) as f:
         return json.load(f)
+
+
+def _unpack(entry):
+    """(rows, (w1, w2)) of a cache entry (an int: rows, no caps)."""
+    if isinstance(entry, dict):
+        w = entry.get("waves") or (0, 0)
+        return int(entry["rows"]), (int(w[0]), int(w[1]))
+    return int(entry), (0, 0)
 
 
 def _store_file(key, rows):
@@ -71,26 +88,41 @@ def time_steps(be, state, mask, cfl, steps):
     return e0.elapsed_time(e1) / steps
 
 
-def tune_strip_rows(be, state, mask, cfl=0.5, candidates=CANDIDATES, steps=4, use_cache=True):
+def tune_strip_rows(be, state, mask, cfl=0.5, candidates=CANDIDATES, steps=4, use_cache=True, waves=True):
     """Choose the fused stage's strip height for backend `be` (an
-    ExnerBackend) on `state` (initialised) and apply it.  Returns (rows,
-    {rows: ms per step}); the timings are empty when the choice came from a
-    cache."""
+    ExnerBackend) on `state` (initialised), then (waves=True) the occupancy
+    caps of WAVE_CAPS, and apply them.  Returns (rows, {candidate: ms per
+    step}): heights keyed by rows, caps by "rows/w<first><second>"; the
+    timings are empty when the choice came from a cache.  be.stage_waves()
+    gives the caps kept."""
     key = _key(be)
     if use_cache:
-        rows = _CACHE.get(key) or _load_file().get(key)
-        if rows:
+        hit = _CACHE.get(key) or _load_file().get(key)
+        if hit:
+            rows, w = _unpack(hit)
             be.set_strip_rows(rows)
-            _CACHE[key] = rows
+            be.set_stage_waves(*w)
+            _CACHE[key] = {"rows": rows, "waves": list(w)}
             return rows, {}
     times = {}
+    be.set_stage_waves(0, 0)
     for rows in candidates:
         be.set_strip_rows(rows)
         times[rows] = time_steps(be, state, mask, cfl, steps)
     best = min(times, key=times.get)
     be.set_strip_rows(best)
-    _CACHE[key] = best
-    _store_file(key, best)
+    w_best, t_best = (0, 0), times[best]
+    if waves and be.dtype == torch.float32:
+        for w in WAVE_CAPS:
+            be.set_stage_waves(*w)
+            t = time_steps(be, state, mask, cfl, steps)
+            times[f"{best}/w{w[0]}{w[1]}"] = t
+            if t < 0.99 * t_best:
+                w_best, t_best = w, t
+    be.set_stage_waves(*w_best)
+    entry = {"rows": best, "waves": list(w_best)}
+    _CACHE[key] = entry
+    _store_file(key, entry)
     return best, times
 
 

@@ -38,8 +38,10 @@ constexpr int SR_STAGE = PXH_SR_STAGE;  // owned rows per fused strip unless the
 // strips, 8192 x 8192) took 820 us with one wave per block, 704 us with 4
 // waves in lockstep and 673 us with 16 (a flat stream of the same bytes:
 // 561 us).  The stage kernels themselves do not gain: 4 waves per block gave
-// 809 / 2168 us (the combine sweep then spills at 4 waves per SIMD) and
-// 815 / 1292 us at 3 waves per SIMD, against 828 / 1240 us with one.
+// 767 / 1216 us against 788 / 1193 us with one (the probe of the second
+// stage's pattern: 1070 us in lockstep blocks of 4 against 1324 us), so the
+// sweeps are held by their arithmetic's latency once the pattern improves;
+// 2 waves per block gave the same step time.
 constexpr int FWPB = PXH_FWPB;
 constexpr int SR_MAX = 128;     // the largest strip height a grid may choose
 
@@ -359,17 +361,57 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
     }
 }
 
+// Dynamic LDS that caps a sweep of FWPB-wavefront blocks at `waves`
+// wavefronts per SIMD (0: no cap; the registers set the occupancy).  A CU's
+// four SIMDs share its LDS, so with this allocation 4 waves / FWPB blocks fit
+// on a CU and one more does not (allocations rounded to 512 bytes).  Fewer
+// resident strips keep fewer row streams open at once, which the access
+// pattern alone prefers (a probe of the second stage's loads and stores:
+// 1297 us at 3 waves per SIMD, 1173 us at 2), against the latency hiding the
+// arithmetic needs; on the 8192^2 step the registers' occupancy wins (2.12 ms
+// uncapped; 2.32 / 2.48 ms with the first / second stage held one wave per
+// SIMD lower).  PXH_STAGE1_WAVES / PXH_STAGE2_WAVES; pyexner_amd.autotune
+// times the caps with the strip height.
+static inline int lds_per_cu() {
+    static int bytes = 0;  // one GPU type per process
+    if (!bytes) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
+            v > 0)
+            bytes = v;
+        else
+            bytes = 65536;
+    }
+    return bytes;
+}
+
+template <typename K>
+static size_t wave_cap_lds(K *kernel, int waves) {
+    if (waves <= 0) return 0;
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void *>(kernel)) != hipSuccess) return 0;
+    const long blocks = (4L * waves) / FWPB;
+    const long per = blocks > 0 ? (long(lds_per_cu()) / blocks) & ~511L : 0;
+    const long dyn = per - long(a.sharedSizeBytes);
+    return dyn > 0 ? size_t(dyn) : 0;
+}
+
 // Host launcher: picks the (LO, MPM, NN) instance for in's compensated bed,
 // the sediment mode and the grid's PXH_NO_NODATA promise.  In a compensated
-// run every set (in, out, old) has z_b_lo.
+// run every set (in, out, old) has z_b_lo.  waves: the occupancy cap
+// (wave_cap_lds).
 template <typename T, bool FAST, int MODE, bool DT>
 static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
                          const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, int sr, T dx,
-                         const SedK &sd, int write_G, T *partial, T *xs, bool no_nodata) {
+                         const SedK &sd, int write_G, T *partial, T *xs, bool no_nodata, int waves = 0) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
-#define PXH_STAGE(LO_, MPM_, NN_)                                                                              \
-    hipLaunchKernelGGL((k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_, NN_>), grid, dim3(SW * FWPB), 0, st, in, out, \
-                       cb, old, m0, dtp, ny, nx, sr, dx, sd, write_G, partial, xs)
+#define PXH_STAGE(LO_, MPM_, NN_)                                                                                  \
+    do {                                                                                                           \
+        auto *kern = &k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_, NN_>;                                          \
+        hipLaunchKernelGGL(kern, grid, dim3(SW * FWPB), wave_cap_lds(kern, waves), st, in, out, cb, old, m0, dtp, \
+                           ny, nx, sr, dx, sd, write_G, partial, xs);                                               \
+    } while (0)
 #define PXH_STAGE_NN(LO_, MPM_)             \
     do {                                    \
         if (no_nodata) PXH_STAGE(LO_, MPM_, true); \

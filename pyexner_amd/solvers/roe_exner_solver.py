@@ -119,6 +119,22 @@ class ExnerBackend:
             self._nn_key = key
         return self._nn_grid
 
+    def set_stage_waves(self, w1, w2):
+        """The most wavefronts per SIMD of the first / second SSPRK2 stage's
+        fused sweep (PXH_STAGE1_WAVES / PXH_STAGE2_WAVES; 0: no cap, registers
+        set the occupancy).  Results do not depend on it.  A hipGraph
+        captured earlier keeps the launch shape it was captured with."""
+        w1, w2 = int(w1), int(w2)
+        if not (0 <= w1 <= 8 and 0 <= w2 <= 8):
+            raise ValueError(f"waves per SIMD ({w1}, {w2}) outside 0..8")
+        mask = (0xF << _hip.STAGE1_WAVES_SHIFT) | (0xF << _hip.STAGE2_WAVES_SHIFT)
+        self.grid.flags = (int(self.grid.flags) & ~mask) | (w1 << _hip.STAGE1_WAVES_SHIFT) \
+            | (w2 << _hip.STAGE2_WAVES_SHIFT)
+
+    def stage_waves(self):
+        f = int(self.grid.flags)
+        return (f >> _hip.STAGE1_WAVES_SHIFT) & 0xF, (f >> _hip.STAGE2_WAVES_SHIFT) & 0xF
+
     def set_strip_rows(self, rows):
         """Owned rows per strip of the fused stage sweep (pyexner_amd.autotune).
         A hipGraph captured earlier keeps the launch shape it was captured

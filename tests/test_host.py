@@ -506,6 +506,24 @@ def test_strip_height_flags_are_checked():
         assert lib.pxh_dt_workspace_size(_hip.Grid(400, 300, _hip.PXH_F32, rows << sh, 0.1)) == -1
 
 
+def test_grid_flag_fields_match_the_c_header(tmp_path):
+    """The pxh_grid.flags fields pyexner_amd._hip packs (strip height,
+    occupancy caps, NO_NODATA, EXACT_F32) are where the header's macros read
+    them."""
+    flags = (_hip.EXACT_F32 | _hip.NO_NODATA | (24 << _hip.STRIP_ROWS_SHIFT) | (3 << _hip.STAGE1_WAVES_SHIFT)
+             | (2 << _hip.STAGE2_WAVES_SHIFT))
+    src = tmp_path / "flags.c"
+    src.write_text('#include <stdio.h>\n#include <stdint.h>\n#include "pyexner_hip.h"\nint main(void) {\n'
+                   f'    int f = {flags};\n'
+                   '    printf("%d %d %d %d %d\\n", PXH_STRIP_ROWS(f), PXH_STAGE1_WAVES(f), PXH_STAGE2_WAVES(f),\n'
+                   '           (f & PXH_NO_NODATA) != 0, (f & PXH_EXACT_F32) != 0);\n    return 0;\n}\n')
+    exe = tmp_path / "flags"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True,
+                   timeout=60)
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=60).stdout.split()
+    assert [int(x) for x in got] == [24, 3, 2, 1, 1]
+
+
 def test_dt_workspace_holds_the_seam_columns():
     """The dt workspace ends with the combine sweeps' seam-column buffer: per
     60-column strip and row, a record of the combined h, hu, hv, G and the
