@@ -19,6 +19,14 @@ here are the reference's:
 
 Float64 runs agree with the numpy restatement to rounding (tests/
 test_oracle_torch_cpu.py).  cbrt is h ** (1/3) here (torch has no cbrt).
+
+The functions are differentiable by torch autograd (oracle/torch_adjoint.py):
+a denominator or argument that only an untaken branch of a `where` uses is
+replaced there by a safe value (1, or the branch's value held constant), so
+that branch contributes a zero derivative instead of 0 * inf = NaN.  Forward
+values are unchanged: the replaced operands only ever feed discarded
+branches.  The sediment rows may hold torch scalars (d50 and porosity with
+requires_grad).
 """
 import numpy as np
 import torch
@@ -28,8 +36,33 @@ from .constants import DRY_TOL, SED_TOL, VEL_TOL, g
 FIELDS = ("h", "hu", "hv", "z", "z_b", "n", "n_b", "G")
 
 
+def _rows(r):
+    """A sediment row as a list of per-class scalars: Python floats from a
+    numpy row, torch scalars (autograd) from a tensor."""
+    if torch.is_tensor(r):
+        return [r.reshape(-1)[k] for k in range(r.numel())]
+    return [float(x) for x in np.asarray(r, dtype=np.float64).reshape(-1)]
+
+
+def _mean(row):
+    if any(torch.is_tensor(x) for x in row):
+        return torch.stack([torch.as_tensor(x, dtype=torch.float64) for x in row]).mean()
+    return float(np.mean(np.asarray(row, dtype=np.float64)))
+
+
 def _xs(f, axis):
     return (f[:, :-1], f[:, 1:]) if axis == "x" else (f[:-1, :], f[1:, :])
+
+
+def _to_lo(e, axis):
+    """Edge values e onto the full grid at each edge's first cell (zeros
+    elsewhere)."""
+    return torch.nn.functional.pad(e, (0, 1) if axis == "x" else (0, 0, 0, 1))
+
+
+def _to_hi(e, axis):
+    """Edge values e onto the full grid at each edge's second cell."""
+    return torch.nn.functional.pad(e, (1, 0) if axis == "x" else (0, 0, 1, 0))
 
 
 def get_lambda(ut, at, ct):
@@ -39,8 +72,12 @@ def get_lambda(ut, at, ct):
     # a = -1: p = (3ac - b^2) / 3, q = (2b^3 - 9abc + 27d) / -27
     p = (-3 * c - b * b) / 3
     q = (2 * (b * (b * b)) + 9 * b * c + 27 * d) / -27
-    cos_t = torch.clamp((3 * q) / (2 * p) * torch.sqrt(-3 / p), -1.0, 1.0)
-    th = torch.arccos(cos_t)
+    raw = (3 * q) / (2 * p) * torch.sqrt(-3 / p)
+    inside = (raw > -1.0) & (raw < 1.0)
+    # at the clip arccos(+-1) is held constant (the clipped argument does not
+    # move), so its derivative is zero there, not arccos' infinite one
+    th_clip = torch.arccos(torch.clamp(raw, -1.0, 1.0)).detach()
+    th = torch.where(inside, torch.arccos(torch.where(inside, raw, torch.zeros_like(raw))), th_clip)
     st = 2 * torch.sqrt(-p / 3)
     shift = b / 3
     lmax = st * torch.cos(th / 3) + shift
@@ -64,7 +101,7 @@ def _dt_edge(si, sj, nx, ny, dx):
     lmin, lmax = get_lambda(ut, at, ct)
     roe = torch.maximum(lmin.abs(), lmax.abs())
     swe = torch.maximum((ut - ct).abs(), (ut + ct).abs())
-    return torch.where(roe < VEL_TOL, dx / swe, dx / roe)
+    return dx / torch.where(roe < VEL_TOL, swe, roe)
 
 
 def dt_2d(s, m0, dx, cfl):
@@ -98,13 +135,16 @@ def _roe_edge(si, sj, nx, ny, dx):
     vt = (vni * ri + vnj * rj) / den
     l1r, l3r = ut - ct, ut + ct
     ei, ej = uni - torch.sqrt(g * hi), unj - torch.sqrt(g * hj)
+    one = torch.ones((), dtype=hi.dtype)
     f1 = (ei < 0.0) & (ej > 0.0)
-    lE1 = torch.where(f1, l1r - ei * (ej - l1r) / (ej - ei), zero)
-    l1 = torch.where(f1, ei * (ej - l1r) / (ej - ei), l1r)
+    d1 = torch.where(f1, ej - ei, one)
+    lE1 = torch.where(f1, l1r - ei * (ej - l1r) / d1, zero)
+    l1 = torch.where(f1, ei * (ej - l1r) / d1, l1r)
     ei, ej = uni + torch.sqrt(g * hi), unj + torch.sqrt(g * hj)
     f3 = (ei < 0.0) & (ej > 0.0)
-    lE3 = torch.where(f3, l3r - ej * (l3r - ei) / (ej - ei), zero)
-    l3 = torch.where(f3, ej * (l3r - ei) / ((ej - ei) + VEL_TOL), l3r)
+    d3 = torch.where(f3, ej - ei, one)
+    lE3 = torch.where(f3, l3r - ej * (l3r - ei) / d3, zero)
+    l3 = torch.where(f3, ej * (l3r - ei) / (d3 + VEL_TOL), l3r)
     dh = hj - hi
     dhu = hj * unj - hi * uni
     dhv = hj * vnj - hi * vni
@@ -116,7 +156,9 @@ def _roe_edge(si, sj, nx, ny, dx):
     zbj = torch.where(zbj > SED_TOL, zbj, torch.full_like(zbj, SED_TOL * 0.01))
     thrust = -g * ht * ((zj + zbj) - (zi + zbi))
     nt = 0.5 * (ni + nj)
-    sf = (nt * nt * torch.sqrt(ut * ut + vt * vt) * ut) / torch.clamp(ht ** (4 / 3), min=DRY_TOL)
+    s2 = ut * ut + vt * vt
+    speed = torch.where(s2 > 0.0, torch.sqrt(torch.where(s2 > 0.0, s2, one)), zero)
+    sf = (nt * nt * speed * ut) / torch.clamp(ht ** (4 / 3), min=DRY_TOL)
     b1 = (thrust - g * ht * sf * dx) / d2
     ones = torch.ones_like(vt)
     zeros = torch.zeros_like(vt)
@@ -155,9 +197,10 @@ def roe_solve_2d(s, m0, dt, dx):
         mi, mj = _xs(m0, axis)
         act = ((hi > DRY_TOL) | (hj > DRY_TOL)) & ~(mi | mj)
         for c in range(3):
-            lo, hi_ = _xs(F[c], axis)
-            lo += torch.where(act, M[c], zero)
-            hi_ += torch.where(act, P[c], zero)
+            # F[i] += M on the edge's first cell, F[j] += P on its second
+            # (written with padding, not in-place slices, for autograd)
+            F[c] = F[c] + _to_lo(torch.where(act, M[c], zero), axis)
+            F[c] = F[c] + _to_hi(torch.where(act, P[c], zero), axis)
     out = dict(s)
     for c, k in enumerate(("h", "hu", "hv")):
         out[k] = s[k] - dt * F[c] / dx
@@ -190,19 +233,21 @@ def compute_g(s, m0, seds, constant):
     hc = torch.clamp(h, min=DRY_TOL)
     u, v = s["hu"] / hc, s["hv"] / hc
     vel2 = u * u + v * v
-    frac, d50, dens, _, _, theta_c, por = (np.asarray(r, dtype=np.float64) for r in seds)
+    frac, d50, dens, theta_c, por = (_rows(seds[i]) for i in (0, 1, 2, 5, 6))
     cbrt_h = hc ** (1.0 / 3.0)
     sqrt_h = torch.sqrt(hc)
     G = torch.zeros_like(h)
+    one = torch.ones((), dtype=h.dtype)
     for k in range(len(frac)):
         sk = dens[k] / 1000.0
         npk = d50[k] ** (1 / 6) / 21.1
         tp = torch.clamp((npk * npk * vel2) / ((sk - 1.0) * d50[k] * cbrt_h), min=0.0)
         dth = torch.clamp(tp - theta_c[k], min=0.0)
         g1 = npk * npk * npk * np.sqrt(g) / ((sk - 1.0) * sqrt_h)
-        g2 = 8.0 * torch.sqrt(dth) / (tp ** 1.5 + 1e-5)
+        # sqrt(dth) at dth = 0 only feeds a term multiplied by dth = 0
+        g2 = 8.0 * torch.sqrt(torch.where(dth > 0.0, dth, one)) / (tp ** 1.5 + 1e-5)
         G = G + frac[k] * g1 * g2 * dth
-    G = G / (1.0 - por.mean())
+    G = G / (1.0 - _mean(por))
     G = torch.where(m0, zero, G)
     return torch.where(hc > DRY_TOL, G, zero)
 
@@ -231,15 +276,15 @@ def exner_solve_2d(s, m0, dt, dx):
         gt = 0.5 * (gi + gj)
         dz = zj2 - zi2
         dq = gt * mj2 * unj - gt * mi2 * uni
-        l4 = torch.where(dz.abs() > SED_TOL, dq / dz, ut)
+        big = dz.abs() > SED_TOL
+        l4 = torch.where(big, dq / torch.where(big, dz, torch.ones_like(dz)), ut)
         qi = gi * mi2 * uni + (gt - gi) * mi2 * uni
         qj = gj * mj2 * unj + (gt - gj) * mj2 * unj
         F = torch.where(l4 >= 0, qi, qj)
         stop = (mi | mj) | ((zbi < SED_TOL) & (zbj < SED_TOL))
         F = torch.where(((hi > DRY_TOL) | (hj > DRY_TOL)) & ~stop, F, zero)
-        lo, hi_ = _xs(fl, axis)
-        lo += F
-        hi_ -= F
+        fl = fl + _to_lo(F, axis)
+        fl = fl - _to_hi(F, axis)
     out = dict(s)
     out["z_b"] = zb - dt * fl / dx
     return out

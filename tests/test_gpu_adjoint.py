@@ -12,7 +12,11 @@ there almost everywhere).
 
 Nothing here has a reference vector: the reference has no gradient code, so
 the derivatives are pinned by the forward path itself through central
-differences (what jax.grad of the reference would have to agree with).
+differences (what jax.grad of the reference would have to agree with), and by
+reverse-mode torch autograd of the float64 whole-array restatement
+(oracle/torch_adjoint.py, pinned on the CPU by tests/test_oracle_torch_adjoint.py),
+the stand-in for jax.grad: the run's gradients agree with it within 1e-12
+(the z_b0 field relative to its largest entry; d50 and porosity relative).
 """
 import numpy as np
 import pytest
@@ -200,6 +204,37 @@ def test_run_gradient_matches_central_differences(cuda):
     fd = diff(outs(zb0, d50_0, por_0 + eps), outs(zb0, d50_0, por_0 - eps), eps)
     print(f"porosity: adjoint {gp:+.12e} FD {fd:+.12e}")
     assert _rel(gp, fd, 0.0) <= TOL
+
+
+def test_run_gradient_matches_torch_autograd(cuda):
+    """24 SSPRK2 steps of the 96 x 40 calibration channel (checkpoints every
+    7): the adjoint kernels' dL/dz_b0, dL/dd50, dL/dporosity against torch
+    autograd through the CPU restatement of the same run."""
+    from oracle import torch_adjoint as ta
+    from tests.test_oracle_torch_adjoint import twin_setup
+    mesh, arr, params, raw, m0, fill, seds = twin_setup(96, 40)
+    run = ExnerGradientRun(mesh, arr, params, n_steps=24, checkpoint_every=7)
+    zb0 = run.raw.z_b.clone()
+    assert torch.equal(zb0.cpu(), raw["z_b"])
+    target = seeded_target(zb0, amp=1e-3)
+    wh = _rnd(zb0.shape, _gen(8)) * 1e-2
+    d50_0, por_0 = float(run.seds[1, 0]), float(run.seds[6, 0])
+    zb = zb0.clone().requires_grad_(True)
+    d50 = torch.tensor(d50_0, dtype=torch.float64, requires_grad=True)
+    por = torch.tensor(por_0, dtype=torch.float64, requires_grad=True)
+    h, hu, hv, zbT = run(zb, d50, por)
+    (0.5 * ((zbT - target) ** 2).sum() + (wh * h).sum()).backward()
+    tgt, whc = target.cpu(), wh.cpu()
+
+    def loss_cpu(s):
+        return 0.5 * ((s["z_b"] - tgt) ** 2).sum() + (whc * s["h"]).sum()
+
+    _, gz, gd, gp = ta.gradients(raw, m0, fill, seds, mesh.dh, params["cfl"], 24, raw["z_b"], d50_0, por_0,
+                                 loss_cpu)
+    ez = float((zb.grad.cpu() - gz).abs().max() / gz.abs().max())
+    ed, ep = abs(float(d50.grad) - gd) / abs(gd), abs(float(por.grad) - gp) / abs(gp)
+    print(f"z_b0 field {ez:.2e} (max |grad| {float(gz.abs().max()):.3e}), d50 {ed:.2e}, porosity {ep:.2e}")
+    assert ez <= 1e-12 and ed <= 1e-12 and ep <= 1e-12, (ez, ed, ep)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient(cuda):
