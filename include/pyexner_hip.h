@@ -50,6 +50,11 @@ extern "C" {
  * cell; the ring covered by boundary polygons), so the fused stage sweeps do
  * not load it.  Results are those of the same call with the mask loaded. */
 #define PXH_NO_NODATA 2
+/* flags bit: the caller promises that the fixed bed z holds one value (z[0])
+ * in every cell, so the fused stage sweeps read it once instead of streaming
+ * z (honoured together with PXH_NO_NODATA, on the float64 and the float32
+ * hardware-math paths).  Results are those of the same call without it. */
+#define PXH_Z_UNIFORM 4
 /* flags bits 8-15: owned rows per strip of the fused stage sweep (0: 32;
  * otherwise 8..128), the launch shape pyexner_amd.autotune chooses per grid. */
 #define PXH_STRIP_ROWS_SHIFT 8
@@ -67,7 +72,7 @@ extern "C" {
 typedef struct {
     int32_t ny, nx;   /* array shape, ghost ring included */
     int32_t dtype;    /* PXH_F32 or PXH_F64 */
-    int32_t flags;    /* PXH_EXACT_F32, PXH_NO_NODATA, PXH_STRIP_ROWS */
+    int32_t flags;    /* PXH_EXACT_F32, PXH_NO_NODATA, PXH_Z_UNIFORM, PXH_STRIP_ROWS, PXH_STAGE*_WAVES */
     double dx;        /* cell size (dx == dy) */
 } pxh_grid;
 

@@ -42,6 +42,7 @@ GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G
 
 EXACT_F32 = 1
 NO_NODATA = 2  # pxh_grid.flags: mask0 is zero in every cell
+Z_UNIFORM = 4  # pxh_grid.flags: z holds one value (z[0]) in every cell
 
 
 class Grid(ctypes.Structure):

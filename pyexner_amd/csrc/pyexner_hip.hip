@@ -475,17 +475,17 @@ static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, con
     const F<T> none{};
     const int sr = strip_rows(g->flags);
     const dim3 fg = stage_grid(ny, nx, sr);
-    const bool nn = (g->flags & PXH_NO_NODATA) != 0;
+    const bool nn = (g->flags & PXH_NO_NODATA) != 0, zu = (g->flags & PXH_Z_UNIFORM) != 0;
     const int w1 = PXH_STAGE1_WAVES(g->flags), w2 = PXH_STAGE2_WAVES(g->flags);
     if (!old)
         launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, sr, dx, sed_k(*sd), write_G,
-                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w1);
+                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w1, zu);
     else if (part)
         launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1, part,
-                                       seam_cols<T>(g, part), nn, w2);
+                                       seam_cols<T>(g, part), nn, w2, zu);
     else
         launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1,
-                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w2);
+                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w2, zu);
     return launch_status();
 }
 

@@ -135,17 +135,25 @@ struct SRow {
     T zl, G;
 };
 
+// Static-field specialisations of the fused stage (template parameter ST):
+// ST_NN: the grid has no nodata cell, so the mask is not loaded; ST_ZU (only
+// with ST_NN): the fixed bed z holds one value, read once, so z is not
+// streamed.  Each is one memory stream of the sweep fewer: 4-5% of its time at
+// 8192^2 for the mask, 2-4% for z (the sweeps are held by their access
+// pattern, which costs about as much per stream as per byte).
+constexpr int ST_NN = 1, ST_ZU = 2;
+
 // LO: the compensated-bed low part is present; MPM: G is computed, not read;
-// NN: the grid has no nodata cell, so the mask is not loaded (one memory
-// stream of the sweep fewer: 4-5% of its time at 8192^2).  All are template
-// parameters so that the row refill has no branches inside it (a branch there
-// makes the compiler copy the fresh mask byte behind a wait for every
-// outstanding load).
-template <typename T, bool LO, bool MPM, bool NN = false>
+// ST: the static-field specialisation above; zu: the uniform z (ST_ZU).  All
+// are template parameters so that the row refill has no branches inside it (a
+// branch there makes the compiler copy the fresh mask byte behind a wait for
+// every outstanding load).
+template <typename T, bool LO, bool MPM, int ST = 0>
 __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, const T *G_in, const uint8_t *m0,
-                                           unsigned ro, unsigned c) {
+                                           unsigned ro, unsigned c, T zu = T(0)) {
     SRow<T> q;
-    if constexpr (NN) q.h = hr_load_no_nodata(in, ro, c);
+    if constexpr ((ST & ST_NN) && (ST & ST_ZU)) q.h = hr_load_static(in, ro, c, zu);
+    else if constexpr (ST & ST_NN) q.h = hr_load_no_nodata(in, ro, c);
     else q.h = hr_load(in, m0, ro, c);
     q.zl = T(0);
     if constexpr (LO) q.zl = ldr(zb_lo_in, ro, c);
@@ -172,7 +180,7 @@ static inline long seam_col_elems(int ny, int nx) { return long((nx + FO - 1) / 
 // Everything by value: a reference to a kernel argument reaches the inlined
 // body as a generic pointer, and the fields read through it land in vector
 // registers.
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, bool NN = false>
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST = 0>
 __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, const F<T> out, const F<T> cb,
                                              const F<T> old, const uint8_t *__restrict__ m0,
                                              const T *__restrict__ dtp, int ny, int nx, T dx, const SedK &sd,
@@ -192,6 +200,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
     if (p.r0 < ny) {
         const T dt = *dtp;
         const T dtdx = dt / dx;
+        const T zu = (ST & ST_ZU) ? in.z[0] : zero;  // the uniform fixed bed (ST_ZU)
         const unsigned ucc = unsigned(p.cc), uc = unsigned(c);
         const bool inE = p.lane <= SW - 3, inW = p.lane >= 2;
         const bool colI = c >= fr.lft && c < fr.rgt;           // interior column
@@ -213,12 +222,12 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
         HCarry<T> hc = hcarry_at<T, FAST>(ins, m0s, row(hr0), row(b0), hr0 > 0, ucc, dx);
         T cur_zl = zero, cur_G = zero;  // bed inputs of the current hydro row
         {
-            const SRow<T> q0 = sr_load<T, LO, MPM, NN>(ins, ins.z_b_lo, ins.G, m0s, row(hr0), ucc);
+            const SRow<T> q0 = sr_load<T, LO, MPM, ST>(ins, ins.z_b_lo, ins.G, m0s, row(hr0), ucc, zu);
             cur_zl = q0.zl;
             cur_G = q0.G;
         }
         // the next hydro row in flight (row k+1 at hydro row k)
-        SRow<T> q1 = sr_load<T, LO, MPM, NN>(ins, ins.z_b_lo, ins.G, m0s, row(hr0 + 1), ucc);
+        SRow<T> q1 = sr_load<T, LO, MPM, ST>(ins, ins.z_b_lo, ins.G, m0s, row(hr0 + 1), ucc, zu);
         // MODE 1: the previous-step row k-1 in flight at hydro row k
         OR<T> o1 = {zero, zero, zero, zero, zero, zero};
         if (MODE == 1) o1 = or_load(olds, row(b0), ucc);
@@ -239,7 +248,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
             }
             const HD<T> nxt = hd_derive<T, FAST>(q1.h);
             const T nxt_zl = q1.zl, nxt_G = q1.G;
-            q1 = sr_load<T, LO, MPM, NN>(ins, ins.z_b_lo, ins.G, m0s, row(k + 2), ucc);  // clamped
+            q1 = sr_load<T, LO, MPM, ST>(ins, ins.z_b_lo, ins.G, m0s, row(k + 2), ucc, zu);  // clamped
             // hydro row k
             const T zb_k = hc.cur.zb;
             const int nod_k = hc.cur.flags & 2;
@@ -345,7 +354,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
     return vmin;
 }
 
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, bool NN>
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST>
 __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                              const uint8_t *__restrict__ m0,
                                                              const T *__restrict__ dtp, int ny, int nx, int sr,
@@ -353,7 +362,7 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
                                                              T *__restrict__ partial, T *__restrict__ xs) {
     __shared__ T red[FWPB];
     const StagePos p = stage_pos(ny, nx, sr);
-    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, NN>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G,
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, ST>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G,
                                                              xs);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
@@ -397,25 +406,34 @@ static size_t wave_cap_lds(K *kernel, int waves) {
     return dyn > 0 ? size_t(dyn) : 0;
 }
 
-// Host launcher: picks the (LO, MPM, NN) instance for in's compensated bed,
-// the sediment mode and the grid's PXH_NO_NODATA promise.  In a compensated
-// run every set (in, out, old) has z_b_lo.  waves: the occupancy cap
-// (wave_cap_lds).
+// Host launcher: picks the (LO, MPM, ST) instance for in's compensated bed,
+// the sediment mode and the grid's PXH_NO_NODATA / PXH_Z_UNIFORM promises
+// (the uniform-z instances exist for float64 and the float32 hardware-math
+// path; the IEEE float32 path streams z).  In a compensated run every set
+// (in, out, old) has z_b_lo.  waves: the occupancy cap (wave_cap_lds).
 template <typename T, bool FAST, int MODE, bool DT>
 static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
                          const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, int sr, T dx,
-                         const SedK &sd, int write_G, T *partial, T *xs, bool no_nodata, int waves = 0) {
+                         const SedK &sd, int write_G, T *partial, T *xs, bool no_nodata, int waves = 0,
+                         bool z_uniform = false) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
-#define PXH_STAGE(LO_, MPM_, NN_)                                                                                  \
+    constexpr bool has_zu = FAST || sizeof(T) == 8;
+#define PXH_STAGE(LO_, MPM_, ST_)                                                                                  \
     do {                                                                                                           \
-        auto *kern = &k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_, NN_>;                                          \
+        auto *kern = &k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_, ST_>;                                          \
         hipLaunchKernelGGL(kern, grid, dim3(SW * FWPB), wave_cap_lds(kern, waves), st, in, out, cb, old, m0, dtp, \
                            ny, nx, sr, dx, sd, write_G, partial, xs);                                               \
     } while (0)
-#define PXH_STAGE_NN(LO_, MPM_)             \
-    do {                                    \
-        if (no_nodata) PXH_STAGE(LO_, MPM_, true); \
-        else PXH_STAGE(LO_, MPM_, false);   \
+#define PXH_STAGE_NN(LO_, MPM_)                                                 \
+    do {                                                                        \
+        if constexpr (has_zu) {                                                 \
+            if (no_nodata && z_uniform) {                                       \
+                PXH_STAGE(LO_, MPM_, ST_NN | ST_ZU);                            \
+                break;                                                          \
+            }                                                                   \
+        }                                                                       \
+        if (no_nodata) PXH_STAGE(LO_, MPM_, ST_NN);                             \
+        else PXH_STAGE(LO_, MPM_, 0);                                           \
     } while (0)
     if constexpr (sizeof(T) == 4) {
         if (lo) {

@@ -197,6 +197,18 @@ __device__ __forceinline__ HR<T> hr_load_no_nodata(const F<T> &s, RO ro, unsigne
     return q;
 }
 
+// hr_load on a grid with no nodata cell whose fixed bed z holds one value
+// zu in every cell (PXH_NO_NODATA with PXH_Z_UNIFORM): neither the mask nor z
+// is loaded.
+template <typename T, typename RO>
+__device__ __forceinline__ HR<T> hr_load_static(const F<T> &s, RO ro, unsigned c, T zu) {
+    HR<T> q;
+    q.h = ldr(s.h, ro, c); q.hu = ldr(s.hu, ro, c); q.hv = ldr(s.hv, ro, c);
+    q.zb = ldr(s.z_b, ro, c); q.z = zu; q.n = ldr(s.n_b, ro, c);
+    q.m = 0;
+    return q;
+}
+
 template <typename T, bool FAST>
 __device__ __forceinline__ HD<T> hd_derive(const HR<T> &q) {
     const T T_ = C<T>(DRY_TOL), S_ = C<T>(SED_TOL), g = C<T>(GRAV), zero = T(0);

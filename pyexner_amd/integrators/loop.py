@@ -181,7 +181,7 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
         """Launch step j (its dt is in st.dt); publish the dt it leaves."""
         if fe:
             st.be.step_into(st.fa, st.fb, st.m0, st.dt.data_ptr(), st._stream(), time=t, states=(st.a, st.b),
-                            no_nodata=st.no_nodata)
+                            no_nodata=st.no_nodata, z_uniform=st.z_uniform)
             st.a, st.b = st.b, st.a
             st.fa, st.fb = st.fb, st.fa
             return
@@ -231,7 +231,7 @@ def run_exner_stepper(state, mask, backend, config, io, mesh, integrator, graphs
                     keep_dt = st.dt.clone()
                     st.dt.fill_(dc)
                     st.be.step_into(st.fa, st.fb, st.m0, st.dt.data_ptr(), st._stream(), time=time,
-                                    states=(st.a, st.b), no_nodata=st.no_nodata)
+                                    states=(st.a, st.b), no_nodata=st.no_nodata, z_uniform=st.z_uniform)
                     io.write_state(probe, mesh, mask[0])
                     st.dt.copy_(keep_dt)
                 step(j, time)

@@ -106,18 +106,20 @@ class ExnerBackend:
         self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
         self.device = device
 
-    def grid_for(self, no_nodata):
+    def grid_for(self, no_nodata, z_uniform=False):
         """The grid of a call: with no_nodata (the caller checked that mask0
         is zero in every cell) PXH_NO_NODATA is set, and the fused stage
-        sweeps skip the mask loads."""
+        sweeps skip the mask loads; with z_uniform as well (the caller checked
+        that z holds one value) PXH_Z_UNIFORM, and they read z once."""
         if not no_nodata:
             return self.grid
         g = self.grid
-        key = (int(g.flags), int(g.ny), int(g.nx))
-        if getattr(self, "_nn_key", None) != key:
-            self._nn_grid = _hip.Grid(g.ny, g.nx, g.dtype, int(g.flags) | _hip.NO_NODATA, g.dx)
-            self._nn_key = key
-        return self._nn_grid
+        extra = _hip.NO_NODATA | (_hip.Z_UNIFORM if z_uniform else 0)
+        key = (int(g.flags), int(g.ny), int(g.nx), extra)
+        cache = self.__dict__.setdefault("_static_grids", {})
+        if key not in cache:
+            cache[key] = _hip.Grid(g.ny, g.nx, g.dtype, int(g.flags) | extra, g.dx)
+        return cache[key]
 
     def set_stage_waves(self, w1, w2):
         """The most wavefronts per SIMD of the first / second SSPRK2 stage's
@@ -155,18 +157,19 @@ class ExnerBackend:
         if self.bc_all is not None and self.bc_all.n_entries:
             _hip.call("pxh_bc_apply", self.grid, fs, self.bc_all, stream)
 
-    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream, time=0.0, states=None, no_nodata=False):
+    def step_into(self, fin, fout, mask0_ptr, dt_ptr, stream, time=0.0, states=None, no_nodata=False,
+                  z_uniform=False):
         """One stage; time: the simulation time the ring fill sees (value
         boundaries whose bed moves with time).  states = (in, out) state
         objects, needed by the sequential stage.  no_nodata: mask0 is zero in
-        every cell (grid_for)."""
+        every cell; z_uniform: z holds one value (grid_for)."""
         if self.sequential:
             self._seq_stage(states[0], states[1], mask0_ptr, dt_ptr, stream, time)
             return
         if self.bc_step is not None:
             self.bc_step.time = float(time)
-        _hip.call("pxh_roe_exner_step", self.grid_for(no_nodata), fin, fout, mask0_ptr, self.sed, self.bc_step,
-                  dt_ptr, stream)
+        _hip.call("pxh_roe_exner_step", self.grid_for(no_nodata, z_uniform), fin, fout, mask0_ptr, self.sed,
+                  self.bc_step, dt_ptr, stream)
 
     def _seq_stage(self, sin, sout, mask0_ptr, dt_ptr, stream, time):
         """step_fn (roe_exner_solver.py:80-116) kernel by kernel with the ring
@@ -197,7 +200,7 @@ class ExnerBackend:
         _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
 
     def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False, time=0.0, states=None,
-                    no_nodata=False):
+                    no_nodata=False, z_uniform=False):
         """dt (unless dt_ready), two stages and the combine; a is advanced in
         place and dt_ptr receives the dt of the new a.  Both stages' ring
         fills see `time`, the step's start time (SSPRK2.py:34-35).  states =
@@ -213,8 +216,8 @@ class ExnerBackend:
             return
         if self.bc_step is not None:
             self.bc_step.time = float(time)
-        _hip.call("pxh_roe_exner_ssprk2", self.grid_for(no_nodata), fa, fb, fc, mask0_ptr, self.sed, self.bc_step,
-                  float(cfl), dt_ptr, self.workspace.data_ptr(), int(bool(dt_ready)), stream)
+        _hip.call("pxh_roe_exner_ssprk2", self.grid_for(no_nodata, z_uniform), fa, fb, fc, mask0_ptr, self.sed,
+                  self.bc_step, float(cfl), dt_ptr, self.workspace.data_ptr(), int(bool(dt_ready)), stream)
 
 
 def _stream():

@@ -28,8 +28,10 @@ class ExnerStepper:
         self.cfl = float(cfl)
         self.mask = mask
         self.m0 = mask[0].data_ptr()
-        # no nodata cell: the fused stage sweeps need not load the mask
+        # no nodata cell: the fused stage sweeps need not load the mask; a
+        # fixed bed z of one value (a static field): they need not stream it
         self.no_nodata = not bool(mask[0].any())
+        self.z_uniform = bool((state.z == state.z.reshape(-1)[0]).all())
         self.a = state.clone()
         names = EVOLVING + (("z_b_lo",) if state.z_b_lo is not None else ())
         self.b = _like(self.a, names)
@@ -58,7 +60,8 @@ class ExnerStepper:
         with such boundaries advances self.time itself)."""
         self.be.ssprk2_into(self.fa, self.fb, self.fc, self.m0, self.cfl, self.dt.data_ptr(), self._stream(),
                             dt_ready=self.dt_ready, time=self.time if time is None else time,
-                            states=(self.a, self.b, self.c), no_nodata=self.no_nodata)
+                            states=(self.a, self.b, self.c), no_nodata=self.no_nodata,
+                            z_uniform=self.z_uniform)
         self.dt_ready = True
 
     def fe_step(self):
@@ -66,7 +69,7 @@ class ExnerStepper:
         dtp = self.dt.data_ptr()
         self.be.dt_into(self.fa, self.m0, self.cfl, dtp, st)
         self.be.step_into(self.fa, self.fb, self.m0, dtp, st, time=self.time, states=(self.a, self.b),
-                          no_nodata=self.no_nodata)
+                          no_nodata=self.no_nodata, z_uniform=self.z_uniform)
         self.a, self.b = self.b, self.a
         self.fa, self.fb = self.fb, self.fa
         self.dt_ready = False

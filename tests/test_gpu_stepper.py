@@ -259,3 +259,32 @@ def test_no_nodata_flag_changes_no_result(cuda, dtype):
     mesh, arr, params = cases.nodata_island()
     sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
     assert not ExnerStepper(sc.backend, sg, mask, 0.5).no_nodata
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_uniform_z_flag_changes_no_result(cuda, dtype):
+    """PXH_Z_UNIFORM only skips streaming a fixed bed z that holds one value:
+    30 SSPRK2 steps of the radial dam break raised onto z = 0.37 m, with and
+    without it (both without the mask loads), are equal bit for bit, dt
+    included, and equal to the same run with the mask loaded.  A varying z
+    does not take the flag."""
+    mesh, arr, params = cases.radial_dambreak(nx=300, ny=260, dx=0.05, n_bumps=8, radius=4.0)
+    arr = dict(arr, z=np.full_like(np.asarray(arr["z"], dtype=np.float64), 0.37))
+    runs = []
+    for nn, zu in ((True, True), (True, False), (False, False)):
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, dtype)
+        st = ExnerStepper(sc.backend, sg, mask, 0.5)
+        assert st.no_nodata and st.z_uniform
+        st.no_nodata, st.z_uniform = nn, zu
+        for _ in range(30):
+            st.ssprk2_step()
+        st.fe_step()
+        torch.cuda.synchronize()
+        runs.append(st)
+    for b in runs[1:]:
+        for f in FIELDS_EXNER + (("z_b_lo",) if runs[0].a.z_b_lo is not None else ()):
+            assert torch.equal(getattr(runs[0].a, f), getattr(b.a, f)), f
+        assert torch.equal(runs[0].dt, b.dt)
+    zb = dict(arr, z=np.asarray(arr["z"]) + 1e-3 * np.asarray(arr["z_b"]))
+    sg, mask, solver, sc = gpu_case(mesh, zb, params, dtype)
+    assert not ExnerStepper(sc.backend, sg, mask, 0.5).z_uniform
