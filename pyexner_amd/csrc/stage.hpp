@@ -236,7 +236,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
         DD<T> dprev{};       // combined dt values of row k-2 (DT)
         bool have_dprev = false;
         wait_vm_all();
-        for (int k = hr0; k < hr1; ++k) {
+        auto body = [&](const int k) {
             // previous-step values of row k-1, consumed before their refill
             CB<T> cbn = {zero, zero, zero, zero};
             T ozb = zero, ozl = zero;
@@ -337,6 +337,20 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
             // lockstep with the block's other strips (every wave of the block
             // runs the same rows)
             if constexpr (FWPB > 1) __syncthreads();
+        };
+        if constexpr (MODE == 0) {
+            // two rows per trip: the rows carried from one trip to the next
+            // alternate between two register sets instead of being copied
+            // (about 40 moves a row); the combine sweep's larger body gains
+            // nothing from it (measured at 8192^2: stage 1 -2.7%, stage 2 +2%)
+            int k = hr0;
+            for (; k + 1 < hr1; k += 2) {
+                body(k);
+                body(k + 1);
+            }
+            if (k < hr1) body(k);
+        } else {
+            for (int k = hr0; k < hr1; ++k) body(k);
         }
     }
     if constexpr (MODE == 1 && DT) {
