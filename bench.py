@@ -559,7 +559,7 @@ def main():
         log("strip rows: " + ", ".join(f"{r}: {t:.3f} ms" for r, t in tune_ms.items()) + f" -> {rows}, "
             f"wave caps {sc.backend.stage_waves()}")
     st = ExnerStepper(sc.backend, state, mask, params["cfl"])
-    no_nodata = st.no_nodata
+    no_nodata, z_uniform = st.no_nodata, st.z_uniform
     init = {f: getattr(state, f).clone() for f in ("h", "hu", "hv", "z_b", "z_b_lo", "G")}
     del state
     torch.cuda.synchronize()
@@ -607,7 +607,7 @@ def main():
     # stage-sweep launch.
     be, m0 = sc.backend, st.m0
     stream = torch.cuda.current_stream().cuda_stream
-    g = be.grid_for(st.no_nodata)  # the grid the stepper's steps run with
+    g = be.grid_for(st.no_nodata, st.z_uniform)  # the grid the stepper's steps run with
     fa, fb, fc = st.fa, st.fb, st.fc
     dtp = st.dt.data_ptr()
     part = be.workspace.data_ptr()
@@ -626,13 +626,15 @@ def main():
         e.record()
         return e
 
-    # the byte models, less the mask byte the stage sweeps skip on a grid
-    # without nodata cells (PXH_NO_NODATA)
+    # the bytes this build moves, less the streams the stage sweeps skip: the
+    # mask on a grid without nodata cells (PXH_NO_NODATA), z when it holds one
+    # value (PXH_Z_UNIFORM); the survey's byte model is kept as it stands
     alg_b, survey_b = dict(ALG_BYTES), dict(SURVEY_BYTES)
-    if st.no_nodata:
-        for k in ("stage_sweep", "stage_sweep_combine_dt"):
+    for k in ("stage_sweep", "stage_sweep_combine_dt"):
+        if st.no_nodata:
             alg_b[k] -= 1
-            survey_b[k] -= 1
+        if st.no_nodata and st.z_uniform:
+            alg_b[k] -= 4
     spans = {k: [] for k in ALG_BYTES}
     steps_span = []
     torch.cuda.synchronize()
@@ -733,6 +735,7 @@ def main():
                    "stage_waves_per_simd_cap": list(sc.backend.stage_waves()),
                    "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune_ms.items()}, "bed": "float32 z_b + z_b_lo (compensated)", "math": "float32 hardware rcp/rsq/log2/exp2",
                    "mask_loads": "skipped (no nodata cell)" if no_nodata else "per row",
+                   "z_loads": "read once (z uniform)" if no_nodata and z_uniform else "per row",
                    "step_alg_bytes_per_cell": step_bytes, "step_alg_GBs": round(step_bytes * value / 1e9, 1),
                    "step_survey_bytes_per_cell": step_survey,
                    "step_survey_frac": round(step_survey * value / 1e9 / HBM_PEAK_GBS, 4)},
