@@ -341,8 +341,8 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
         if constexpr (MODE == 0) {
             // two rows per trip: the rows carried from one trip to the next
             // alternate between two register sets instead of being copied
-            // (about 40 moves a row); the combine sweep's larger body gains
-            // nothing from it (measured at 8192^2: stage 1 -2.7%, stage 2 +2%)
+            // (about 40 moves a row); the combine sweep's larger body loses
+            // by it (measured at 8192^2: stage 1 -5%, stage 2 +4%)
             int k = hr0;
             for (; k + 1 < hr1; k += 2) {
                 body(k);
