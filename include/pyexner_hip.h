@@ -48,12 +48,16 @@ extern "C" {
 #define PXH_EXACT_F32 1  /* flags bit: float32 with IEEE division/sqrt/libm (default: hardware rcp/rsq/log2/exp2) */
 /* flags bit: the caller promises that mask0 is zero in every cell (no nodata
  * cell; the ring covered by boundary polygons), so the fused stage sweeps do
- * not load it.  Results are those of the same call with the mask loaded. */
+ * not load it.  Results are those of the same call with the mask loaded.
+ * In the flags of pxh_ensemble_plan and pxh_roe_exner_stage_sweep_multi the
+ * promise holds for every member / domain. */
 #define PXH_NO_NODATA 2
 /* flags bit: the caller promises that the fixed bed z holds one value (z[0])
  * in every cell, so the fused stage sweeps read it once instead of streaming
  * z (honoured together with PXH_NO_NODATA, on the float64 and the float32
- * hardware-math paths).  Results are those of the same call without it. */
+ * hardware-math paths).  Results are those of the same call without it.
+ * For an ensemble or a multi-domain sweep: each member's / domain's z holds
+ * one value, its own first cell's. */
 #define PXH_Z_UNIFORM 4
 /* flags bits 8-15: owned rows per strip of the fused stage sweep (0: 32;
  * otherwise 8..128), the launch shape pyexner_amd.autotune chooses per grid. */

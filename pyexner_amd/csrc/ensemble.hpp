@@ -49,7 +49,9 @@ __device__ __forceinline__ int find_member(const EnsMember *__restrict__ tab, in
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM>
+// ST: the static-field specialisation of stage_strip_run, a promise that
+// holds for every member (each member's uniform z is its own first cell).
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST>
 __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                         const uint8_t *__restrict__ m0, const T *__restrict__ dts,
                                                         const EnsMember *__restrict__ tab, int n, int sr,
@@ -63,10 +65,10 @@ __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> 
     const int local = lb - m.sblk0;
     const int by = local / m.sgx, bx = local - by * m.sgx;
     const StagePos p = stage_pos_at(bx, by, m.ny, m.nx, sr);
-    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM>(p, shifted(in, m.off), shifted(out, m.off),
-                                                         shifted(cb, m.off), shifted(old, m.off), m0 + m.off,
-                                                         dts + i, m.ny, m.nx, static_cast<T>(m.dx), seds[i],
-                                                         write_G, DT ? xs + m.xs0 : nullptr);
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, ST>(p, shifted(in, m.off), shifted(out, m.off),
+                                                             shifted(cb, m.off), shifted(old, m.off), m0 + m.off,
+                                                             dts + i, m.ny, m.nx, static_cast<T>(m.dx), seds[i],
+                                                             write_G, DT ? xs + m.xs0 : nullptr);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
         if (threadIdx.x == 0) partial[m.part0 + local] = vmin;
@@ -146,8 +148,9 @@ __device__ __forceinline__ int find_domain(const pxh_sweep_domain *__restrict__ 
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// The fused stage sweep over several domains (pxh_roe_exner_stage_sweep_multi).
-template <typename T, bool FAST, int MODE, bool LO, bool MPM>
+// The fused stage sweep over several domains (pxh_roe_exner_stage_sweep_multi);
+// ST as for k_ens_stage, holding for every domain.
+template <typename T, bool FAST, int MODE, bool LO, bool MPM, int ST>
 __global__ __launch_bounds__(SW *FWPB) void k_multi_stage(const pxh_sweep_domain *__restrict__ doms, int n, int sr,
                                                           const T *__restrict__ dtp, SedK sd, int write_G) {
     int lb, unused;
@@ -160,13 +163,13 @@ __global__ __launch_bounds__(SW *FWPB) void k_multi_stage(const pxh_sweep_domain
     const StagePos p = stage_pos_at(bx, by, ny, nx, sr);
     const F<T> none{};
     if constexpr (MODE == 0)
-        stage_strip_run<T, FAST, 0, false, LO, MPM>(p, dev_fields<T>(d.in), dev_fields<T>(d.out), none, none,
-                                                    d.mask0, dtp, ny, nx, static_cast<T>(d.dx), sd, write_G,
-                                                    nullptr);
+        stage_strip_run<T, FAST, 0, false, LO, MPM, ST>(p, dev_fields<T>(d.in), dev_fields<T>(d.out), none, none,
+                                                        d.mask0, dtp, ny, nx, static_cast<T>(d.dx), sd, write_G,
+                                                        nullptr);
     else
-        stage_strip_run<T, FAST, 1, false, LO, MPM>(p, dev_fields<T>(d.in), none, dev_fields<T>(d.out),
-                                                    dev_fields<T>(d.old), d.mask0, dtp, ny, nx,
-                                                    static_cast<T>(d.dx), sd, write_G, nullptr);
+        stage_strip_run<T, FAST, 1, false, LO, MPM, ST>(p, dev_fields<T>(d.in), none, dev_fields<T>(d.out),
+                                                        dev_fields<T>(d.old), d.mask0, dtp, ny, nx,
+                                                        static_cast<T>(d.dx), sd, write_G, nullptr);
 }
 
 // The frame sweep over several domains (pxh_roe_exner_frame_sweep_multi):

@@ -1019,9 +1019,23 @@ static void ens_stage(const EnsPlanHeader &h, const EnsMember *tab, const SedK *
                       const F<T> &out, const F<T> &cb, const F<T> &old, const uint8_t *m0, const T *dts,
                       int write_G, T *partial, T *xs) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
-#define PXH_ENS(LO_, MPM_)                                                                                       \
-    hipLaunchKernelGGL((k_ens_stage<T, FAST, MODE, DT, LO_, MPM_>), dim3(h.sblocks), dim3(SW * FWPB), 0, st, in,  \
-                       out, cb, old, m0, dts, tab, h.n, strip_rows(h.flags), seds, write_G, partial, xs)
+    // the plan's PXH_NO_NODATA / PXH_Z_UNIFORM promises, as launch_stage takes the grid's
+    const bool nn = (h.flags & PXH_NO_NODATA) != 0, zu = nn && (h.flags & PXH_Z_UNIFORM) != 0;
+    constexpr bool has_zu = FAST || sizeof(T) == 8;
+#define PXH_ENS_ST(LO_, MPM_, ST_)                                                                                  \
+    hipLaunchKernelGGL((k_ens_stage<T, FAST, MODE, DT, LO_, MPM_, ST_>), dim3(h.sblocks), dim3(SW * FWPB), 0, st,  \
+                       in, out, cb, old, m0, dts, tab, h.n, strip_rows(h.flags), seds, write_G, partial, xs)
+#define PXH_ENS(LO_, MPM_)                              \
+    do {                                                \
+        if constexpr (has_zu) {                         \
+            if (zu) {                                   \
+                PXH_ENS_ST(LO_, MPM_, ST_NN | ST_ZU);   \
+                break;                                  \
+            }                                           \
+        }                                               \
+        if (nn) PXH_ENS_ST(LO_, MPM_, ST_NN);           \
+        else PXH_ENS_ST(LO_, MPM_, 0);                  \
+    } while (0)
     if constexpr (sizeof(T) == 4) {
         if (lo) {
             if (h.mpm) PXH_ENS(true, true);
@@ -1032,6 +1046,7 @@ static void ens_stage(const EnsPlanHeader &h, const EnsMember *tab, const SedK *
     if (h.mpm) PXH_ENS(false, true);
     else PXH_ENS(false, false);
 #undef PXH_ENS
+#undef PXH_ENS_ST
 }
 
 template <typename T, bool FAST>
@@ -1112,12 +1127,33 @@ using namespace pxh;
 #define DISPATCH(g, fn, ...) ((g)->dtype == PXH_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
 // The multi-domain fused sweeps (pxh_roe_exner_stage_sweep_multi / _frame_sweep_multi).
+// flags: PXH_NO_NODATA / PXH_Z_UNIFORM, promises for every domain, picked as
+// launch_stage picks them for one grid.
+template <typename T, bool FAST, int MODE, bool LO, bool MPM>
+static void multi_stage_k(const pxh_sweep_domain *doms, int n, int nblocks, int sr, int flags, const T *dtp,
+                          const SedK &sd, int write_G, hipStream_t st) {
+    const bool nn = (flags & PXH_NO_NODATA) != 0, zu = nn && (flags & PXH_Z_UNIFORM) != 0;
+    const dim3 grid(nblocks), block(SW * FWPB);
+    if constexpr (FAST || sizeof(T) == 8) {
+        if (zu) {
+            hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, ST_NN | ST_ZU>), grid, block, 0, st, doms, n,
+                               sr, dtp, sd, write_G);
+            return;
+        }
+    }
+    if (nn)
+        hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, ST_NN>), grid, block, 0, st, doms, n, sr, dtp, sd,
+                           write_G);
+    else
+        hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, 0>), grid, block, 0, st, doms, n, sr, dtp, sd,
+                           write_G);
+}
+
 template <typename T, bool FAST>
-static void multi_stage_t(const pxh_sweep_domain *doms, int n, int nblocks, int sr, bool second, bool lo,
+static void multi_stage_t(const pxh_sweep_domain *doms, int n, int nblocks, int sr, bool second, bool lo, int flags,
                           const T *dtp, const SedK &sd, int write_G, hipStream_t st) {
-#define PXH_MS(MODE_, LO_, MPM_)                                                                            \
-    hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE_, LO_, MPM_>), dim3(nblocks), dim3(SW * FWPB), 0, st, doms, n, \
-                       sr, dtp, sd, write_G)
+#define PXH_MS(MODE_, LO_, MPM_) \
+    multi_stage_k<T, FAST, MODE_, LO_, MPM_>(doms, n, nblocks, sr, flags, dtp, sd, write_G, st)
 #define PXH_MS_MODE(LO_, MPM_)         \
     if (second) PXH_MS(1, LO_, MPM_); \
     else PXH_MS(0, LO_, MPM_)
@@ -1366,14 +1402,14 @@ int pxh_roe_exner_stage_sweep_multi(int32_t dtype, int32_t flags, const pxh_swee
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int sr = strip_rows(flags);
     if (dtype == PXH_F64)
-        multi_stage_t<double, false>(doms, n, nblocks, sr, second, false, static_cast<const double *>(dt_dev),
+        multi_stage_t<double, false>(doms, n, nblocks, sr, second, false, flags, static_cast<const double *>(dt_dev),
                                      sed_k(*sd), write_G, st);
     else if (flags & PXH_EXACT_F32)
-        multi_stage_t<float, false>(doms, n, nblocks, sr, second, lo, static_cast<const float *>(dt_dev), sed_k(*sd),
-                                    write_G, st);
+        multi_stage_t<float, false>(doms, n, nblocks, sr, second, lo, flags, static_cast<const float *>(dt_dev),
+                                    sed_k(*sd), write_G, st);
     else
-        multi_stage_t<float, true>(doms, n, nblocks, sr, second, lo, static_cast<const float *>(dt_dev), sed_k(*sd),
-                                   write_G, st);
+        multi_stage_t<float, true>(doms, n, nblocks, sr, second, lo, flags, static_cast<const float *>(dt_dev),
+                                   sed_k(*sd), write_G, st);
     return launch_status();
 }
 

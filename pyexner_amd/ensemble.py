@@ -26,7 +26,11 @@ class Ensemble:
     """Members given as (mesh, arrays, params) triples, all Roe-Exner with
     frame-local rings, one dtype; `cfl` is shared (each member its own dt)."""
 
-    def __init__(self, members, dtype=torch.float32, device="cuda", cfl=0.5, exact_f32=False, keep_members=False):
+    def __init__(self, members, dtype=torch.float32, device="cuda", cfl=0.5, exact_f32=False, keep_members=False,
+                 static=True):
+        """static: check once whether no member has a nodata cell and each
+        member's z holds one value, and if so let the fused stage sweeps skip
+        those streams (PXH_NO_NODATA / PXH_Z_UNIFORM; results unchanged)."""
         self.n = len(members)
         # with keep_members, (backend, state, mask) of every member set up alone
         # (for comparisons against single-domain stepping)
@@ -101,7 +105,11 @@ class Ensemble:
             self._mem[k].off, self._mem[k].ny, self._mem[k].nx = int(self.offsets[k]), shape[0], shape[1]
             self._mem[k].dx, self._mem[k].sed = dx, sed
         self._code = _hip.PXH_F32 if dtype == torch.float32 else _hip.PXH_F64
-        self._flags = meta[0][3]
+        self.no_nodata = not bool(self.mask.any())
+        self.z_uniform = self.no_nodata and all(bool((s.z == s.z.reshape(-1)[0]).all()) for s in states)
+        self._flags = int(meta[0][3]) & ~(_hip.NO_NODATA | _hip.Z_UNIFORM)
+        if static and self.no_nodata:
+            self._flags |= _hip.NO_NODATA | (_hip.Z_UNIFORM if self.z_uniform else 0)
         self.device = device
         self.workspace = None
         self._build_plan()

@@ -162,12 +162,16 @@ class TiledExner:
     """Roe-Exner SSPRK2 on a par = (parNy, parNx) decomposition of an
     initialised single-domain state (init_fn applied), every tile on this device."""
 
-    def __init__(self, backend, state, mask, boundaries, par, cfl, fused=None, direct=True, multi=True):
+    def __init__(self, backend, state, mask, boundaries, par, cfl, fused=None, direct=True, multi=True,
+                 static=True):
         """fused: the fused-stage path (three ghost rings); default when the
         ring fill is frame-local.  direct: exchange ghost strips by batched
         device copies (default) rather than pack/unpack per side.  multi: run
         every tile's fused sweeps in one launch each (default) rather than
-        one launch per tile."""
+        one launch per tile.  static: check once whether no tile has a
+        nodata cell and each tile's z holds one value, and if so let the
+        fused stage sweeps skip those streams (PXH_NO_NODATA /
+        PXH_Z_UNIFORM; results unchanged)."""
         self.be = backend
         self.cfl = float(cfl)
         self.par = tuple(par)
@@ -190,6 +194,13 @@ class TiledExner:
         self.hw = HW_FUSED if self.fused else HW
         self.tiles = {(ty, tx): Tile(backend, state, mask, entries, handlers, (ty, tx), self.par, self.rows,
                                      self.cols, device, values, self.hw) for ty in range(pny) for tx in range(pnx)}
+        ts = list(self.tiles.values())
+        self.no_nodata = not any(bool(t.mask[0].any()) for t in ts)
+        self.z_uniform = self.no_nodata and all(bool((t.a.z == t.a.z.reshape(-1)[0]).all()) for t in ts)
+        if static and self.no_nodata:
+            extra = _hip.NO_NODATA | (_hip.Z_UNIFORM if self.z_uniform else 0)
+            for t in ts:
+                t.grid.flags = int(t.grid.flags) | extra
         self.dt = torch.zeros((), dtype=state.h.dtype, device=device)
         self.dt_ready = False
         self.direct = bool(direct)

@@ -238,3 +238,25 @@ def test_tile_strip_height_changes_no_result(cuda):
     assert torch.equal(runs[0][0], runs[1][0])
     for f in FIELDS_EXNER:
         assert torch.equal(runs[0][1][f], runs[1][1][f]), f
+
+
+@pytest.mark.parametrize("multi", [True, False])
+def test_tile_static_flags_change_no_result(cuda, multi):
+    """With no nodata cell and a uniform z the tiles' fused stage sweeps skip
+    the mask and z streams (PXH_NO_NODATA | PXH_Z_UNIFORM in every tile's
+    grid, through the multi-domain launch and through one launch per tile):
+    float32 tiled runs with and without the promises are equal bit for bit."""
+    mesh, arr, params = cases.radial_dambreak(nx=130, ny=140, dx=0.05, n_bumps=8, radius=2.0)
+    sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+    runs = []
+    for static in (True, False):
+        tl = TiledExner(sc.backend, sg, mask, sc.boundaries, (2, 2), 0.5, multi=multi, static=static)
+        flags = {int(t.grid.flags) & (_hip.NO_NODATA | _hip.Z_UNIFORM) for t in tl.tiles.values()}
+        assert flags == {_hip.NO_NODATA | _hip.Z_UNIFORM if static else 0}
+        for _ in range(20):
+            tl.ssprk2_step()
+        torch.cuda.synchronize()
+        runs.append((tl.dt.clone(), {f: tl.gather(f) for f in FIELDS_EXNER + ("z_b_lo",)}))
+    assert torch.equal(runs[0][0], runs[1][0])
+    for f in runs[0][1]:
+        assert torch.equal(runs[0][1][f], runs[1][1][f]), f
