@@ -459,7 +459,8 @@ def adjoint_run(a, torch):
     calibration channel in float64, checkpoint every a.adj_every steps.  Times
     the forward pass (fused steps + checkpoints), the backward pass (segment
     replays + adjoint steps + init adjoint), and the same number of plain
-    float64 forward steps."""
+    float64 forward steps; with --autotune the float64 strip height is tuned
+    on this grid first (it sets the forward sweeps of both)."""
     from pyexner_amd import cases
     from pyexner_amd.adjoint import ExnerGradientRun, seeded_target
     from pyexner_amd.stepper import ExnerStepper
@@ -485,6 +486,11 @@ def adjoint_run(a, torch):
         return t1 - t0, t2 - t1, float(loss.detach()), zb.grad, float(d50.grad), float(por.grad)
 
     once()  # warm-up
+    rows64 = 32
+    if a.autotune:
+        # 32-row strips give this 1024 x 256 grid 144 wavefronts for 1024 SIMDs
+        from pyexner_amd.autotune import tune_strip_rows
+        rows64, _ = tune_strip_rows(run.be, run.s0, run.mask, run.cfl, steps=5, use_cache=False)
     tf, tb, loss, g, gd, gp = once()
     # plain float64 forward steps of the same case (fused SSPRK2, no checkpoints)
     stp = ExnerStepper(run.be, run.s0, run.mask, run.cfl)
@@ -519,9 +525,10 @@ def adjoint_run(a, torch):
     if not torch.isfinite(g).all():
         raise RuntimeError("non-finite bed gradient")
     log(f"adjoint: {a.adj_steps} steps 1024x256 f64: forward {tf * 1e3:.1f} ms, backward {tb * 1e3:.1f} ms; "
-        f"plain forward step {ms_fwd:.3f} ms, backward per step {ms_bwd:.3f} ms ({ms_bwd / ms_fwd:.2f}x)")
+        f"plain forward step {ms_fwd:.3f} ms, backward per step {ms_bwd:.3f} ms ({ms_bwd / ms_fwd:.2f}x), "
+        f"strip rows {rows64}; float32 forward step {ms_fwd32:.3f} ms ({ms_bwd / ms_fwd32:.2f}x)")
     return {"case": "calibration_channel 1024x256, dx 0.02, seed 5; target z_b0 + seeded perturbation (seed 6)",
-            "dtype": "f64", "steps": a.adj_steps, "checkpoint_every": a.adj_every,
+            "dtype": "f64", "steps": a.adj_steps, "checkpoint_every": a.adj_every, "strip_rows": rows64,
             "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "plain_forward_ms_per_step": ms_fwd,
             "backward_ms_per_step": ms_bwd, "adjoint_step_vs_forward_step": ms_bwd / ms_fwd,
             "f32_forward_ms_per_step": ms_fwd32, "f32_forward_strip_rows": rows32,

@@ -396,8 +396,7 @@ int pxh_ensemble_ssprk2(const void *plan, const void *plan_dev, const pxh_fields
  * with respect to the first sediment class's A, B, C (pxh_sediments),
  * one_minus_por and nb_sed, accumulated (may be NULL).  The ring-fill program
  * must not read a cell it writes.  workspace: pxh_adj_workspace_size bytes,
- * sized with the program of most entries that the calls will pass (may be
- * NULL when they pass none). */
+ * which depend on the grid only (largest_bc, may be NULL, is validated). */
 int64_t pxh_adj_workspace_size(const pxh_grid *grid, const pxh_bc *largest_bc);
 /* One stage (pxh_roe_exner_step in -> out at dt): lam_out (h, hu, hv, z_b, G)
  * is the adjoint of out and is consumed; the adjoint of in accumulates into

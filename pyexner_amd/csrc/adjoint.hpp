@@ -84,13 +84,13 @@ __device__ __forceinline__ void block_sum_to(double v, double *slot) {
 // lam[target] = 0.  A program's targets are never its sources, so this is
 // the transpose of the forward fill whatever the order.  The momentum
 // transform of an entry is a product of symmetric reflections, so its
-// transpose applies the same reflections in reverse order.  Two passes:
-// k_adj_bc_take moves each entry's (transformed) target adjoint into
-// take[2k, 2k+1] and zeroes the target; k_adj_bc_give adds them to the
-// sources.  With src_sorted (entries of one group ordered by src) the first
-// entry of each run of equal sources adds the run's sum, in entry order, so
-// the result does not depend on scheduling; otherwise every entry adds its
-// own with an atomic.
+// transpose applies the same reflections in reverse order.  One pass: an
+// entry moves its (transformed) target adjoint to its source and zeroes the
+// target; no cell is both (a program of one dependency level), so no entry
+// reads what another writes.  With src_sorted (entries of one group ordered
+// by src) the first entry of each run of equal sources takes the run's
+// targets and adds their sum, in entry order, so the result does not depend
+// on scheduling; otherwise every entry adds its own with an atomic.
 __device__ __forceinline__ double *adj_field(const F<double> &lam, int group, int q) {
     switch (group) {
         case PXH_G_H: return q ? nullptr : lam.h;
@@ -102,20 +102,20 @@ __device__ __forceinline__ double *adj_field(const F<double> &lam, int group, in
     }
 }
 
-__global__ __launch_bounds__(256) void k_adj_bc_take(F<double> lam, const pxh_bc_entry *__restrict__ ent, int n,
-                                                     pxh_bc bc, uint32_t groups, double *__restrict__ take) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const pxh_bc_entry e = ent[k];
-    double a = 0.0, b = 0.0;
-    if (((groups >> e.group) & 1u) && (e.nchain & PXH_BC_VALUE)) {
+// Entry e's target adjoint, transformed for its source, into (a, b); the
+// target zeroed.
+__device__ __forceinline__ void bc_take(const F<double> &lam, const pxh_bc_entry &e, const pxh_bc &bc, double &a,
+                                        double &b) {
+    a = 0.0;
+    b = 0.0;
+    double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
+    const int t = e.target;
+    if (e.nchain & PXH_BC_VALUE) {
         // a set value carries no derivative from the state (the target's
         // adjoint stops there); an offset passes it on like a copy; a NaN
         // momentum component leaves the cell as it was (identity)
         pxh_bc_value v{};
         if (e.chain[0] < bc.n_values) v = bc.values[e.chain[0]];
-        double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
-        const int t = e.target;
         const bool keep0 = v.kind == PXH_BV_CONST && e.group == PXH_G_HUV && !(v.a == v.a);
         const bool keep1 = v.kind == PXH_BV_CONST && e.group == PXH_G_HUV && !(v.b == v.b);
         if (f0 && !keep0) {
@@ -123,39 +123,37 @@ __global__ __launch_bounds__(256) void k_adj_bc_take(F<double> lam, const pxh_bc
             f0[t] = 0.0;
         }
         if (f1 && !keep1) f1[t] = 0.0;
-    } else if ((groups >> e.group) & 1u) {
-        double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
-        const int t = e.target;
-        if (f0) {
-            a = f0[t];
-            f0[t] = 0.0;
-        }
-        if (f1) {
-            b = f1[t];
-            f1[t] = 0.0;
-        }
-        if (e.group == PXH_G_HUV)
-            for (int q = e.nchain - 1; q >= 0; --q) bc_transform<double>(bc.handlers[e.chain[q]], a, b);
+        return;
     }
-    take[2 * long(k)] = a;
-    take[2 * long(k) + 1] = b;
+    if (f0) {
+        a = f0[t];
+        f0[t] = 0.0;
+    }
+    if (f1) {
+        b = f1[t];
+        f1[t] = 0.0;
+    }
+    if (e.group == PXH_G_HUV)
+        for (int q = e.nchain - 1; q >= 0; --q) bc_transform<double>(bc.handlers[e.chain[q]], a, b);
 }
 
-__global__ __launch_bounds__(256) void k_adj_bc_give(F<double> lam, const pxh_bc_entry *__restrict__ ent, int n,
-                                                     uint32_t groups, int src_sorted,
-                                                     const double *__restrict__ take) {
+__global__ __launch_bounds__(256) void k_adj_bc_move(F<double> lam, const pxh_bc_entry *__restrict__ ent, int n,
+                                                     pxh_bc bc, uint32_t groups, int src_sorted) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     const pxh_bc_entry e = ent[k];
     if (!((groups >> e.group) & 1u)) return;
     double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
     if (!f0 && !f1) return;
-    double a = take[2 * long(k)], b = take[2 * long(k) + 1];
+    if (src_sorted && k > 0 && ent[k - 1].group == e.group && ent[k - 1].src == e.src) return;  // not the run's first
+    double a, b;
+    bc_take(lam, e, bc, a, b);
     if (src_sorted) {
-        if (k > 0 && ent[k - 1].group == e.group && ent[k - 1].src == e.src) return;  // not the run's first
         for (int j = k + 1; j < n && ent[j].group == e.group && ent[j].src == e.src; ++j) {
-            a += take[2 * long(j)];
-            b += take[2 * long(j) + 1];
+            double aj, bj;
+            bc_take(lam, ent[j], bc, aj, bj);
+            a += aj;
+            b += bj;
         }
     }
     if (f0) atomicAdd(&f0[e.src], a);
@@ -304,10 +302,11 @@ __device__ __forceinline__ HydroCell<T> hc_lift(const HydroCell<double> &c) {
     return r;
 }
 
-__global__ __launch_bounds__(ADJ_BX *ADJ_BY) void k_adj_hydro(F<double> in, const uint8_t *__restrict__ m0,
-                                                              const double *__restrict__ dtp, int ny, int nx,
-                                                              double dx, F<double> lam1, F<double> lam_in,
-                                                              double *__restrict__ part_dt) {
+// Two wavefronts per SIMD (the dual-number edges want 270 registers, one
+// wavefront's worth; held to 256 the kernel spills 20 and runs 5% faster).
+__global__ __launch_bounds__(ADJ_BX *ADJ_BY) __attribute__((amdgpu_waves_per_eu(2))) void k_adj_hydro(
+    F<double> in, const uint8_t *__restrict__ m0, const double *__restrict__ dtp, int ny, int nx, double dx,
+    F<double> lam1, F<double> lam_in, double *__restrict__ part_dt) {
     const int c = blockIdx.x * ADJ_BX + threadIdx.x;
     const int r = blockIdx.y * ADJ_BY + threadIdx.y;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x;
@@ -538,14 +537,17 @@ __global__ __launch_bounds__(256) void k_adj_dt_apply(F<double> s, int nx, doubl
     lam_s.h[j] += w * v.d[4]; lam_s.hu[j] += w * v.d[5]; lam_s.hv[j] += w * v.d[6]; lam_s.G[j] += w * v.d[7];
 }
 
-// out[k] (+)= sum over n partials part[j * stride + k0 + k], k < cnt; one
-// block per k.  accumulate = 0 overwrites.
+// out[k] (+)= sum over nseg segments s and n partials of
+// part[s * seg + j * stride + k0 + k]; one block per k.  accumulate = 0
+// overwrites.
 __global__ __launch_bounds__(256) void k_adj_sum(const double *__restrict__ part, long n, int stride, int k0,
-                                                 double *__restrict__ out, int accumulate) {
+                                                 double *__restrict__ out, int accumulate, int nseg = 1,
+                                                 long seg = 0) {
     __shared__ double red[256];
     const int k = blockIdx.x;
     double v = 0.0;
-    for (long j = threadIdx.x; j < n; j += 256) v += part[j * stride + k0 + k];
+    for (int q = 0; q < nseg; ++q)
+        for (long j = threadIdx.x; j < n; j += 256) v += part[q * seg + j * stride + k0 + k];
     red[threadIdx.x] = v;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

@@ -418,8 +418,9 @@ def test_ensemble_plan_sizes_and_rejections():
 def test_adjoint_programs_and_workspace():
     """The adjoint's ring-fill program holds the forward entries sorted by
     (group, src), flagged PXH_BC_SRC_SORTED; pxh_adj_workspace_size (host-only)
-    adds two doubles per entry of the program it is given and refuses float32
-    grids (the adjoint is double only)."""
+    depends on the grid alone (the transposed fill moves each run of equal
+    sources in one pass, with no scratch per entry) and refuses float32 grids
+    (the adjoint is double only)."""
     for mk in (lambda: cases.calibration_channel(nx=40, ny=20), lambda: cases.periodic_box(nx=24, ny=20)):
         mesh, arr, params = mk()
         bm = BoundaryManager(params, mesh.local_X, mesh.local_Y, device="cpu")
@@ -435,7 +436,7 @@ def test_adjoint_programs_and_workspace():
         g64 = _hip.Grid(20, 40, _hip.PXH_F64, 0, 0.02)
         base = lib.pxh_adj_workspace_size(g64, None)
         assert base >= 11 * 8 * 20 * 40
-        assert lib.pxh_adj_workspace_size(g64, bc) == base + 16 * bm.n_entries
+        assert lib.pxh_adj_workspace_size(g64, bc) == base
         assert lib.pxh_adj_workspace_size(_hip.Grid(20, 40, _hip.PXH_F32, 0, 0.02), None) == -1
 
 
