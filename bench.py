@@ -94,6 +94,8 @@ def parse():
     p.add_argument("--adj-steps", type=int, default=200,
                    help="steps of the gradient run (bed calibration, 1024x256 float64; 0: skip)")
     p.add_argument("--adj-every", type=int, default=20, help="checkpoint interval of the gradient run")
+    p.add_argument("--adj-rows", type=int, default=0,
+                   help="float64 strip height of the gradient run (0: tuned with --autotune 1, else 32)")
     p.add_argument("--accuracy", type=int, default=1,
                    help="also run the timed steps in float64 and report the float32 result's deviation (1) or not (0)")
     p.add_argument("--tile-rows", type=int, default=0,
@@ -459,8 +461,8 @@ def adjoint_run(a, torch):
     calibration channel in float64, checkpoint every a.adj_every steps.  Times
     the forward pass (fused steps + checkpoints), the backward pass (segment
     replays + adjoint steps + init adjoint), and the same number of plain
-    float64 forward steps; with --autotune the float64 strip height is tuned
-    on this grid first (it sets the forward sweeps of both)."""
+    float64 forward steps; the float64 strip height (it sets the forward
+    sweeps of both) is a.adj_rows, or tuned on this grid with --autotune."""
     from pyexner_amd import cases
     from pyexner_amd.adjoint import ExnerGradientRun, seeded_target
     from pyexner_amd.stepper import ExnerStepper
@@ -486,8 +488,10 @@ def adjoint_run(a, torch):
         return t1 - t0, t2 - t1, float(loss.detach()), zb.grad, float(d50.grad), float(por.grad)
 
     once()  # warm-up
-    rows64 = 32
-    if a.autotune:
+    rows64 = a.adj_rows or 32
+    if a.adj_rows:
+        run.be.set_strip_rows(a.adj_rows)
+    elif a.autotune:
         # 32-row strips give this 1024 x 256 grid 144 wavefronts for 1024 SIMDs
         from pyexner_amd.autotune import tune_strip_rows
         rows64, _ = tune_strip_rows(run.be, run.s0, run.mask, run.cfl, steps=5, use_cache=False)
