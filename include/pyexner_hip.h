@@ -480,20 +480,48 @@ int pxh_halo_unpack(const pxh_grid *grid, const pxh_fields *s, int hw, uint32_t 
  * all n of the sweep launched.  lo = 1 when the fields carry z_b_lo (float32
  * compensated bed).  Every domain must be valid for the single-domain entry
  * with grid (ny, nx, dtype, flags, dx): the table lives on the device, so
- * only the scalars are checked here.  No dt is reduced (flags, sed, dt_dev
- * and PXH_SWEEP_SKIP_G in sweep_flags as in the single-domain entries). */
+ * only the scalars are checked here (flags, sed, dt_dev and
+ * PXH_SWEEP_SKIP_G in sweep_flags as in the single-domain entries).
+ * The next dt, reduced inside the second stage as the single-domain fused
+ * step does: a second-stage sweep given dt_partial and xs writes one CFL
+ * minimum per block into dt_partial[0, nblocks) (over the edges between the
+ * cells off each domain's two-cell frame) and each domain's seam records at
+ * xs + xs0 (elements of the dtype; xs0: the running sum of
+ * pxh_multi_seam_elems over the domains before it); then, after the frame
+ * sweep, pxh_dt_frame_multi adds each domain's frame edges and strip-seam
+ * edges and reduces everything into *dt_out = max(cfl * min, 1e-6).
+ * `shared` (bit PXH_HALO_N/S/W/E): the domain's sides that are ghost rings
+ * of a neighbouring tile; the frame cells there hold no valid combined
+ * state, and edges touching them are left to the neighbour, whose owned
+ * cells they are.  tblk0: the domain's first block of pxh_dt_frame_multi,
+ * the running sum of pxh_multi_dt_frame_blocks.  With three ghost rings per
+ * shared side (the fused stage's halo) every edge of the decomposed grid is
+ * an edge between valid cells of some domain, so the result is the dt of
+ * the whole grid. */
 typedef struct {
     pxh_fields in, out, old;
     const uint8_t *mask0;
     int32_t ny, nx;
     int32_t sblk0, fblk0;
     double dx;
+    int64_t xs0;
+    int32_t tblk0;
+    uint32_t shared;
 } pxh_sweep_domain;
 int64_t pxh_multi_stage_blocks(const pxh_grid *grid);
 int64_t pxh_multi_frame_blocks(int32_t ny, int32_t nx);
+int64_t pxh_multi_seam_elems(int32_t ny, int32_t nx);
+int64_t pxh_multi_dt_frame_blocks(const pxh_grid *grid);
 int pxh_roe_exner_stage_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,
                                     int32_t nblocks, int32_t second, int32_t lo, const pxh_sediments *sed,
-                                    const void *dt_dev, int sweep_flags, void *stream);
+                                    const void *dt_dev, int sweep_flags, void *dt_partial, void *xs,
+                                    void *stream);
+/* nblocks: the sum of pxh_multi_dt_frame_blocks over the n domains; nstage:
+ * the second-stage sweep's block count, whose minima lie in
+ * dt_partial[0, nstage); this call writes its own partials after them
+ * (dt_partial holds nstage + nblocks elements of the dtype) and the dt. */
+int pxh_dt_frame_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n, int32_t nblocks,
+                       int32_t nstage, double cfl, void *dt_partial, const void *xs, void *dt_out, void *stream);
 int pxh_roe_exner_frame_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,
                                     int32_t nblocks, int32_t second, const pxh_sediments *sed, const void *dt_dev,
                                     int sweep_flags, void *stream);

@@ -81,7 +81,8 @@ class BCHandler(ctypes.Structure):
 class SweepDomain(ctypes.Structure):  # pxh_sweep_domain ("in" is a Python keyword)
     _fields_ = [("in_", Fields), ("out", Fields), ("old", Fields), ("mask0", ctypes.c_void_p),
                 ("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("sblk0", ctypes.c_int32), ("fblk0", ctypes.c_int32),
-                ("dx", ctypes.c_double)]
+                ("dx", ctypes.c_double), ("xs0", ctypes.c_int64), ("tblk0", ctypes.c_int32),
+                ("shared", ctypes.c_uint32)]
 
 
 class DtDomain(ctypes.Structure):    # pxh_dt_domain
@@ -155,9 +156,15 @@ _SIGS = {
                                        ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_multi_stage_blocks": (ctypes.c_int64, [_P(Grid)]),
     "pxh_multi_frame_blocks": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "pxh_multi_seam_elems": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "pxh_multi_dt_frame_blocks": (ctypes.c_int64, [_P(Grid)]),
     "pxh_roe_exner_stage_sweep_multi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
                                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(Sediments),
-                                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+                                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "pxh_dt_frame_multi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pxh_roe_exner_frame_sweep_multi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
                                                        ctypes.c_int32, ctypes.c_int32, _P(Sediments), ctypes.c_void_p,
                                                        ctypes.c_int, ctypes.c_void_p]),

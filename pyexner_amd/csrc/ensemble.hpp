@@ -149,10 +149,14 @@ __device__ __forceinline__ int find_domain(const pxh_sweep_domain *__restrict__ 
 }
 
 // The fused stage sweep over several domains (pxh_roe_exner_stage_sweep_multi);
-// ST as for k_ens_stage, holding for every domain.
-template <typename T, bool FAST, int MODE, bool LO, bool MPM, int ST>
+// ST as for k_ens_stage, holding for every domain.  DT (second stage): each
+// block's CFL minimum over its domain's interior edges into partial[block],
+// each domain's seam records at xs + xs0.
+template <typename T, bool FAST, int MODE, bool LO, bool MPM, int ST, bool DT>
 __global__ __launch_bounds__(SW *FWPB) void k_multi_stage(const pxh_sweep_domain *__restrict__ doms, int n, int sr,
-                                                          const T *__restrict__ dtp, SedK sd, int write_G) {
+                                                          const T *__restrict__ dtp, SedK sd, int write_G,
+                                                          T *__restrict__ partial, T *__restrict__ xs) {
+    __shared__ T red[FWPB];
     int lb, unused;
     xcd_block(lb, unused);  // one-dimensional grid
     const pxh_sweep_domain &d = doms[find_domain<&pxh_sweep_domain::sblk0>(doms, n, lb)];
@@ -166,10 +170,29 @@ __global__ __launch_bounds__(SW *FWPB) void k_multi_stage(const pxh_sweep_domain
         stage_strip_run<T, FAST, 0, false, LO, MPM, ST>(p, dev_fields<T>(d.in), dev_fields<T>(d.out), none, none,
                                                         d.mask0, dtp, ny, nx, static_cast<T>(d.dx), sd, write_G,
                                                         nullptr);
-    else
-        stage_strip_run<T, FAST, 1, false, LO, MPM, ST>(p, dev_fields<T>(d.in), none, dev_fields<T>(d.out),
-                                                        dev_fields<T>(d.old), d.mask0, dtp, ny, nx,
-                                                        static_cast<T>(d.dx), sd, write_G, nullptr);
+    else {
+        T vmin = stage_strip_run<T, FAST, 1, DT, LO, MPM, ST>(p, dev_fields<T>(d.in), none, dev_fields<T>(d.out),
+                                                              dev_fields<T>(d.old), d.mask0, dtp, ny, nx,
+                                                              static_cast<T>(d.dx), sd, write_G,
+                                                              DT ? xs + d.xs0 : nullptr);
+        if constexpr (DT) {
+            vmin = wave_block_min_n<FWPB>(vmin, red);
+            if (threadIdx.x == 0) partial[lb] = vmin;
+        }
+    }
+}
+
+// The frame and strip-seam edges of every domain's combined state (old) for
+// pxh_dt_frame_multi, one partial minimum per block; frame edges touching a
+// shared side's ghost rings are skipped (band_dt_cell).
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_multi_band_dt(const pxh_sweep_domain *__restrict__ doms, int n, int sr,
+                                                       const T *__restrict__ xs, T *__restrict__ partial) {
+    const pxh_sweep_domain &d = doms[find_domain<&pxh_sweep_domain::tblk0>(doms, n, blockIdx.x)];
+    const long t = long(blockIdx.x - d.tblk0) * 256 + threadIdx.x;
+    const T v = band_dt_cell<T, FAST>(t, frame_cells(d.ny, d.nx), band_dt_work(d.ny, d.nx, sr), dev_fields<T>(d.old),
+                                      d.mask0, xs + d.xs0, d.ny, d.nx, static_cast<T>(d.dx), sr, d.shared);
+    block256_min_to(v, partial + blockIdx.x);
 }
 
 // The frame sweep over several domains (pxh_roe_exner_frame_sweep_multi):

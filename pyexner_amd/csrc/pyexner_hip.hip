@@ -1129,34 +1129,36 @@ using namespace pxh;
 // The multi-domain fused sweeps (pxh_roe_exner_stage_sweep_multi / _frame_sweep_multi).
 // flags: PXH_NO_NODATA / PXH_Z_UNIFORM, promises for every domain, picked as
 // launch_stage picks them for one grid.
-template <typename T, bool FAST, int MODE, bool LO, bool MPM>
+template <typename T, bool FAST, int MODE, bool LO, bool MPM, bool DT>
 static void multi_stage_k(const pxh_sweep_domain *doms, int n, int nblocks, int sr, int flags, const T *dtp,
-                          const SedK &sd, int write_G, hipStream_t st) {
+                          const SedK &sd, int write_G, T *partial, T *xs, hipStream_t st) {
     const bool nn = (flags & PXH_NO_NODATA) != 0, zu = nn && (flags & PXH_Z_UNIFORM) != 0;
     const dim3 grid(nblocks), block(SW * FWPB);
     if constexpr (FAST || sizeof(T) == 8) {
         if (zu) {
-            hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, ST_NN | ST_ZU>), grid, block, 0, st, doms, n,
-                               sr, dtp, sd, write_G);
+            hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, ST_NN | ST_ZU, DT>), grid, block, 0, st, doms,
+                               n, sr, dtp, sd, write_G, partial, xs);
             return;
         }
     }
     if (nn)
-        hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, ST_NN>), grid, block, 0, st, doms, n, sr, dtp, sd,
-                           write_G);
+        hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, ST_NN, DT>), grid, block, 0, st, doms, n, sr, dtp,
+                           sd, write_G, partial, xs);
     else
-        hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, 0>), grid, block, 0, st, doms, n, sr, dtp, sd,
-                           write_G);
+        hipLaunchKernelGGL((k_multi_stage<T, FAST, MODE, LO, MPM, 0, DT>), grid, block, 0, st, doms, n, sr, dtp, sd,
+                           write_G, partial, xs);
 }
 
+// partial, xs: the second stage's dt minima and seam records (both or neither).
 template <typename T, bool FAST>
 static void multi_stage_t(const pxh_sweep_domain *doms, int n, int nblocks, int sr, bool second, bool lo, int flags,
-                          const T *dtp, const SedK &sd, int write_G, hipStream_t st) {
-#define PXH_MS(MODE_, LO_, MPM_) \
-    multi_stage_k<T, FAST, MODE_, LO_, MPM_>(doms, n, nblocks, sr, flags, dtp, sd, write_G, st)
-#define PXH_MS_MODE(LO_, MPM_)         \
-    if (second) PXH_MS(1, LO_, MPM_); \
-    else PXH_MS(0, LO_, MPM_)
+                          const T *dtp, const SedK &sd, int write_G, T *partial, T *xs, hipStream_t st) {
+#define PXH_MS(MODE_, LO_, MPM_, DT_) \
+    multi_stage_k<T, FAST, MODE_, LO_, MPM_, DT_>(doms, n, nblocks, sr, flags, dtp, sd, write_G, partial, xs, st)
+#define PXH_MS_MODE(LO_, MPM_)                   \
+    if (second && partial) PXH_MS(1, LO_, MPM_, true); \
+    else if (second) PXH_MS(1, LO_, MPM_, false); \
+    else PXH_MS(0, LO_, MPM_, false)
     if constexpr (sizeof(T) == 4) {
         if (lo) {
             if (sd.mpm) { PXH_MS_MODE(true, true); }
@@ -1168,6 +1170,19 @@ static void multi_stage_t(const pxh_sweep_domain *doms, int n, int nblocks, int 
     else { PXH_MS_MODE(false, false); }
 #undef PXH_MS_MODE
 #undef PXH_MS
+}
+
+// The frame and seam edges of every domain (k_multi_band_dt) after the second
+// stage's per-block minima in partial[0, nstage), then the final reduction.
+template <typename T, bool FAST>
+static int dt_frame_multi_t(const pxh_sweep_domain *doms, int n, int nblocks, int nstage, int sr, T cfl,
+                            T *partial, const T *xs, T *dt_out, hipStream_t st) {
+    hipLaunchKernelGGL((k_multi_band_dt<T, FAST>), dim3(nblocks), dim3(256), 0, st, doms, n, sr, xs,
+                       partial + nstage);
+    if (int e = launch_status()) return e;
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const T *>(partial),
+                       nstage + nblocks, cfl, 1, dt_out);
+    return launch_status();
 }
 
 template <typename T, bool FAST>
@@ -1390,12 +1405,23 @@ int64_t pxh_multi_frame_blocks(int32_t ny, int32_t nx) {
     return (frame_cells(ny, nx) + 255) / 256;
 }
 
+int64_t pxh_multi_seam_elems(int32_t ny, int32_t nx) {
+    if (ny < 2 || nx < 2) return PXH_E_ARG;
+    return seam_col_elems(ny, nx);
+}
+
+int64_t pxh_multi_dt_frame_blocks(const pxh_grid *g) {
+    if (!grid_ok(g)) return PXH_E_ARG;
+    if (!stage_fits(g)) return PXH_E_RANGE;
+    return (band_dt_work(g->ny, g->nx, strip_rows(g->flags)) + 255) / 256;
+}
+
 int pxh_roe_exner_stage_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,
                                     int32_t nblocks, int32_t second, int32_t lo, const pxh_sediments *sd,
-                                    const void *dt_dev, int sweep_flags, void *stream) {
+                                    const void *dt_dev, int sweep_flags, void *dt_partial, void *xs, void *stream) {
     const int rows = PXH_STRIP_ROWS(flags);
     if ((dtype != PXH_F32 && dtype != PXH_F64) || !doms || n <= 0 || nblocks <= 0 || !sd || !dt_dev ||
-        (rows != 0 && (rows < 8 || rows > 128)))
+        (rows != 0 && (rows < 8 || rows > 128)) || (!dt_partial != !xs) || (dt_partial && !second))
         return PXH_E_ARG;
     if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
     const int write_G = (sweep_flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
@@ -1403,14 +1429,37 @@ int pxh_roe_exner_stage_sweep_multi(int32_t dtype, int32_t flags, const pxh_swee
     const int sr = strip_rows(flags);
     if (dtype == PXH_F64)
         multi_stage_t<double, false>(doms, n, nblocks, sr, second, false, flags, static_cast<const double *>(dt_dev),
-                                     sed_k(*sd), write_G, st);
+                                     sed_k(*sd), write_G, static_cast<double *>(dt_partial),
+                                     static_cast<double *>(xs), st);
     else if (flags & PXH_EXACT_F32)
         multi_stage_t<float, false>(doms, n, nblocks, sr, second, lo, flags, static_cast<const float *>(dt_dev),
-                                    sed_k(*sd), write_G, st);
+                                    sed_k(*sd), write_G, static_cast<float *>(dt_partial), static_cast<float *>(xs),
+                                    st);
     else
         multi_stage_t<float, true>(doms, n, nblocks, sr, second, lo, flags, static_cast<const float *>(dt_dev),
-                                   sed_k(*sd), write_G, st);
+                                   sed_k(*sd), write_G, static_cast<float *>(dt_partial), static_cast<float *>(xs),
+                                   st);
     return launch_status();
+}
+
+int pxh_dt_frame_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n, int32_t nblocks,
+                       int32_t nstage, double cfl, void *dt_partial, const void *xs, void *dt_out, void *stream) {
+    const int rows = PXH_STRIP_ROWS(flags);
+    if ((dtype != PXH_F32 && dtype != PXH_F64) || !doms || n <= 0 || nblocks <= 0 || nstage < 0 || !dt_partial ||
+        !xs || !dt_out || (rows != 0 && (rows < 8 || rows > 128)) || int64_t(nstage) + nblocks >= (1LL << 31))
+        return PXH_E_ARG;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int sr = strip_rows(flags);
+    if (dtype == PXH_F64)
+        return dt_frame_multi_t<double, false>(doms, n, nblocks, nstage, sr, cfl, static_cast<double *>(dt_partial),
+                                               static_cast<const double *>(xs), static_cast<double *>(dt_out), st);
+    if (flags & PXH_EXACT_F32)
+        return dt_frame_multi_t<float, false>(doms, n, nblocks, nstage, sr, static_cast<float>(cfl),
+                                              static_cast<float *>(dt_partial), static_cast<const float *>(xs),
+                                              static_cast<float *>(dt_out), st);
+    return dt_frame_multi_t<float, true>(doms, n, nblocks, nstage, sr, static_cast<float>(cfl),
+                                         static_cast<float *>(dt_partial), static_cast<const float *>(xs),
+                                         static_cast<float *>(dt_out), st);
 }
 
 int pxh_roe_exner_frame_sweep_multi(int32_t dtype, int32_t flags, const pxh_sweep_domain *doms, int32_t n,

@@ -657,26 +657,34 @@ __host__ __device__ inline long fused_seam_edges(int ny, int nx, int sr) {
 template <typename T, bool FAST>
 __device__ __forceinline__ T band_dt_cell(long t, long nband, long nwork, const F<T> s,
                                           const uint8_t *__restrict__ m0, const T *__restrict__ xs, int ny, int nx,
-                                          T dx, int sr) {
+                                          T dx, int sr, unsigned shared = 0) {
     T v = inf_v<T>();
     if (t < nband) {
         int r, c;
         frame_cell(t, ny, nx, r, c);
+        // shared (PXH_HALO_* bits): sides whose outer two cells are a tile's
+        // ghost rings, which hold no valid combined state; edges touching
+        // them are left to the neighbour that owns those cells
+        auto ok = [&](int rr, int cc) {
+            return !(((shared & PXH_HALO_N) && rr < 2) || ((shared & PXH_HALO_S) && rr >= ny - 2) ||
+                     ((shared & PXH_HALO_W) && cc < 2) || ((shared & PXH_HALO_E) && cc >= nx - 2));
+        };
+        if (!ok(r, c)) return v;
         const long i = long(r) * nx + c;
         const DD<T> me = dd_at<T, FAST>(s, m0, i);
-        if (c + 1 < nx) {
+        if (c + 1 < nx && ok(r, c + 1)) {
             const DD<T> e = dd_at<T, FAST>(s, m0, i + 1);
             if (dd_active(me, e)) v = pmin(v, dt_edge_d<0, T, FAST>(me, e, dx));
         }
-        if (c > 0) {
+        if (c > 0 && ok(r, c - 1)) {
             const DD<T> w = dd_at<T, FAST>(s, m0, i - 1);
             if (dd_active(w, me)) v = pmin(v, dt_edge_d<0, T, FAST>(w, me, dx));
         }
-        if (r + 1 < ny) {
+        if (r + 1 < ny && ok(r + 1, c)) {
             const DD<T> so = dd_at<T, FAST>(s, m0, i + nx);
             if (dd_active(me, so)) v = pmin(v, dt_edge_d<1, T, FAST>(me, so, dx));
         }
-        if (r > 0) {
+        if (r > 0 && ok(r - 1, c)) {
             const DD<T> n = dd_at<T, FAST>(s, m0, i - nx);
             if (dd_active(n, me)) v = pmin(v, dt_edge_d<1, T, FAST>(n, me, dx));
         }

@@ -392,28 +392,40 @@ class TiledExner:
 
     def _sweep_table(self, src, dst, old):
         """The tiles as a device table of pxh_sweep_domain for one stage
-        (src -> dst, combined into old when given) and the stage and frame
-        block counts of the multi-domain sweeps (built once per stage kind and
+        (src -> dst, combined into old when given), the stage, frame and
+        frame-dt block counts of the multi-domain sweeps, and the second
+        stage's dt buffers: the per-block minima (stage blocks, then frame-dt
+        blocks) and every tile's seam records (built once per stage kind and
         strip height)."""
         key = (src, dst, old, int(next(iter(self.tiles.values())).grid.flags))
         if key not in self._sweep_tabs:
             lib = _hip.load()
             tiles = list(self.tiles.values())
             table = (_hip.SweepDomain * len(tiles))()
-            sb = fb = 0
+            sb = fb = tb = xe = 0
             for k, t in enumerate(tiles):
                 d = table[k]
                 d.in_, d.out = t.fields(src), t.fields(dst)
                 if old is not None:
                     d.old = t.fields(old)
                 d.mask0, d.ny, d.nx, d.sblk0, d.fblk0, d.dx = t.m0, t.grid.ny, t.grid.nx, sb, fb, t.grid.dx
+                d.xs0, d.tblk0 = xe, tb
+                d.shared = sum(side for side in _SIDES if t.ring[side])
                 ns, nf = lib.pxh_multi_stage_blocks(t.grid), lib.pxh_multi_frame_blocks(t.grid.ny, t.grid.nx)
-                if ns <= 0 or nf <= 0:
+                nt, ne = lib.pxh_multi_dt_frame_blocks(t.grid), lib.pxh_multi_seam_elems(t.grid.ny, t.grid.nx)
+                if ns <= 0 or nf <= 0 or nt <= 0 or ne <= 0:
                     raise ValueError("a tile the fused stage cannot sweep")
                 sb += ns
                 fb += nf
+                tb += nt
+                xe += ne
             raw = np.frombuffer(bytes(table), dtype=np.uint8).copy()
-            self._sweep_tabs[key] = (torch.from_numpy(raw).to(self.dt.device), len(tiles), sb, fb)
+            dev = self.dt.device
+            part = xs = None
+            if old is not None:
+                part = torch.empty(sb + tb, dtype=self.dt.dtype, device=dev)
+                xs = torch.empty(xe, dtype=self.dt.dtype, device=dev)
+            self._sweep_tabs[key] = (torch.from_numpy(raw).to(dev), len(tiles), sb, fb, tb, part, xs)
         return self._sweep_tabs[key]
 
     def _dt_table(self):
@@ -472,15 +484,21 @@ class TiledExner:
         skip = _hip.SWEEP_SKIP_G if old is None else 0
         if self.multi:
             # every tile's sweep in one launch each; the ring fills stay per tile
-            tab, n, nsb, nfb = self._sweep_table(src, dst, old)
+            tab, n, nsb, nfb, ntb, part, xs = self._sweep_table(src, dst, old)
             g = self.tiles[(0, 0)].grid
             lo = int(self.tiles[(0, 0)].a.z_b_lo is not None)
             second = int(old is not None)
+            # the second stage reduces the next dt as it goes (its minima and
+            # seam records, then the frame and seam edges after the frame sweep)
+            pp, xp = (part.data_ptr(), xs.data_ptr()) if second else (None, None)
             _hip.call("pxh_roe_exner_stage_sweep_multi", g.dtype, g.flags, tab.data_ptr(), n, nsb, second, lo, sed,
-                      dtp, skip, st)
+                      dtp, skip, pp, xp, st)
             self._fill(dst, "hy")
             _hip.call("pxh_roe_exner_frame_sweep_multi", g.dtype, g.flags, tab.data_ptr(), n, nfb, second, sed, dtp,
                       skip, st)
+            if second:
+                _hip.call("pxh_dt_frame_multi", g.dtype, g.flags, tab.data_ptr(), n, ntb, nsb, self.cfl, pp, xp, dtp,
+                          st)
             self._fill(old if old is not None else dst, "bed" if old is not None else "bed1")
             self.exchange(old if old is not None else dst, _STEP)
             return
@@ -507,8 +525,9 @@ class TiledExner:
         if not self.dt_ready:
             self.compute_dt()
         self._fused_stage("a", "b", None)
-        self._fused_stage("b", "c", "a")
-        self.compute_dt()  # the next step's dt, from the exchanged state
+        self._fused_stage("b", "c", "a")  # with multi, leaves the next step's dt
+        if not self.multi:
+            self.compute_dt()  # the next step's dt, from the exchanged state
         self.dt_ready = True
 
     def halo_bytes_per_step(self):

@@ -576,7 +576,7 @@ def test_multi_domain_sweeps_check_their_arguments():
     with the single-domain geometry and reject bad scalars before any launch
     (host-only checks: no sweep runs)."""
     lib = _hip.load()
-    assert ctypes.sizeof(_hip.SweepDomain) == 248
+    assert ctypes.sizeof(_hip.SweepDomain) == 264
     sh = _hip.STRIP_ROWS_SHIFT
     assert lib.pxh_multi_stage_blocks(_hip.Grid(2054, 2054, _hip.PXH_F32, 12 << sh, 0.05)) == 35 * 172
     assert lib.pxh_multi_stage_blocks(_hip.Grid(2054, 2054, _hip.PXH_F32, 0, 0.05)) == 35 * 65
@@ -585,12 +585,33 @@ def test_multi_domain_sweeps_check_their_arguments():
     assert lib.pxh_multi_frame_blocks(1, 5) == _hip.PXH_E_ARG
     sed = _hip.Sediments()
     p = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks first
-    assert lib.pxh_roe_exner_stage_sweep_multi(9, 0, p, 1, 8, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
-    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, None, 1, 8, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
-    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 0, 8, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
-    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 200 << sh, p, 1, 8, 0, 0, sed, p, 0, None) == \
+    assert lib.pxh_roe_exner_stage_sweep_multi(9, 0, p, 1, 8, 0, 0, sed, p, 0, None, None, None) == _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, None, 1, 8, 0, 0, sed, p, 0, None, None, None) == \
         _hip.PXH_E_ARG
-    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 1, 8, 0, 0, sed, None, 0, None) == _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 0, 8, 0, 0, sed, p, 0, None, None, None) == \
+        _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 200 << sh, p, 1, 8, 0, 0, sed, p, 0, None, None, None) == \
+        _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 1, 8, 0, 0, sed, None, 0, None, None, None) == \
+        _hip.PXH_E_ARG
+    # the dt buffers: both or neither, and only for the second stage
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 1, 8, 1, 0, sed, p, 0, p, None, None) == \
+        _hip.PXH_E_ARG
+    assert lib.pxh_roe_exner_stage_sweep_multi(_hip.PXH_F32, 0, p, 1, 8, 0, 0, sed, p, 0, p, p, None) == \
+        _hip.PXH_E_ARG
+    # the tiles' frame dt: the frame's cells and the strip seams' edges (more
+    # seams with shorter strips); seam records of 10 values per strip column
+    # and row
+    g24 = _hip.Grid(2054, 2054, _hip.PXH_F32, 24 << sh, 0.05)
+    g12 = _hip.Grid(2054, 2054, _hip.PXH_F32, 12 << sh, 0.05)
+    assert lib.pxh_multi_dt_frame_blocks(g12) > lib.pxh_multi_dt_frame_blocks(g24) > \
+        lib.pxh_multi_frame_blocks(2054, 2054)
+    assert lib.pxh_multi_seam_elems(2054, 2054) == -(-2054 // 60) * 2054 * 10
+    assert lib.pxh_multi_seam_elems(1, 9) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_frame_multi(_hip.PXH_F32, 0, p, 1, 8, 4, 0.5, None, p, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_frame_multi(_hip.PXH_F32, 0, p, 1, 8, 4, 0.5, p, None, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_frame_multi(_hip.PXH_F32, 0, p, 1, 0, 4, 0.5, p, p, p, None) == _hip.PXH_E_ARG
+    assert lib.pxh_dt_frame_multi(_hip.PXH_F32, 0, None, 1, 8, 4, 0.5, p, p, p, None) == _hip.PXH_E_ARG
     assert lib.pxh_roe_exner_frame_sweep_multi(_hip.PXH_F64, 0, p, 1, 0, 0, sed, p, 0, None) == _hip.PXH_E_ARG
     assert lib.pxh_roe_exner_frame_sweep_multi(_hip.PXH_F64, 0, None, 1, 8, 0, sed, p, 0, None) == _hip.PXH_E_ARG
     sed.mpm, sed.n_sed = 1, 0  # MPM without a sediment class
