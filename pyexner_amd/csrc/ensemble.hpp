@@ -183,15 +183,19 @@ __global__ __launch_bounds__(SW *FWPB) void k_multi_stage(const pxh_sweep_domain
 }
 
 // The frame and strip-seam edges of every domain's combined state (old) for
-// pxh_dt_frame_multi, one partial minimum per block; frame edges touching a
-// shared side's ghost rings are skipped (band_dt_cell).
+// pxh_dt_frame_multi, one partial minimum per block, folded with a share of
+// the nprev per-block minima `prev` of the second-stage sweep (grid-strided),
+// so the final reduction reads this kernel's partials only; frame edges
+// touching a shared side's ghost rings are skipped (band_dt_cell).
 template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_multi_band_dt(const pxh_sweep_domain *__restrict__ doms, int n, int sr,
-                                                       const T *__restrict__ xs, T *__restrict__ partial) {
+                                                       const T *__restrict__ xs, const T *__restrict__ prev,
+                                                       int nprev, T *__restrict__ partial) {
     const pxh_sweep_domain &d = doms[find_domain<&pxh_sweep_domain::tblk0>(doms, n, blockIdx.x)];
     const long t = long(blockIdx.x - d.tblk0) * 256 + threadIdx.x;
-    const T v = band_dt_cell<T, FAST>(t, frame_cells(d.ny, d.nx), band_dt_work(d.ny, d.nx, sr), dev_fields<T>(d.old),
-                                      d.mask0, xs + d.xs0, d.ny, d.nx, static_cast<T>(d.dx), sr, d.shared);
+    T v = band_dt_cell<T, FAST>(t, frame_cells(d.ny, d.nx), band_dt_work(d.ny, d.nx, sr), dev_fields<T>(d.old),
+                                d.mask0, xs + d.xs0, d.ny, d.nx, static_cast<T>(d.dx), sr, d.shared);
+    for (long k = long(blockIdx.x) * 256 + threadIdx.x; k < nprev; k += long(gridDim.x) * 256) v = pmin(v, prev[k]);
     block256_min_to(v, partial + blockIdx.x);
 }
 

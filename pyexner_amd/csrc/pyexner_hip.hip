@@ -1172,16 +1172,17 @@ static void multi_stage_t(const pxh_sweep_domain *doms, int n, int nblocks, int 
 #undef PXH_MS
 }
 
-// The frame and seam edges of every domain (k_multi_band_dt) after the second
-// stage's per-block minima in partial[0, nstage), then the final reduction.
+// The frame and seam edges of every domain (k_multi_band_dt, which folds in
+// the second stage's per-block minima in partial[0, nstage)), then the final
+// reduction of its own partials.
 template <typename T, bool FAST>
 static int dt_frame_multi_t(const pxh_sweep_domain *doms, int n, int nblocks, int nstage, int sr, T cfl,
                             T *partial, const T *xs, T *dt_out, hipStream_t st) {
     hipLaunchKernelGGL((k_multi_band_dt<T, FAST>), dim3(nblocks), dim3(256), 0, st, doms, n, sr, xs,
-                       partial + nstage);
+                       static_cast<const T *>(partial), nstage, partial + nstage);
     if (int e = launch_status()) return e;
-    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const T *>(partial),
-                       nstage + nblocks, cfl, 1, dt_out);
+    hipLaunchKernelGGL(k_dt_final<T>, dim3(1), dim3(FIN_THREADS), 0, st, static_cast<const T *>(partial + nstage),
+                       nblocks, cfl, 1, dt_out);
     return launch_status();
 }
 
