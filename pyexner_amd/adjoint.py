@@ -122,33 +122,51 @@ class ExnerGradientRun:
         return a
 
     # -- backward --------------------------------------------------------------
-    def _replay(self, seg):
+    def _pool(self):
+        """Buffers of one replayed segment: a for each of its steps and the
+        state after the last, b and c for each step."""
+        k = self.every
+        return ([self._buffers(self.s0) for _ in range(k + 1)], [self._buffers(self.s0) for _ in range(k)],
+                [self._buffers(self.s0) for _ in range(k)])
+
+    def _replay(self, seg, pool=None):
         """States of the steps of segment seg: per step (a, b, c, dt) with b
         and c the full outputs of the two stages (the fused step keeps only
         the frame of c), recomputed stage by stage from the segment's
         checkpoint with the forward's dts: b = S(a), c = S(b),
-        a' = 0.5 a + 0.5 c."""
+        a' = 0.5 a + 0.5 c.  pool (from _pool): the buffers to write, else
+        new ones.  Runs on the current stream."""
         n0 = seg * self.every
         n1 = min(self.n_steps, n0 + self.every)
-        a = self.s0.replace(**{k: v.clone() for k, v in self.checkpoints[seg].items()})
+        if pool is None:
+            m = n1 - n0
+            pool = ([self._buffers(self.s0) for _ in range(m + 1)], [self._buffers(self.s0) for _ in range(m)],
+                    [self._buffers(self.s0) for _ in range(m)])
+        aa, bb, cc = pool
+        for k, v in self.checkpoints[seg].items():
+            getattr(aa[0], k).copy_(v)
         out = []
         st = _stream()
-        for n in range(n0, n1):
+        for j, n in enumerate(range(n0, n1)):
             dt = self.dts[n]
-            b, c = self._buffers(a), self._buffers(a)
+            a, b, c, nxt = aa[j], bb[j], cc[j], aa[j + 1]
             fa, fb, fc = fields_struct(a), fields_struct(b), fields_struct(c)
             self.be.step_into(fa, fb, self.m0, dt.data_ptr(), st, no_nodata=self.no_nodata, z_uniform=self.z_uniform)
             self.be.step_into(fb, fc, self.m0, dt.data_ptr(), st, no_nodata=self.no_nodata, z_uniform=self.z_uniform)
-            nxt = self._buffers(a)
             self.be.combine_into(fa, fc, fields_struct(nxt), st)
             out.append((a, b, c, dt))
-            a = nxt
         return out
 
     def backward(self, lam):
         """lam: dict of adjoints of the final (h, hu, hv, z_b, G) (missing =
         zero).  Returns (dL/dz_b0, dL/dd50, dL/dporosity) with the adjoint of
-        the initialised state in self.lam_init."""
+        the initialised state in self.lam_init.
+
+        The replay of segment s - 1 needs only its checkpoint, so it runs on
+        a second stream while the adjoint steps of segment s run on the
+        current one (two buffer pools, alternately; events order each replay
+        after the adjoint that last read its pool, and each adjoint after its
+        replay): on a grid too small to fill the GPU the two overlap."""
         shape = self.s0.h.shape
         z = lambda: torch.zeros(shape, dtype=torch.float64, device=self.device)  # noqa: E731
         L = self.s0.replace(**{k: (lam[k].detach().clone().to(torch.float64).contiguous() if lam.get(k) is not None
@@ -156,13 +174,37 @@ class ExnerGradientRun:
         fl = fields_struct(L)
         lam_par = torch.zeros(8, dtype=torch.float64, device=self.device)
         ws = self.adj_ws.data_ptr()
-        for seg in reversed(range(len(self.checkpoints))):
-            steps = self._replay(seg)
+        nseg = len(self.checkpoints)
+        main = torch.cuda.current_stream()
+        side = torch.cuda.Stream()
+        pools = [self._pool() for _ in range(min(nseg, 2))]
+        done = [None, None]  # per pool: the adjoint's last read of it
+        pending = {}
+
+        def replay_on_side(seg):
+            k = seg % 2
+            if done[k] is not None:
+                side.wait_event(done[k])
+            else:
+                side.wait_stream(main)
+            with torch.cuda.stream(side):
+                steps = self._replay(seg, pools[k])
+            ev = torch.cuda.Event()
+            ev.record(side)
+            pending[seg] = (steps, ev)
+
+        replay_on_side(nseg - 1)
+        for seg in reversed(range(nseg)):
+            if seg > 0:
+                replay_on_side(seg - 1)
+            steps, ev = pending.pop(seg)
+            main.wait_event(ev)
             for a_n, b_n, c_n, dt_n in reversed(steps):
                 _hip.call("pxh_adj_roe_exner_ssprk2", self.be.grid, fields_struct(a_n), fields_struct(b_n),
                           fields_struct(c_n), self.m0, self.be.sed, self.bc_step, self.cfl, dt_n.data_ptr(), fl,
                           lam_par.data_ptr(), ws, _stream())
-            del steps
+            done[seg % 2] = torch.cuda.Event()
+            done[seg % 2].record(main)
         _hip.call("pxh_adj_roe_exner_init", self.be.grid, fields_struct(self.s0), self.m0, self.be.sed, self.bc_all,
                   fl, lam_par.data_ptr(), ws, _stream())
         self.lam_init = L
