@@ -104,10 +104,11 @@ __device__ __forceinline__ double *adj_field(const F<double> &lam, int group, in
 
 // Entry e's target adjoint, transformed for its source, into (a, b); the
 // target zeroed.
-__device__ __forceinline__ void bc_take(const F<double> &lam, const pxh_bc_entry &e, const pxh_bc &bc, double &a,
+__device__ __forceinline__ void bc_take(const F<double> &lam, const BcEnt &be, const pxh_bc &bc, double &a,
                                         double &b) {
     a = 0.0;
     b = 0.0;
+    const pxh_bc_entry e = be.first();
     double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
     const int t = e.target;
     if (e.nchain & PXH_BC_VALUE) {
@@ -134,30 +135,37 @@ __device__ __forceinline__ void bc_take(const F<double> &lam, const pxh_bc_entry
         f1[t] = 0.0;
     }
     if (e.group == PXH_G_HUV)
-        for (int q = e.nchain - 1; q >= 0; --q) bc_transform<double>(bc.handlers[e.chain[q]], a, b);
+        for (int q = e.nchain - 1; q >= 0; --q) bc_transform<double>(bc.handlers[be.chain(q)], a, b);
 }
 
 __global__ __launch_bounds__(256) void k_adj_bc_move(F<double> lam, const pxh_bc_entry *__restrict__ ent, int n,
                                                      pxh_bc bc, uint32_t groups, int src_sorted) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
-    const pxh_bc_entry e = ent[k];
-    if (!((groups >> e.group) & 1u)) return;
-    double *f0 = adj_field(lam, e.group, 0), *f1 = adj_field(lam, e.group, 1);
+    const BcEnt be = bc_ent(ent, k);
+    const unsigned grp = be.group();
+    const int src = be.src();
+    if (!((groups >> grp) & 1u)) return;
+    double *f0 = adj_field(lam, int(grp), 0), *f1 = adj_field(lam, int(grp), 1);
     if (!f0 && !f1) return;
-    if (src_sorted && k > 0 && ent[k - 1].group == e.group && ent[k - 1].src == e.src) return;  // not the run's first
+    if (src_sorted && k > 0) {
+        const BcEnt prev = bc_ent(ent, k - 1);
+        if (prev.group() == grp && prev.src() == src) return;  // not the run's first
+    }
     double a, b;
-    bc_take(lam, e, bc, a, b);
+    bc_take(lam, be, bc, a, b);
     if (src_sorted) {
-        for (int j = k + 1; j < n && ent[j].group == e.group && ent[j].src == e.src; ++j) {
+        for (int j = k + 1; j < n; ++j) {
+            const BcEnt nj = bc_ent(ent, j);
+            if (nj.group() != grp || nj.src() != src) break;
             double aj, bj;
-            bc_take(lam, ent[j], bc, aj, bj);
+            bc_take(lam, nj, bc, aj, bj);
             a += aj;
             b += bj;
         }
     }
-    if (f0) atomicAdd(&f0[e.src], a);
-    if (f1) atomicAdd(&f1[e.src], b);
+    if (f0) atomicAdd(&f0[src], a);
+    if (f1) atomicAdd(&f1[src], b);
 }
 
 // The hydro update before the ring fill at the h targets of the program (the

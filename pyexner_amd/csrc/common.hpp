@@ -126,6 +126,35 @@ __device__ __forceinline__ void str(T *p, unsigned row_off, unsigned col, T v) {
     *(PXH_GLOBAL T *)((PXH_GLOBAL char *)base + (ro + col * unsigned(sizeof(T)))) = v;
 }
 
+// A ring-fill entry (pxh_bc_entry, 16 bytes) read as four words: its fields,
+// and the bytes of its handler chain by shifts (an entry struct indexed by a
+// loop variable would be copied to scratch memory for every lane).
+static_assert(sizeof(pxh_bc_entry) == 16, "a ring-fill entry is read as four words");
+struct BcEnt {
+    uint4 w;
+    __device__ int target() const { return int(w.x); }
+    __device__ int src() const { return int(w.y); }
+    __device__ unsigned group() const { return w.z & 0xffu; }
+    __device__ unsigned nchain() const { return (w.z >> 8) & 0xffu; }
+    __device__ unsigned chain(int q) const {  // byte 10 + q of the entry
+        const unsigned b = 2u + unsigned(q);
+        return ((b < 4u ? w.z : w.w) >> (8u * (b & 3u))) & 0xffu;
+    }
+    // the entry as the struct, for code that reads its first chain byte only
+    __device__ pxh_bc_entry first() const {
+        pxh_bc_entry e{};
+        e.target = target();
+        e.src = src();
+        e.group = uint8_t(group());
+        e.nchain = uint8_t(nchain());
+        e.chain[0] = uint8_t(chain(0));
+        return e;
+    }
+};
+__device__ __forceinline__ BcEnt bc_ent(const pxh_bc_entry *ent, long k) {
+    return BcEnt{reinterpret_cast<const uint4 *>(ent)[k]};
+}
+
 // f with every field pointer moved by off elements (null stays null).
 template <typename T>
 __host__ __device__ __forceinline__ F<T> shifted(const F<T> &f, long off) {

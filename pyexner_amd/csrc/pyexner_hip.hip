@@ -356,8 +356,6 @@ struct BcSel {
     int off[PXH_N_GROUPS + 1];
 };
 
-static_assert(sizeof(pxh_bc_entry) == 16, "k_bc reads an entry as four words");
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restrict__ ent, BcSel sel, pxh_bc bc) {
     const int k = blockIdx.x * 256 + threadIdx.x;
@@ -368,25 +366,13 @@ __global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restri
 #pragma unroll
     for (int q = 1; q < PXH_N_GROUPS; ++q)
         if (q < sel.n && k >= sel.off[q]) first = sel.start[q] - sel.off[q];
-    // the 16-byte entry as four words: the handler chain's bytes are read from
-    // them by shifts (an entry struct indexed by a loop variable would be
-    // copied to scratch memory)
-    const uint4 w = reinterpret_cast<const uint4 *>(ent)[first + k];
-    pxh_bc_entry e;
-    e.target = int(w.x);
-    e.src = int(w.y);
-    e.group = uint8_t(w.z);
-    e.nchain = uint8_t(w.z >> 8);
-    e.chain[0] = uint8_t(w.z >> 16);
+    const BcEnt be = bc_ent(ent, first + k);
+    const pxh_bc_entry e = be.first();
     if (!((bc.group_mask >> e.group) & 1u)) return;
     if (e.nchain & PXH_BC_VALUE) {
         bc_value_apply<T>(s, e, bc);
         return;
     }
-    auto chain = [&](int q) -> unsigned {  // chain[q]: byte 10 + q of the entry
-        const unsigned b = 2u + unsigned(q);
-        return ((b < 4u ? w.z : w.w) >> (8u * (b & 3u))) & 0xffu;
-    };
     const int t = e.target, src = e.src;
     switch (e.group) {
         case PXH_G_H: s.h[t] = s.h[src]; break;
@@ -400,7 +386,7 @@ __global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restri
         case PXH_G_G: s.G[t] = s.G[src]; break;
         case PXH_G_HUV: {
             T hu = s.hu[src], hv = s.hv[src];
-            for (int q = 0; q < e.nchain; ++q) bc_transform<T>(bc.handlers[chain(q)], hu, hv);
+            for (int q = 0; q < e.nchain; ++q) bc_transform<T>(bc.handlers[be.chain(q)], hu, hv);
             s.hu[t] = hu;
             s.hv[t] = hv;
         } break;
