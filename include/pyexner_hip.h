@@ -59,6 +59,14 @@ extern "C" {
  * For an ensemble or a multi-domain sweep: each member's / domain's z holds
  * one value, its own first cell's. */
 #define PXH_Z_UNIFORM 4
+/* flags bit, honoured by pxh_roe_exner_ssprk2 (and _noalloc) on the float32
+ * hardware-math path with a frame-local ring fill: b->h holds 4 * ny * nx
+ * elements, and the first stage's output keeps h, hu, hv, z_b there as one
+ * record of four per cell (b->hu, hv, z_b are not read or written: b is
+ * scratch), which the second stage reads as one stream instead of four.
+ * Results are those of the same call without it; other entries and paths
+ * ignore it (b->h's first ny * nx elements are then b's h). */
+#define PXH_PACKED_STAGE 8
 /* flags bits 8-15: owned rows per strip of the fused stage sweep (0: 32;
  * otherwise 8..128), the launch shape pyexner_amd.autotune chooses per grid. */
 #define PXH_STRIP_ROWS_SHIFT 8
@@ -76,7 +84,8 @@ extern "C" {
 typedef struct {
     int32_t ny, nx;   /* array shape, ghost ring included */
     int32_t dtype;    /* PXH_F32 or PXH_F64 */
-    int32_t flags;    /* PXH_EXACT_F32, PXH_NO_NODATA, PXH_Z_UNIFORM, PXH_STRIP_ROWS, PXH_STAGE*_WAVES */
+    int32_t flags;    /* PXH_EXACT_F32, PXH_NO_NODATA, PXH_Z_UNIFORM, PXH_PACKED_STAGE, PXH_STRIP_ROWS,
+                         PXH_STAGE*_WAVES */
     double dx;        /* cell size (dx == dy) */
 } pxh_grid;
 

@@ -43,6 +43,7 @@ GROUP_FIELDS = {G_H: ("h",), G_HUV: ("hu", "hv"), G_Z: ("z",), G_ZB: ("z_b",), G
 EXACT_F32 = 1
 NO_NODATA = 2  # pxh_grid.flags: mask0 is zero in every cell
 Z_UNIFORM = 4  # pxh_grid.flags: z holds one value (z[0]) in every cell
+PACKED_STAGE = 8  # pxh_grid.flags: b.h holds 4 ny nx elements, the first stage's output as h, hu, hv, z_b records
 
 
 class Grid(ctypes.Structure):

@@ -177,34 +177,39 @@ __device__ __forceinline__ T outflux_depth(T h_src, T hu_src, T hv_src, T q, T n
 // A value entry of the ring fill (pxh_bc_value); the compensated-bed low part
 // of a set bed is zero, of an offset bed the source's.
 template <typename T>
-__device__ __forceinline__ void bc_value_apply(const F<T> &s, const pxh_bc_entry &e, const pxh_bc &bc) {
+// sq: the element stride of s.h, hu, hv, z_b (4 for the packed records of
+// PXH_PACKED_STAGE, else 1); the other fields are plain arrays.
+__device__ __forceinline__ void bc_value_apply(const F<T> &s, const pxh_bc_entry &e, const pxh_bc &bc, int sq = 1) {
     if (e.chain[0] >= bc.n_values) return;  // no such value: the entry is ignored
     const pxh_bc_value v = bc.values[e.chain[0]];
     const int t = e.target, src = e.src;
     T *f = group_field(s, e.group);
     if (!f) return;
+    const bool q4 = e.group == PXH_G_H || e.group == PXH_G_HUV || e.group == PXH_G_ZB;
+    const long ft = q4 ? long(t) * sq : t, fs = q4 ? long(src) * sq : src;  // f's elements of t and src
+    const long tq = long(t) * sq, sqi = long(src) * sq;                      // h, hu, hv's
     switch (v.kind) {
         case PXH_BV_CONST:
             if (e.group == PXH_G_HUV) {
-                if (v.a == v.a) s.hu[t] = static_cast<T>(v.a);
-                if (v.b == v.b) s.hv[t] = static_cast<T>(v.b);
+                if (v.a == v.a) s.hu[tq] = static_cast<T>(v.a);
+                if (v.b == v.b) s.hv[tq] = static_cast<T>(v.b);
             } else {
-                f[t] = static_cast<T>(v.a);
+                f[ft] = static_cast<T>(v.a);
                 if (e.group == PXH_G_ZB && s.z_b_lo) s.z_b_lo[t] = T(0);
             }
             break;
         case PXH_BV_AFFINE_T:
-            f[t] = static_cast<T>(v.b) * static_cast<T>(bc.time) + static_cast<T>(v.a);
+            f[ft] = static_cast<T>(v.b) * static_cast<T>(bc.time) + static_cast<T>(v.a);
             if (e.group == PXH_G_ZB && s.z_b_lo) s.z_b_lo[t] = T(0);
             break;
         case PXH_BV_OFFSET:
-            f[t] = f[src] + static_cast<T>(v.a);
+            f[ft] = f[fs] + static_cast<T>(v.a);
             if (e.group == PXH_G_ZB && s.z_b_lo) s.z_b_lo[t] = s.z_b_lo[src];
             break;
         case PXH_BV_OUTFLUX:
             if (e.group == PXH_G_H)
-                s.h[t] = outflux_depth<T>(s.h[src], s.hu[src], s.hv[src], static_cast<T>(v.a), static_cast<T>(v.b),
-                                          static_cast<T>(v.c));
+                s.h[tq] = outflux_depth<T>(s.h[sqi], s.hu[sqi], s.hv[sqi], static_cast<T>(v.a),
+                                           static_cast<T>(v.b), static_cast<T>(v.c));
             break;
         default: break;
     }

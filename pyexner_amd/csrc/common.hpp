@@ -155,6 +155,21 @@ __device__ __forceinline__ BcEnt bc_ent(const pxh_bc_entry *ent, long k) {
     return BcEnt{reinterpret_cast<const uint4 *>(ent)[k]};
 }
 
+// Four consecutive floats per cell (the packed records of PXH_PACKED_STAGE)
+// with the row addressing of ldr / str above: element (row_off + col) of an
+// array of four-float records, 32-bit byte offsets from a moved base.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4v ldr4(const float *p, unsigned row_off, unsigned col) {
+    const unsigned long base = uni64(reinterpret_cast<unsigned long>(p));
+    const unsigned ro = __builtin_amdgcn_readfirstlane(row_off * 16u);
+    return *(const PXH_GLOBAL f4v *)((const PXH_GLOBAL char *)base + (ro + col * 16u));
+}
+__device__ __forceinline__ void str4(float *p, unsigned row_off, unsigned col, f4v v) {
+    const unsigned long base = uni64(reinterpret_cast<unsigned long>(p));
+    const unsigned ro = __builtin_amdgcn_readfirstlane(row_off * 16u);
+    *(PXH_GLOBAL f4v *)((PXH_GLOBAL char *)base + (ro + col * 16u)) = v;
+}
+
 // f with every field pointer moved by off elements (null stays null).
 template <typename T>
 __host__ __device__ __forceinline__ F<T> shifted(const F<T> &f, long off) {

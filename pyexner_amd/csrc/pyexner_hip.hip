@@ -357,7 +357,10 @@ struct BcSel {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restrict__ ent, BcSel sel, pxh_bc bc) {
+// sq: the element stride of s.h, hu, hv, z_b (4 where they are the packed
+// records of PXH_PACKED_STAGE, else 1).
+__global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restrict__ ent, BcSel sel, pxh_bc bc,
+                                            int sq) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= sel.off[sel.n]) return;
     // the range holding k, with constant indices into sel (scalar registers,
@@ -370,25 +373,26 @@ __global__ __launch_bounds__(256) void k_bc(F<T> s, const pxh_bc_entry *__restri
     const pxh_bc_entry e = be.first();
     if (!((bc.group_mask >> e.group) & 1u)) return;
     if (e.nchain & PXH_BC_VALUE) {
-        bc_value_apply<T>(s, e, bc);
+        bc_value_apply<T>(s, e, bc, sq);
         return;
     }
     const int t = e.target, src = e.src;
+    const long tq = long(t) * sq, sqi = long(src) * sq;
     switch (e.group) {
-        case PXH_G_H: s.h[t] = s.h[src]; break;
+        case PXH_G_H: s.h[tq] = s.h[sqi]; break;
         case PXH_G_Z: s.z[t] = s.z[src]; break;
         case PXH_G_ZB:
-            s.z_b[t] = s.z_b[src];
+            s.z_b[tq] = s.z_b[sqi];
             if (s.z_b_lo) s.z_b_lo[t] = s.z_b_lo[src];
             break;
         case PXH_G_N: s.n[t] = s.n[src]; break;
         case PXH_G_NB: s.n_b[t] = s.n_b[src]; break;
         case PXH_G_G: s.G[t] = s.G[src]; break;
         case PXH_G_HUV: {
-            T hu = s.hu[src], hv = s.hv[src];
+            T hu = s.hu[sqi], hv = s.hv[sqi];
             for (int q = 0; q < e.nchain; ++q) bc_transform<T>(bc.handlers[be.chain(q)], hu, hv);
-            s.hu[t] = hu;
-            s.hv[t] = hv;
+            s.hu[tq] = hu;
+            s.hv[tq] = hv;
         } break;
         default: break;
     }
@@ -420,7 +424,10 @@ static T *seam_cols(const pxh_grid *g, void *ws) {
 }
 
 template <typename T>
-static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_t groups, hipStream_t st) {
+// sq: the element stride of s.h, hu, hv, z_b (4 for the packed records of
+// PXH_PACKED_STAGE).
+static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_t groups, hipStream_t st,
+                     int sq = 1) {
     if (!bc || bc->n_entries <= 0) return PXH_OK;
     pxh_bc b = *bc;
     b.group_mask &= groups;
@@ -434,7 +441,8 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
             sel.start[0] = bc->level_start[L];
             sel.off[1] = bc->level_start[L + 1] - bc->level_start[L];
             if (sel.off[1] <= 0) continue;
-            hipLaunchKernelGGL(k_bc<T>, dim3((sel.off[1] + 255) / 256), dim3(256), 0, st, s, bc->entries, sel, b);
+            hipLaunchKernelGGL(k_bc<T>, dim3((sel.off[1] + 255) / 256), dim3(256), 0, st, s, bc->entries, sel, b,
+                               sq);
             if (int e = launch_status()) return e;
         }
         return PXH_OK;
@@ -454,7 +462,7 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
         sel.off[1] = bc->n_entries;
     }
     const int n = sel.off[sel.n];
-    hipLaunchKernelGGL(k_bc<T>, dim3((n + 255) / 256), dim3(256), 0, st, s, bc->entries, sel, b);
+    hipLaunchKernelGGL(k_bc<T>, dim3((n + 255) / 256), dim3(256), 0, st, s, bc->entries, sel, b, sq);
     return launch_status();
 }
 
@@ -473,8 +481,11 @@ static int bc_launch(const pxh_grid *g, const F<T> &s, const pxh_bc *bc, uint32_
 //     folded with a share of the stage sweep's minima, in partial[nf, nf +
 //     nbd), followed by the final reduction of those nbd minima into *dtp.
 template <typename T, bool FAST>
+// packed: the first stage's output (out without old, in with it) keeps its
+// h, hu, hv, z_b as the four-float records of PXH_PACKED_STAGE at its h.
 static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, const F<T> *old, const uint8_t *m0,
-                         const pxh_sediments *sd, const T *dtp, int write_G, T *part, hipStream_t st) {
+                         const pxh_sediments *sd, const T *dtp, int write_G, T *part, hipStream_t st,
+                         bool packed = false) {
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     const F<T> none{};
@@ -484,20 +495,22 @@ static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, con
     const int w1 = PXH_STAGE1_WAVES(g->flags), w2 = PXH_STAGE2_WAVES(g->flags);
     if (!old)
         launch_stage<T, FAST, 0, false>(fg, st, in, out, none, none, m0, dtp, ny, nx, sr, dx, sed_k(*sd), write_G,
-                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w1, zu);
+                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w1, zu, packed);
     else if (part)
         launch_stage<T, FAST, 1, true>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1, part,
-                                       seam_cols<T>(g, part), nn, w2, zu);
+                                       seam_cols<T>(g, part), nn, w2, zu, packed);
     else
         launch_stage<T, FAST, 1, false>(fg, st, in, none, out, *old, m0, dtp, ny, nx, sr, dx, sed_k(*sd), 1,
-                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w2, zu);
+                                        static_cast<T *>(nullptr), static_cast<T *>(nullptr), nn, w2, zu, packed);
     return launch_status();
 }
 
 template <typename T, bool FAST>
+// hs / zs / os: the element strides of hyd's h, hu, hv / of bed.z_b / of
+// out.z_b (4 where they are the packed records of PXH_PACKED_STAGE).
 static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, const F<T> &out, const F<T> *old,
                          const uint8_t *m0, const pxh_sediments *sd, T *dtp, int write_G, double cfl, T *part,
-                         hipStream_t st) {
+                         hipStream_t st, int hs = 1, int zs = 1, int os = 1) {
     const int ny = g->ny, nx = g->nx;
     const T dx = static_cast<T>(g->dx);
     const F<T> none{};
@@ -505,11 +518,11 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
     const int nbb = int((nband + 255) / 256);  // >= 1: a non-empty grid has a frame
     if (!old) {
         hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 0>), dim3(nbb), dim3(256), 0, st, hyd, bed, out, none, m0, dtp, ny,
-                           nx, dx, sed_k(*sd), write_G, nband);
+                           nx, dx, sed_k(*sd), write_G, nband, hs, zs, os);
         return launch_status();
     }
     hipLaunchKernelGGL((k_rx_band_bed<T, FAST, 1>), dim3(nbb), dim3(256), 0, st, hyd, bed, none, *old, m0, dtp, ny, nx,
-                       dx, sed_k(*sd), 1, nband);
+                       dx, sed_k(*sd), 1, nband, hs, zs, os);
     if (int e = launch_status()) return e;
     if (!part) return PXH_OK;
     const int sr = strip_rows(g->flags);
@@ -529,11 +542,16 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
 // G + Exner -> fill (z_b, G).  With a frame-local boundary program: stage
 // sweep -> fill (h, hu, hv) -> frame sweep; the caller fills (z_b, G) after.
 template <typename T, bool FAST>
+// packed: out keeps h, hu, hv, z_b as the four-float records of
+// PXH_PACKED_STAGE, out.hu / hv / z_b pointing at out.h + 1 / 2 / 3.
 static int rx_stage0_fused(const pxh_grid *g, const F<T> &in, const F<T> &out, const uint8_t *m0,
-                           const pxh_sediments *sd, const pxh_bc *bc, const T *dtp, int write_G, hipStream_t st) {
-    if (int e = stage_sweep_f<T, FAST>(g, in, out, nullptr, m0, sd, dtp, write_G, nullptr, st)) return e;
-    if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st)) return e;
-    return frame_sweep_f<T, FAST>(g, out, in, out, nullptr, m0, sd, const_cast<T *>(dtp), write_G, 0.0, nullptr, st);
+                           const pxh_sediments *sd, const pxh_bc *bc, const T *dtp, int write_G, hipStream_t st,
+                           bool packed = false) {
+    const int sq = packed ? 4 : 1;
+    if (int e = stage_sweep_f<T, FAST>(g, in, out, nullptr, m0, sd, dtp, write_G, nullptr, st, packed)) return e;
+    if (int e = bc_launch<T>(g, out, bc, (1u << PXH_G_H) | (1u << PXH_G_HUV), st, sq)) return e;
+    return frame_sweep_f<T, FAST>(g, out, in, out, nullptr, m0, sd, const_cast<T *>(dtp), write_G, 0.0, nullptr, st,
+                                  sq, 1, sq);
 }
 
 // The fused stage addresses a strip's rows with 32-bit byte offsets from the
@@ -639,13 +657,23 @@ static int rx_ssprk2_t(const pxh_grid *g, const pxh_fields *a_, const pxh_fields
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (sd->mpm ? 0u : (1u << PXH_G_G));
     if (frame_fused(g, bc)) {
+        // PXH_PACKED_STAGE (float32 hardware math): b's h, hu, hv, z_b as
+        // four-float records at b.h, which holds 4 ny nx elements
+        const bool pk = FAST && sizeof(T) == 4 && (g->flags & PXH_PACKED_STAGE);
+        F<T> bq = b;
+        if (pk) {
+            bq.hu = b.h + 1;
+            bq.hv = b.h + 2;
+            bq.z_b = b.h + 3;
+        }
+        const int sq = pk ? 4 : 1;
         // stage 1: b = S(a)
-        if (int e = rx_stage0_fused<T, FAST>(g, a, b, m0, sd, bc, dtp, sd->mpm ? 0 : 1, st)) return e;
-        if (int e = bc_launch<T>(g, b, bc, BED, st)) return e;
+        if (int e = rx_stage0_fused<T, FAST>(g, a, bq, m0, sd, bc, dtp, sd->mpm ? 0 : 1, st, pk)) return e;
+        if (int e = bc_launch<T>(g, bq, bc, BED, st, sq)) return e;
         // stage 2: c = S(b), a = 0.5 a + 0.5 c, and the dt of the new a
-        if (int e = stage_sweep_f<T, FAST>(g, b, c, &a, m0, sd, dtp, 1, part, st)) return e;
+        if (int e = stage_sweep_f<T, FAST>(g, bq, c, &a, m0, sd, dtp, 1, part, st, pk)) return e;
         if (int e = bc_launch<T>(g, c, bc, HY, st)) return e;
-        if (int e = frame_sweep_f<T, FAST>(g, c, b, c, &a, m0, sd, dtp, 1, cfl, part, st)) return e;
+        if (int e = frame_sweep_f<T, FAST>(g, c, bq, c, &a, m0, sd, dtp, 1, cfl, part, st, 1, sq, 1)) return e;
         return bc_launch<T>(g, a, bc, (1u << PXH_G_ZB) | (1u << PXH_G_G), st);
     }
     hipLaunchKernelGGL((k_rx_hydro_strip<T, FAST>), hg, sb, 0, st, a, b, m0, dtp, ny, nx, dx, 1);

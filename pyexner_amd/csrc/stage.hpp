@@ -148,11 +148,13 @@ constexpr int ST_NN = 1, ST_ZU = 2;
 // are template parameters so that the row refill has no branches inside it (a
 // branch there makes the compiler copy the fresh mask byte behind a wait for
 // every outstanding load).
-template <typename T, bool LO, bool MPM, int ST = 0>
+// PK: in's h, hu, hv, z_b are the packed records at pq (PXH_PACKED_STAGE).
+template <typename T, bool LO, bool MPM, int ST = 0, bool PK = false>
 __device__ __forceinline__ SRow<T> sr_load(const F<T> &in, const T *zb_lo_in, const T *G_in, const uint8_t *m0,
-                                           unsigned ro, unsigned c, T zu = T(0)) {
+                                           unsigned ro, unsigned c, T zu = T(0), const T *pq = nullptr) {
     SRow<T> q;
-    if constexpr ((ST & ST_NN) && (ST & ST_ZU)) q.h = hr_load_static(in, ro, c, zu);
+    if constexpr (PK) q.h = hr_load_q<(ST & ST_NN) != 0, (ST & ST_NN) && (ST & ST_ZU)>(in, pq, m0, ro, c, zu);
+    else if constexpr ((ST & ST_NN) && (ST & ST_ZU)) q.h = hr_load_static(in, ro, c, zu);
     else if constexpr (ST & ST_NN) q.h = hr_load_no_nodata(in, ro, c);
     else q.h = hr_load(in, m0, ro, c);
     q.zl = T(0);
@@ -180,7 +182,7 @@ static inline long seam_col_elems(int ny, int nx) { return long((nx + FO - 1) / 
 // Everything by value: a reference to a kernel argument reaches the inlined
 // body as a generic pointer, and the fields read through it land in vector
 // registers.
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST = 0>
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST = 0, bool PK = false>
 __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, const F<T> out, const F<T> cb,
                                              const F<T> old, const uint8_t *__restrict__ m0,
                                              const T *__restrict__ dtp, int ny, int nx, T dx, const SedK &sd,
@@ -219,15 +221,24 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
         const F<T> ins = shifted(in, sh), outs = shifted(out, sh), cbs = shifted(cb, sh), olds = shifted(old, sh);
         const uint8_t *m0s = m0 + sh;
         auto row = [&](int r) { return unsigned((r <= rlast ? r : rlast) - b0) * unsigned(nx); };
-        HCarry<T> hc = hcarry_at<T, FAST>(ins, m0s, row(hr0), row(b0), hr0 > 0, ucc, dx);
+        // PK (PXH_PACKED_STAGE): the first stage's output keeps h, hu, hv, z_b
+        // as one record of four per cell at out.h (MODE 0 writes the records,
+        // MODE 1 reads them from in.h), moved to row b0 like the other fields
+        constexpr bool PKI = PK && MODE == 1, PKO = PK && MODE == 0;
+        const T *qin = nullptr;
+        T *qout = nullptr;
+        if constexpr (PKI) qin = in.h + 4 * sh;
+        if constexpr (PKO) qout = out.h + 4 * sh;
+        T ph = zero, phu = zero, phv = zero;  // PKO: the hydro of row k-1, stored with its bed
+        HCarry<T> hc = hcarry_at<T, FAST, unsigned, PKI>(ins, m0s, row(hr0), row(b0), hr0 > 0, ucc, dx, qin);
         T cur_zl = zero, cur_G = zero;  // bed inputs of the current hydro row
         {
-            const SRow<T> q0 = sr_load<T, LO, MPM, ST>(ins, ins.z_b_lo, ins.G, m0s, row(hr0), ucc, zu);
+            const SRow<T> q0 = sr_load<T, LO, MPM, ST, PKI>(ins, ins.z_b_lo, ins.G, m0s, row(hr0), ucc, zu, qin);
             cur_zl = q0.zl;
             cur_G = q0.G;
         }
         // the next hydro row in flight (row k+1 at hydro row k)
-        SRow<T> q1 = sr_load<T, LO, MPM, ST>(ins, ins.z_b_lo, ins.G, m0s, row(hr0 + 1), ucc, zu);
+        SRow<T> q1 = sr_load<T, LO, MPM, ST, PKI>(ins, ins.z_b_lo, ins.G, m0s, row(hr0 + 1), ucc, zu, qin);
         // MODE 1: the previous-step row k-1 in flight at hydro row k
         OR<T> o1 = {zero, zero, zero, zero, zero, zero};
         if (MODE == 1) o1 = or_load(olds, row(b0), ucc);
@@ -248,16 +259,19 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
             }
             const HD<T> nxt = hd_derive<T, FAST>(q1.h);
             const T nxt_zl = q1.zl, nxt_G = q1.G;
-            q1 = sr_load<T, LO, MPM, ST>(ins, ins.z_b_lo, ins.G, m0s, row(k + 2), ucc, zu);  // clamped
+            q1 = sr_load<T, LO, MPM, ST, PKI>(ins, ins.z_b_lo, ins.G, m0s, row(k + 2), ucc, zu, qin);  // clamped
             // hydro row k
             const T zb_k = hc.cur.zb;
             const int nod_k = hc.cur.flags & 2;
             T h1, hu1, hv1;
-            hydro_row<T, FAST>(hc, nxt, k, hr0, c, p.hval, inE, inW, ny, nx, dx, dt, dtdx, ins, m0s, sh, h1, hu1, hv1);
+            hydro_row<T, FAST, PKI>(hc, nxt, k, hr0, c, p.hval, inE, inW, ny, nx, dx, dt, dtdx, ins, m0s, sh, h1, hu1,
+                                    hv1, qin);
             const bool own_k = p.out && k >= p.r0 && k < p.r1;
             if (own_k) {
                 const unsigned ro = row(k);
-                if (MODE == 0) {
+                if (MODE == 0 && PKO) {
+                    // stored with the row's bed on the next row (below)
+                } else if (MODE == 0) {
                     str(outs.h, ro, uc, h1);
                     str(outs.hu, ro, uc, hu1);
                     str(outs.hv, ro, uc, hv1);
@@ -279,6 +293,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                 if (ed_active(edp, edk)) fy = ex_edge_d<1, T, FAST>(edp, edk);
                 const int rb = k - 1;
                 const bool rowI = rb >= fr.top && rb < fr.bot && rb >= p.r0 && rb < p.r1;
+                T zst = edp.zb;  // PKO: the bed stored with row rb (frame cells: the frame sweep's to write)
                 if (rowI) {
                     const ED<T> east = ed_shdn(edp);
                     T fx = zero;
@@ -295,7 +310,8 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                         T zb1 = edp.zb - inc, zl1 = zero;
                         if constexpr (LO) cb_sub(edp.zb, edp.zl, inc, zb1, zl1);
                         if (MODE == 0) {
-                            str(outs.z_b, i, uc, zb1);
+                            if constexpr (PKO) zst = zb1;
+                            else str(outs.z_b, i, uc, zb1);
                             if constexpr (LO) str(outs.z_b_lo, i, uc, zl1);
                             if (write_G) str(outs.G, i, uc, edp.G);
                         } else {
@@ -332,6 +348,14 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                     }
                 }
                 fn = fy;
+                if constexpr (PKO) {
+                    if (p.out && rb >= p.r0 && rb < p.r1) str4(qout, row(rb), uc, f4v{ph, phu, phv, zst});
+                }
+            }
+            if constexpr (PKO) {
+                ph = h1;
+                phu = hu1;
+                phv = hv1;
             }
             edp = edk;
             // lockstep with the block's other strips (every wave of the block
@@ -349,6 +373,12 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
                 body(k + 1);
             }
             if (k < hr1) body(k);
+            if constexpr (PKO) {
+                // the last hydro row has no row below it in this strip: owned
+                // only on the grid's last row, a frame row
+                const int kl = hr1 - 1;
+                if (p.out && kl >= p.r0 && kl < p.r1) str4(qout, row(kl), uc, f4v{ph, phu, phv, edp.zb});
+            }
         } else {
             for (int k = hr0; k < hr1; ++k) body(k);
         }
@@ -368,7 +398,7 @@ __device__ __forceinline__ T stage_strip_run(const StagePos p, const F<T> in, co
     return vmin;
 }
 
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST>
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST, bool PK>
 __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                              const uint8_t *__restrict__ m0,
                                                              const T *__restrict__ dtp, int ny, int nx, int sr,
@@ -376,8 +406,8 @@ __global__ __launch_bounds__(SW *FWPB) void k_rx_stage_strip(F<T> in, F<T> out, 
                                                              T *__restrict__ partial, T *__restrict__ xs) {
     __shared__ T red[FWPB];
     const StagePos p = stage_pos(ny, nx, sr);
-    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, ST>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd, write_G,
-                                                             xs);
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, ST, PK>(p, in, out, cb, old, m0, dtp, ny, nx, dx, sd,
+                                                                 write_G, xs);
     if (MODE == 1 && DT) {
         vmin = wave_block_min_n<FWPB>(vmin, red);
         if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = vmin;
@@ -425,18 +455,32 @@ static size_t wave_cap_lds(K *kernel, int waves) {
 // (the uniform-z instances exist for float64 and the float32 hardware-math
 // path; the IEEE float32 path streams z).  In a compensated run every set
 // (in, out, old) has z_b_lo.  waves: the occupancy cap (wave_cap_lds).
+// packed (float32 hardware math; first stage with out, second with in the
+// first stage's output): that output's h, hu, hv, z_b are four-float records
+// at its h (PXH_PACKED_STAGE).
 template <typename T, bool FAST, int MODE, bool DT>
 static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &out, const F<T> &cb,
                          const F<T> &old, const uint8_t *m0, const T *dtp, int ny, int nx, int sr, T dx,
                          const SedK &sd, int write_G, T *partial, T *xs, bool no_nodata, int waves = 0,
-                         bool z_uniform = false) {
+                         bool z_uniform = false, bool packed = false) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
     constexpr bool has_zu = FAST || sizeof(T) == 8;
-#define PXH_STAGE(LO_, MPM_, ST_)                                                                                  \
+    constexpr bool has_pk = FAST && sizeof(T) == 4;
+#define PXH_STAGE_PK(LO_, MPM_, ST_, PK_)                                                                          \
     do {                                                                                                           \
-        auto *kern = &k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_, ST_>;                                          \
+        auto *kern = &k_rx_stage_strip<T, FAST, MODE, DT, LO_, MPM_, ST_, PK_>;                                     \
         hipLaunchKernelGGL(kern, grid, dim3(SW * FWPB), wave_cap_lds(kern, waves), st, in, out, cb, old, m0, dtp, \
                            ny, nx, sr, dx, sd, write_G, partial, xs);                                               \
+    } while (0)
+#define PXH_STAGE(LO_, MPM_, ST_)                      \
+    do {                                               \
+        if constexpr (has_pk) {                        \
+            if (packed) {                              \
+                PXH_STAGE_PK(LO_, MPM_, ST_, true);    \
+                break;                                 \
+            }                                          \
+        }                                              \
+        PXH_STAGE_PK(LO_, MPM_, ST_, false);           \
     } while (0)
 #define PXH_STAGE_NN(LO_, MPM_)                                                 \
     do {                                                                        \
@@ -460,6 +504,7 @@ static void launch_stage(dim3 grid, hipStream_t st, const F<T> &in, const F<T> &
     else PXH_STAGE_NN(false, false);
 #undef PXH_STAGE_NN
 #undef PXH_STAGE
+#undef PXH_STAGE_PK
 }
 
 // ---------------------------------------------------------------------------
@@ -472,11 +517,13 @@ __device__ __forceinline__ T ldl(const T *p, long i) {
 }
 
 template <typename T, bool FAST>
+// hs / zs: the element strides of hyd's h, hu, hv and of zb_in (4 for the
+// packed records of PXH_PACKED_STAGE, else 1).
 __device__ __forceinline__ ED<T> ed_load_lane(const F<T> &hyd, const T *zb_in, const T *zb_lo_in, const T *G_in,
-                                              const uint8_t *m0, const SedK &sd, long i) {
+                                              const uint8_t *m0, const SedK &sd, long i, int hs = 1, int zs = 1) {
     ER<T> q;
-    q.h = ldl(hyd.h, i); q.hu = ldl(hyd.hu, i); q.hv = ldl(hyd.hv, i);
-    q.zb = ldl(zb_in, i);
+    q.h = ldl(hyd.h, i * hs); q.hu = ldl(hyd.hu, i * hs); q.hv = ldl(hyd.hv, i * hs);
+    q.zb = ldl(zb_in, i * zs);
     q.G = sd.mpm ? T(0) : ldl(G_in, i);
     q.zl = T(0);
     if constexpr (sizeof(T) == 4) {
@@ -488,39 +535,42 @@ __device__ __forceinline__ ED<T> ed_load_lane(const F<T> &hyd, const T *zb_in, c
 
 template <typename T, bool FAST>
 __device__ __forceinline__ ED<T> ed_at(const F<T> &hyd, const F<T> &bed, const uint8_t *m0, const SedK &sd,
-                                       long i) {
-    return ed_load_lane<T, FAST>(hyd, bed.z_b, bed.z_b_lo, bed.G, m0, sd, i);
+                                       long i, int hs = 1, int zs = 1) {
+    return ed_load_lane<T, FAST>(hyd, bed.z_b, bed.z_b_lo, bed.G, m0, sd, i, hs, zs);
 }
 
 // compute_G + exner_solve_2D of the frame's cells from the filled hyd and the
 // stage input bed; MODE 0: out.{z_b, z_b_lo (, G)}; MODE 1: old = 0.5 old +
 // 0.5 result over h, hu, hv, G, z_b, z_b_lo.
 template <typename T, bool FAST, int MODE>
+// hs: the element stride of hyd's h, hu, hv; zs: of bed.z_b; os: of out.z_b
+// (4 where they are the packed records of PXH_PACKED_STAGE, else 1).
 __device__ __forceinline__ void band_bed_cell(long t, const F<T> hyd, const F<T> bed, const F<T> out, const F<T> old,
                                               const uint8_t *__restrict__ m0, const T *__restrict__ dtp, int ny,
-                                              int nx, T dx, const SedK &sd, int write_G) {
+                                              int nx, T dx, const SedK &sd, int write_G, int hs = 1, int zs = 1,
+                                              int os = 1) {
     int r, c;
     frame_cell(t, ny, nx, r, c);
     const T zero = T(0), hf = T(0.5);
     const T dt = *dtp;
     const long i = long(r) * nx + c;
-    const ED<T> me = ed_at<T, FAST>(hyd, bed, m0, sd, i);
+    const ED<T> me = ed_at<T, FAST>(hyd, bed, m0, sd, i, hs, zs);
     const bool hasS = r + 1 < ny;
     T fx = zero, fw = zero, fn = zero, fy = zero;
     if (c + 1 < nx) {
-        const ED<T> e = ed_at<T, FAST>(hyd, bed, m0, sd, i + 1);
+        const ED<T> e = ed_at<T, FAST>(hyd, bed, m0, sd, i + 1, hs, zs);
         if (ed_active(me, e)) fx = ex_edge_d<0, T, FAST>(me, e);
     }
     if (c > 0) {
-        const ED<T> w = ed_at<T, FAST>(hyd, bed, m0, sd, i - 1);
+        const ED<T> w = ed_at<T, FAST>(hyd, bed, m0, sd, i - 1, hs, zs);
         if (ed_active(w, me)) fw = ex_edge_d<0, T, FAST>(w, me);
     }
     if (r > 0) {
-        const ED<T> n = ed_at<T, FAST>(hyd, bed, m0, sd, i - nx);
+        const ED<T> n = ed_at<T, FAST>(hyd, bed, m0, sd, i - nx, hs, zs);
         if (ed_active(n, me)) fn = ex_edge_d<1, T, FAST>(n, me);
     }
     if (hasS) {
-        const ED<T> s = ed_at<T, FAST>(hyd, bed, m0, sd, i + nx);
+        const ED<T> s = ed_at<T, FAST>(hyd, bed, m0, sd, i + nx, hs, zs);
         if (ed_active(me, s)) fy = ex_edge_d<1, T, FAST>(me, s);
     }
     T div = zero;
@@ -534,7 +584,7 @@ __device__ __forceinline__ void band_bed_cell(long t, const F<T> hyd, const F<T>
         if (bed.z_b_lo) cb_sub(me.zb, me.zl, inc, zb1, zl1);
     }
     if (MODE == 0) {
-        out.z_b[i] = zb1;
+        out.z_b[i * os] = zb1;
         if constexpr (sizeof(T) == 4) {
             if (out.z_b_lo) out.z_b_lo[i] = zl1;
         }
@@ -565,9 +615,9 @@ template <typename T, bool FAST, int MODE>
 __global__ __launch_bounds__(256) void k_rx_band_bed(F<T> hyd, F<T> bed, F<T> out, F<T> old,
                                                       const uint8_t *__restrict__ m0, const T *__restrict__ dtp,
                                                       int ny, int nx, T dx, SedK sd, int write_G,
-                                                      long nband) {
+                                                      long nband, int hs, int zs, int os) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    if (t < nband) band_bed_cell<T, FAST, MODE>(t, hyd, bed, out, old, m0, dtp, ny, nx, dx, sd, write_G);
+    if (t < nband) band_bed_cell<T, FAST, MODE>(t, hyd, bed, out, old, m0, dtp, ny, nx, dx, sd, write_G, hs, zs, os);
 }
 
 template <typename T, bool FAST>

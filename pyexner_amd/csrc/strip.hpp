@@ -255,6 +255,34 @@ __device__ __forceinline__ HD<T> hd_load_lane(const F<T> &s, const uint8_t *m0, 
     return hd_derive<T, FAST>(q);
 }
 
+// The packed form of a first SSPRK2 stage's output (PXH_PACKED_STAGE, float
+// only): h, hu, hv, z_b of cell i at q[4 i .. 4 i + 3]; the static fields and
+// the mask as in s.  NN: no nodata cell (the mask is not loaded); ZU: z holds
+// the one value zu (not loaded).
+template <bool NN, bool ZU, typename T>
+__device__ __forceinline__ HR<T> hr_load_q(const F<T> &s, const T *q, const uint8_t *m0, unsigned ro, unsigned c,
+                                           T zu) {
+    static_assert(sizeof(T) == 4, "packed records hold float32");
+    HR<T> r;
+    const f4v v = ldr4(reinterpret_cast<const float *>(q), ro, c);
+    r.h = v.x; r.hu = v.y; r.hv = v.z; r.zb = v.w;
+    r.z = ZU ? zu : ldr(s.z, ro, c);
+    r.n = ldr(s.n_b, ro, c);
+    r.m = NN ? 0 : ldr(m0, ro, c);
+    return r;
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ HD<T> hd_load_lane_q(const F<T> &s, const T *q, const uint8_t *m0, long i) {
+    static_assert(sizeof(T) == 4, "packed records hold float32");
+    HR<T> r;
+    const f4v v = *(const PXH_GLOBAL f4v *)(reinterpret_cast<const float *>(q) + 4 * i);
+    r.h = v.x; r.hu = v.y; r.hv = v.z; r.zb = v.w;
+    r.z = ((const PXH_GLOBAL T *)s.z)[i]; r.n = ((const PXH_GLOBAL T *)s.n_b)[i];
+    r.m = ((const PXH_GLOBAL uint8_t *)m0)[i];
+    return hd_derive<T, FAST>(r);
+}
+
 template <typename T>
 __device__ __forceinline__ HD<T> hd_shdn(const HD<T> &a) {
     HD<T> b;
@@ -482,26 +510,31 @@ __device__ __forceinline__ void edge_or_zero(bool active, const HD<T> &I, const 
 // (r, c) differ from lane to lane (each lane checks its own pending neighbour),
 // so every load here is addressed per lane.  in and m0 may be moved by sh
 // elements (to a strip's first row): cell (r, c) is then element r nx + c - sh.
-template <typename T, bool FAST>
+// PK: in's h, hu, hv, z_b are the packed records at q (moved by sh cells too).
+template <typename T, bool FAST, bool PK = false>
 __device__ __forceinline__ T h1_at(const F<T> &in, const uint8_t *m0, long sh, T dt, int ny, int nx, T dx, int r,
-                                  int c) {
+                                  int c, const T *q = nullptr) {
     const long i = long(r) * nx + c - sh;
-    const HD<T> me = hd_load_lane<T, FAST>(in, m0, i);
+    auto at = [&](long j) {
+        if constexpr (PK) return hd_load_lane_q<T, FAST>(in, q, m0, j);
+        else return hd_load_lane<T, FAST>(in, m0, j);
+    };
+    const HD<T> me = at(i);
     T acc = T(0), P[3], M[3];
     if (c + 1 < nx) {
-        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i + 1);
+        const HD<T> o = at(i + 1);
         if (hd_active(me, o)) { rx_edge_d<0, T, FAST>(me, o, dx, P, M); acc = acc + M[0]; }
     }
     if (c > 0) {
-        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i - 1);
+        const HD<T> o = at(i - 1);
         if (hd_active(o, me)) { rx_edge_d<0, T, FAST>(o, me, dx, P, M); acc = acc + P[0]; }
     }
     if (r + 1 < ny) {
-        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i + nx);
+        const HD<T> o = at(i + nx);
         if (hd_active(me, o)) { rx_edge_d<1, T, FAST>(me, o, dx, P, M); acc = acc + M[0]; }
     }
     if (r > 0) {
-        const HD<T> o = hd_load_lane<T, FAST>(in, m0, i - nx);
+        const HD<T> o = at(i - nx);
         if (hd_active(o, me)) { rx_edge_d<1, T, FAST>(o, me, dx, P, M); acc = acc + P[0]; }
     }
     return me.h - dt * acc / dx;
@@ -519,18 +552,23 @@ struct HCarry {
 
 // The carry at a first row (offset ro): that row derived, and the edge from
 // the row above it (offset ro_up) when there is one.
-template <typename T, bool FAST, typename RO>
+// PK: in's h, hu, hv, z_b are the packed records at q (moved like in).
+template <typename T, bool FAST, typename RO, bool PK = false>
 __device__ __forceinline__ HCarry<T> hcarry_at(const F<T> &in, const uint8_t *m0, RO ro, RO ro_up, bool has_up,
-                                               unsigned ucc, T dx) {
+                                               unsigned ucc, T dx, const T *q = nullptr) {
     const T zero = T(0);
+    auto load = [&](RO r) {
+        if constexpr (PK) return hd_derive<T, FAST>(hr_load_q<false, false>(in, q, m0, r, ucc, zero));
+        else return hd_load<T, FAST>(in, m0, r, ucc);
+    };
     HCarry<T> hc;
-    hc.cur = hd_load<T, FAST>(in, m0, ro, ucc);
+    hc.cur = load(ro);
     hc.pn0 = hc.pn1 = hc.pn2 = zero;
     hc.zr_n = zero;
     hc.nod_n = 0;
     hc.h1_n = zero;
     if (has_up) {
-        const HD<T> up = hd_load<T, FAST>(in, m0, ro_up, ucc);
+        const HD<T> up = load(ro_up);
         if (hd_active(up, hc.cur)) {
             T P[3], M[3];
             rx_edge_d<1, T, FAST>(up, hc.cur, dx, P, M);
@@ -555,10 +593,11 @@ __device__ __forceinline__ HCarry<T> hcarry_init(const F<T> &in, const uint8_t *
 // updated depth is complete here); inE / inW: the east / west neighbour's
 // updated depth is held by this wavefront; rfirst: the first row this
 // wavefront updated (the row above it is not held).
-template <typename T, bool FAST>
+// PK: in's h, hu, hv, z_b are the packed records at q (moved by sh cells too).
+template <typename T, bool FAST, bool PK = false>
 __device__ __forceinline__ void hydro_row(HCarry<T> &hc, const HD<T> &nxt, int r, int rfirst, int c, bool mc,
                                           bool inE, bool inW, int ny, int nx, T dx, T dt, T dtdx, const F<T> &in,
-                                          const uint8_t *m0, long sh, T &h1, T &hu1, T &hv1) {
+                                          const uint8_t *m0, long sh, T &h1, T &hu1, T &hv1, const T *q = nullptr) {
     const T T_ = C<T>(DRY_TOL), zero = T(0);
     const HD<T> &cur = hc.cur;
     const bool hasS = r + 1 < ny;
@@ -614,7 +653,7 @@ __device__ __forceinline__ void hydro_row(HCarry<T> &hc, const HD<T> &nxt, int r
                 far &= far - 1;
                 const int rr = r + (k == 2 ? -1 : (k == 3 ? 1 : 0));
                 const int cc = c + (k == 0 ? 1 : (k == 1 ? -1 : 0));
-                if (h1_at<T, FAST>(in, m0, sh, dt, ny, nx, dx, rr, cc) < T_) {
+                if (h1_at<T, FAST, PK>(in, m0, sh, dt, ny, nx, dx, rr, cc, q) < T_) {
                     if (k < 2) {
                         su = true;
                         far &= ~3;

@@ -106,15 +106,17 @@ class ExnerBackend:
         self.workspace = torch.empty(int(ws), dtype=torch.uint8, device=device)
         self.device = device
 
-    def grid_for(self, no_nodata, z_uniform=False):
+    def grid_for(self, no_nodata, z_uniform=False, packed=False):
         """The grid of a call: with no_nodata (the caller checked that mask0
         is zero in every cell) PXH_NO_NODATA is set, and the fused stage
         sweeps skip the mask loads; with z_uniform as well (the caller checked
-        that z holds one value) PXH_Z_UNIFORM, and they read z once."""
-        if not no_nodata:
+        that z holds one value) PXH_Z_UNIFORM, and they read z once; with
+        packed (b.h holds 4 ny nx elements) PXH_PACKED_STAGE."""
+        extra = (_hip.NO_NODATA | (_hip.Z_UNIFORM if z_uniform else 0)) if no_nodata else 0
+        extra |= _hip.PACKED_STAGE if packed else 0
+        if not extra:
             return self.grid
         g = self.grid
-        extra = _hip.NO_NODATA | (_hip.Z_UNIFORM if z_uniform else 0)
         key = (int(g.flags), int(g.ny), int(g.nx), extra)
         cache = self.__dict__.setdefault("_static_grids", {})
         if key not in cache:
@@ -200,11 +202,13 @@ class ExnerBackend:
         _hip.call("pxh_ssprk2_combine", self.grid, fa, fb, fo, stream)
 
     def ssprk2_into(self, fa, fb, fc, mask0_ptr, cfl, dt_ptr, stream, dt_ready=False, time=0.0, states=None,
-                    no_nodata=False, z_uniform=False):
+                    no_nodata=False, z_uniform=False, packed=False):
         """dt (unless dt_ready), two stages and the combine; a is advanced in
         place and dt_ptr receives the dt of the new a.  Both stages' ring
         fills see `time`, the step's start time (SSPRK2.py:34-35).  states =
-        (a, b, c), needed by the sequential stage."""
+        (a, b, c), needed by the sequential stage.  packed: fb.h points at
+        4 ny nx elements (PXH_PACKED_STAGE; b is then scratch whose hu, hv,
+        z_b are not written)."""
         if self.sequential:
             a, b, c = states
             if not dt_ready:
@@ -216,7 +220,7 @@ class ExnerBackend:
             return
         if self.bc_step is not None:
             self.bc_step.time = float(time)
-        _hip.call("pxh_roe_exner_ssprk2", self.grid_for(no_nodata, z_uniform), fa, fb, fc, mask0_ptr, self.sed,
+        _hip.call("pxh_roe_exner_ssprk2", self.grid_for(no_nodata, z_uniform, packed), fa, fb, fc, mask0_ptr, self.sed,
                   self.bc_step, float(cfl), dt_ptr, self.workspace.data_ptr(), int(bool(dt_ready)), stream)
 
 

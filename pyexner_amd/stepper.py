@@ -17,6 +17,18 @@ def _like(state, names):
     return state.replace(**kw)
 
 
+def _wide_h(state):
+    """state with h moved to the front of a 4 ny nx buffer: the room the
+    packed first-stage records (PXH_PACKED_STAGE) need when the buffer
+    serves as b; Forward-Euler steps, which swap a and b, read the first
+    ny nx elements as h."""
+    h = state.h
+    wide = torch.empty(4 * h.numel(), dtype=h.dtype, device=h.device)
+    v = wide[:h.numel()].view(h.shape)
+    v.copy_(h)
+    return state.replace(h=v)
+
+
 EVOLVING = ("h", "hu", "hv", "z_b", "G")
 
 
@@ -35,6 +47,12 @@ class ExnerStepper:
         self.a = state.clone()
         names = EVOLVING + (("z_b_lo",) if state.z_b_lo is not None else ())
         self.b = _like(self.a, names)
+        # float32: b's h holds the first stage's output as h, hu, hv, z_b
+        # records (one stream into the second stage instead of four); a gets
+        # the same room since Forward-Euler steps swap a and b
+        self.packed = state.h.dtype == torch.float32
+        if self.packed:
+            self.a, self.b = _wide_h(self.a), _wide_h(self.b)
         self.c = _like(self.a, names)
         self.fa, self.fb, self.fc = (fields_struct(s) for s in (self.a, self.b, self.c))
         self.dt = torch.zeros((), dtype=state.h.dtype, device=state.h.device)
@@ -61,7 +79,7 @@ class ExnerStepper:
         self.be.ssprk2_into(self.fa, self.fb, self.fc, self.m0, self.cfl, self.dt.data_ptr(), self._stream(),
                             dt_ready=self.dt_ready, time=self.time if time is None else time,
                             states=(self.a, self.b, self.c), no_nodata=self.no_nodata,
-                            z_uniform=self.z_uniform)
+                            z_uniform=self.z_uniform, packed=self.packed)
         self.dt_ready = True
 
     def fe_step(self):

@@ -288,3 +288,71 @@ def test_uniform_z_flag_changes_no_result(cuda, dtype):
     zb = dict(arr, z=np.asarray(arr["z"]) + 1e-3 * np.asarray(arr["z_b"]))
     sg, mask, solver, sc = gpu_case(mesh, zb, params, dtype)
     assert not ExnerStepper(sc.backend, sg, mask, 0.5).z_uniform
+
+
+def _packed_case(name):
+    if name == "radial":
+        return cases.radial_dambreak(nx=300, ny=260, dx=0.05, n_bumps=8, radius=4.0)
+    if name == "channel":
+        return cases.channel(nx=256, ny=96, n_bumps=6)
+    if name == "wetdry":
+        return cases.wetdry_obstacle()
+    if name == "periodic":
+        return cases.periodic_box()
+    if name == "nodata":
+        return cases.nodata_island()
+    return _value_case(name)
+
+
+@pytest.mark.parametrize("name", ["radial", "channel", "wetdry", "periodic", "nodata", "berthon", "outflux",
+                                  "steepfall"])
+def test_packed_stage_changes_no_result(cuda, name):
+    """PXH_PACKED_STAGE (float32 FAST) only changes how the first stage's
+    output is laid out in b: 24 SSPRK2 steps, a Forward-Euler step (which
+    swaps a and b, so b's wide h then serves as a) and 6 more SSPRK2 steps
+    with and without it are equal bit for bit, dt included, over reflective,
+    periodic, value (moving and offset bed, influx, outflux) rings and a
+    grid with nodata cells."""
+    mesh, arr, params = _packed_case(name)
+    runs = []
+    for pk in (True, False):
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+        st = ExnerStepper(sc.backend, sg, mask, 0.5)
+        assert st.packed and st.b.h.untyped_storage().nbytes() >= 4 * st.b.h.numel() * 4
+        st.packed = pk
+        for k in range(24):
+            st.ssprk2_step(time=0.01 * k)
+        st.fe_step()
+        for k in range(6):
+            st.ssprk2_step(time=0.25 + 0.01 * k)
+        torch.cuda.synchronize()
+        runs.append(st)
+    a, b = runs
+    for f in FIELDS_EXNER + (("z_b_lo",) if a.a.z_b_lo is not None else ()):
+        assert torch.equal(getattr(a.a, f), getattr(b.a, f)), f
+    assert torch.equal(a.dt, b.dt)
+
+
+def test_packed_stage_graph_replays_equal_eager(cuda):
+    """A hipGraph of 3 packed SSPRK2 steps, replayed 4 times with eager work
+    and synchronisations between the replays, leaves the state and dt of 12
+    eager unpacked steps bit for bit."""
+    mesh, arr, params = cases.radial_dambreak(nx=300, ny=260, dx=0.05, n_bumps=8, radius=4.0)
+    runs = []
+    for pk in (True, False):
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+        runs.append(ExnerStepper(sc.backend, sg, mask, 0.5))
+        runs[-1].packed = pk
+    g, e = runs
+    g.capture(3)  # one eager warm-up step, then 3 captured steps
+    e.ssprk2_step()
+    for _ in range(4):
+        g.replay()
+        for _ in range(3):
+            e.ssprk2_step()
+        torch.cuda.synchronize()
+        scratch = torch.empty(1 << 20, device="cuda").fill_(7.0)
+        assert float(scratch.sum()) == 7.0 * (1 << 20)
+        for f in FIELDS_EXNER + (("z_b_lo",) if g.a.z_b_lo is not None else ()):
+            assert torch.equal(getattr(g.a, f), getattr(e.a, f)), f
+        assert torch.equal(g.dt, e.dt)
