@@ -65,7 +65,9 @@ extern "C" {
  * record of four per cell (b->hu, hv, z_b are not read or written: b is
  * scratch), which the second stage reads as one stream instead of four.
  * Results are those of the same call without it; other entries and paths
- * ignore it (b->h's first ny * nx elements are then b's h). */
+ * ignore it (b->h's first ny * nx elements are then b's h).  The separate
+ * stage / frame sweep entries take the same records as a view
+ * (pxh_roe_exner_stage_sweep). */
 #define PXH_PACKED_STAGE 8
 /* flags bits 8-15: owned rows per strip of the fused stage sweep (0: 32;
  * otherwise 8..128), the launch shape pyexner_amd.autotune chooses per grid. */
@@ -337,7 +339,14 @@ int pxh_dt_finalize(const pxh_grid *grid, const void *dt_partial, double cfl, vo
  *     reduces stale minima.  Pass dt_partial to both calls or to neither.
  * flags: PXH_SWEEP_SKIP_G leaves out.G unwritten in MPM mode (the next stage
  * recomputes G; the first SSPRK2 stage uses it).  dt_partial holds
- * pxh_dt_workspace_size(grid) bytes. */
+ * pxh_dt_workspace_size(grid) bytes.
+ * Records (float32 without PXH_EXACT_F32): a pxh_fields whose hu, hv, z_b
+ * point one, two and three elements past h (h's buffer holding 4 ny nx
+ * elements) views the four-float records of PXH_PACKED_STAGE, and these
+ * entries accept one where pxh_roe_exner_ssprk2 keeps them: the first
+ * stage's out (stage sweep, then frame sweep with hyd = out) and the second
+ * stage's in (stage sweep) and bed (frame sweep); pxh_bc_apply fills such a
+ * view too.  Every other entry refuses one (PXH_E_ARG). */
 #define PXH_SWEEP_SKIP_G 1
 int pxh_roe_exner_stage_sweep(const pxh_grid *grid, const pxh_fields *in, const pxh_fields *out,
                               const pxh_fields *old, const uint8_t *mask0, const pxh_sediments *sed,
@@ -440,7 +449,8 @@ int pxh_roe_dt(const pxh_grid *grid, const pxh_fields *s, double cfl, void *dt_o
                void *workspace, void *stream);
 
 /* ---- boundaries / halos -------------------------------------------------- */
-/* Apply the composed boundary program in place (groups bc->group_mask). */
+/* Apply the composed boundary program in place (groups bc->group_mask); s
+ * may be a view of four-element records (pxh_roe_exner_stage_sweep). */
 int pxh_bc_apply(const pxh_grid *grid, const pxh_fields *s, const pxh_bc *bc, void *stream);
 
 /* Halo strips of a subdomain tile: make_halo_exchange (solvers/kernels/roe_exner.py:719-803),

@@ -245,8 +245,13 @@ static inline bool bc_ok(const pxh_bc *bc) {
     return true;
 }
 
-static inline bool fields_ok(const pxh_fields *f, bool exner) {
+// records: accept a view of four-element records (hu, hv, z_b one, two and
+// three elements past h), which the stage and frame sweeps and the ring fill
+// read; elsewhere an hu that starts within one element past h overlaps h.
+static inline bool fields_ok(const pxh_fields *f, bool exner, bool records = false) {
     if (!f || !f->h || !f->hu || !f->hv || !f->z || !f->n) return false;
+    const long d = static_cast<const char *>(f->hu) - static_cast<const char *>(f->h);
+    if (!records && d > 0 && d <= 8) return false;
     if (exner && (!f->z_b || !f->n_b || !f->G)) return false;
     return true;
 }

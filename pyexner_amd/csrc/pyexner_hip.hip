@@ -505,9 +505,9 @@ static int stage_sweep_f(const pxh_grid *g, const F<T> &in, const F<T> &out, con
     return launch_status();
 }
 
-template <typename T, bool FAST>
 // hs / zs / os: the element strides of hyd's h, hu, hv / of bed.z_b / of
 // out.z_b (4 where they are the packed records of PXH_PACKED_STAGE).
+template <typename T, bool FAST>
 static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, const F<T> &out, const F<T> *old,
                          const uint8_t *m0, const pxh_sediments *sd, T *dtp, int write_G, double cfl, T *part,
                          hipStream_t st, int hs = 1, int zs = 1, int os = 1) {
@@ -541,9 +541,9 @@ static int frame_sweep_f(const pxh_grid *g, const F<T> &hyd, const F<T> &bed, co
 // two ring fills: hydro + momentum corrections -> fill (h, hu, hv) ->
 // G + Exner -> fill (z_b, G).  With a frame-local boundary program: stage
 // sweep -> fill (h, hu, hv) -> frame sweep; the caller fills (z_b, G) after.
-template <typename T, bool FAST>
 // packed: out keeps h, hu, hv, z_b as the four-float records of
 // PXH_PACKED_STAGE, out.hu / hv / z_b pointing at out.h + 1 / 2 / 3.
+template <typename T, bool FAST>
 static int rx_stage0_fused(const pxh_grid *g, const F<T> &in, const F<T> &out, const uint8_t *m0,
                            const pxh_sediments *sd, const pxh_bc *bc, const T *dtp, int write_G, hipStream_t st,
                            bool packed = false) {
@@ -755,7 +755,7 @@ static int rx_bed_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_fiel
 template <typename T>
 static int rx_stage_sweep(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *out_, const pxh_fields *old_,
                           const uint8_t *m0, const pxh_sediments *sd, const void *dt, int write_G, void *partial,
-                          hipStream_t st) {
+                          hipStream_t st, bool packed) {
     const F<T> in = cast_fields<T>(in_), out = cast_fields<T>(out_);
     F<T> old{};
     if (old_) old = cast_fields<T>(old_);
@@ -763,7 +763,7 @@ static int rx_stage_sweep(const pxh_grid *g, const pxh_fields *in_, const pxh_fi
     T *part = static_cast<T *>(partial);
     if constexpr (sizeof(T) == 4) {
         if (!(g->flags & PXH_EXACT_F32))
-            return stage_sweep_f<T, true>(g, in, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, part, st);
+            return stage_sweep_f<T, true>(g, in, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, part, st, packed);
     }
     return stage_sweep_f<T, false>(g, in, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, part, st);
 }
@@ -771,7 +771,7 @@ static int rx_stage_sweep(const pxh_grid *g, const pxh_fields *in_, const pxh_fi
 template <typename T>
 static int rx_frame_sweep(const pxh_grid *g, const pxh_fields *hyd_, const pxh_fields *bed_, const pxh_fields *out_,
                           const pxh_fields *old_, const uint8_t *m0, const pxh_sediments *sd, void *dt, int write_G,
-                          double cfl, void *partial, hipStream_t st) {
+                          double cfl, void *partial, hipStream_t st, int hs, int zs, int os) {
     const F<T> hyd = cast_fields<T>(hyd_), bed = cast_fields<T>(bed_);
     F<T> out{}, old{};
     if (out_) out = cast_fields<T>(out_);
@@ -780,9 +780,11 @@ static int rx_frame_sweep(const pxh_grid *g, const pxh_fields *hyd_, const pxh_f
     T *part = static_cast<T *>(partial);
     if constexpr (sizeof(T) == 4) {
         if (!(g->flags & PXH_EXACT_F32))
-            return frame_sweep_f<T, true>(g, hyd, bed, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, cfl, part, st);
+            return frame_sweep_f<T, true>(g, hyd, bed, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, cfl, part, st,
+                                          hs, zs, os);
     }
-    return frame_sweep_f<T, false>(g, hyd, bed, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, cfl, part, st);
+    return frame_sweep_f<T, false>(g, hyd, bed, out, old_ ? &old : nullptr, m0, sd, dtp, write_G, cfl, part, st, hs,
+                                   zs, os);
 }
 
 template <typename T>
@@ -853,8 +855,20 @@ static int roe_step(const pxh_grid *g, const pxh_fields *in_, const pxh_fields *
 }
 
 template <typename T>
-static int bc_apply_t(const pxh_grid *g, const pxh_fields *s_, const pxh_bc *bc, hipStream_t st) {
-    return bc_launch<T>(g, cast_fields<T>(s_), bc, 0xffffffffu, st);
+static int bc_apply_t(const pxh_grid *g, const pxh_fields *s_, const pxh_bc *bc, hipStream_t st, int sq) {
+    return bc_launch<T>(g, cast_fields<T>(s_), bc, 0xffffffffu, st, sq);
+}
+
+// A pxh_fields whose hu, hv, z_b point one, two and three elements past h is
+// a view of four-element records (h, hu, hv, z_b of a cell side by side): the
+// layout the float32 fused step keeps its first stage's output in
+// (PXH_PACKED_STAGE), which the separate sweep entries and the ring fill
+// accept where that step makes it.
+static bool packed_view(const pxh_grid *g, const pxh_fields *f) {
+    if (!f || !f->h) return false;
+    const long es = g->dtype == PXH_F64 ? 8 : 4;
+    const char *h = static_cast<const char *>(f->h);
+    return f->hu == h + es && f->hv == h + 2 * es && f->z_b == h + 3 * es;
 }
 
 // A compensated bed is float32-only and all-or-nothing across the buffers of a call.
@@ -1384,11 +1398,12 @@ int pxh_dt_finalize(const pxh_grid *g, const void *dt_partial, double cfl, void 
 int pxh_roe_exner_stage_sweep(const pxh_grid *g, const pxh_fields *in, const pxh_fields *out, const pxh_fields *old,
                               const uint8_t *m0, const pxh_sediments *sd, const void *dt_dev, int flags,
                               void *dt_partial, void *stream) {
-    if (!grid_ok(g) || !fields_ok(in, true) || !fields_ok(out, false) || !m0 || !sd || !dt_dev) return PXH_E_ARG;
+    if (!grid_ok(g) || !fields_ok(in, true, true) || !fields_ok(out, false, true) || !m0 || !sd || !dt_dev)
+        return PXH_E_ARG;
     if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
     if (in->h == out->h || in->hu == out->hu || in->hv == out->hv) return PXH_E_ARG;
     if (!old) {
-        if (!fields_ok(out, true) || in->z_b == out->z_b || in->G == out->G || !lo_ok(g, {in, out}) ||
+        if (!fields_ok(out, true, true) || in->z_b == out->z_b || in->G == out->G || !lo_ok(g, {in, out}) ||
             (in->z_b_lo && in->z_b_lo == out->z_b_lo))
             return PXH_E_ARG;
     } else {
@@ -1400,19 +1415,27 @@ int pxh_roe_exner_stage_sweep(const pxh_grid *g, const pxh_fields *in, const pxh
                 return PXH_E_ARG;
         if (old->z_b_lo && old->z_b_lo == in->z_b_lo) return PXH_E_ARG;
     }
+    // records: the first stage's output or the second stage's input, on the
+    // float32 hardware-math path
+    const bool pin = packed_view(g, in), pout = packed_view(g, out);
+    if ((pin || pout) && (g->dtype != PXH_F32 || (g->flags & PXH_EXACT_F32) || (pin && !old) || (pout && old) ||
+                          (pin && pout)))
+        return PXH_E_ARG;
+    if (packed_view(g, old)) return PXH_E_ARG;
     if (!stage_fits(g)) return PXH_E_RANGE;
     const int write_G = (flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
     return DISPATCH(g, rx_stage_sweep, g, in, out, old, m0, sd, dt_dev, write_G, old ? dt_partial : nullptr,
-                    static_cast<hipStream_t>(stream));
+                    static_cast<hipStream_t>(stream), pin || pout);
 }
 
 int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const pxh_fields *bed, const pxh_fields *out,
                               const pxh_fields *old, const uint8_t *m0, const pxh_sediments *sd, void *dt_dev,
                               int flags, double cfl, void *dt_partial, void *stream) {
-    if (!grid_ok(g) || !fields_ok(hyd, false) || !fields_ok(bed, true) || !m0 || !sd || !dt_dev) return PXH_E_ARG;
+    if (!grid_ok(g) || !fields_ok(hyd, false, true) || !fields_ok(bed, true, true) || !m0 || !sd || !dt_dev)
+        return PXH_E_ARG;
     if (sd->n_sed < 0 || sd->n_sed > PXH_MAX_SED || (sd->mpm && sd->n_sed < 1)) return PXH_E_RANGE;
     if (!old) {
-        if (!fields_ok(out, true) || bed->z_b == out->z_b || bed->G == out->G || !lo_ok(g, {bed, out}) ||
+        if (!fields_ok(out, true, true) || bed->z_b == out->z_b || bed->G == out->G || !lo_ok(g, {bed, out}) ||
             (bed->z_b_lo && bed->z_b_lo == out->z_b_lo))
             return PXH_E_ARG;
     } else {
@@ -1422,9 +1445,15 @@ int pxh_roe_exner_frame_sweep(const pxh_grid *g, const pxh_fields *hyd, const px
             if (q == hyd->h || q == hyd->hu || q == hyd->hv || q == bed->z_b || q == bed->G) return PXH_E_ARG;
         if (old->z_b_lo && old->z_b_lo == bed->z_b_lo) return PXH_E_ARG;
     }
+    // records as the fused step has them: first stage hyd = out records (or
+    // neither), bed plain; second stage hyd plain, bed records or plain
+    const bool ph = packed_view(g, hyd), pb = packed_view(g, bed);
+    if (!old && (pb || ph != packed_view(g, out))) return PXH_E_ARG;
+    if (old && (ph || packed_view(g, old))) return PXH_E_ARG;
     const int write_G = (flags & PXH_SWEEP_SKIP_G) && sd->mpm ? 0 : 1;
     return DISPATCH(g, rx_frame_sweep, g, hyd, bed, old ? nullptr : out, old, m0, sd, dt_dev, write_G, cfl,
-                    old ? dt_partial : nullptr, static_cast<hipStream_t>(stream));
+                    old ? dt_partial : nullptr, static_cast<hipStream_t>(stream), ph ? 4 : 1, pb ? 4 : 1,
+                    !old && ph ? 4 : 1);
 }
 
 int64_t pxh_multi_stage_blocks(const pxh_grid *g) {
@@ -1635,7 +1664,7 @@ int pxh_roe_dt(const pxh_grid *g, const pxh_fields *s, double cfl, void *dt_out,
 int pxh_bc_apply(const pxh_grid *g, const pxh_fields *s, const pxh_bc *bc, void *stream) {
     if (!grid_ok(g) || !s || !bc) return PXH_E_ARG;
     if (!bc_ok(bc)) return PXH_E_ARG;
-    return DISPATCH(g, bc_apply_t, g, s, bc, static_cast<hipStream_t>(stream));
+    return DISPATCH(g, bc_apply_t, g, s, bc, static_cast<hipStream_t>(stream), packed_view(g, s) ? 4 : 1);
 }
 
 int64_t pxh_halo_bytes(const pxh_grid *g, const pxh_fields *s, int hw, uint32_t groups, int sides) {

@@ -32,6 +32,18 @@ def _wide_h(state):
 EVOLVING = ("h", "hu", "hv", "z_b", "G")
 
 
+def record_fields(state):
+    """pxh_fields of state with h, hu, hv, z_b as the view of the four-element
+    records that PXH_PACKED_STAGE keeps at state.h (a _wide_h buffer): hu, hv,
+    z_b one, two and three elements past h.  The separate stage / frame sweep
+    entries and pxh_bc_apply take it where the fused step keeps its first
+    stage's output."""
+    f = fields_struct(state)
+    es = state.h.element_size()
+    f.hu, f.hv, f.z_b = f.h + es, f.h + 2 * es, f.h + 3 * es
+    return f
+
+
 class ExnerStepper:
     """Roe-Exner stepping on an initialised state (init_fn already applied)."""
 

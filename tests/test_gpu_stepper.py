@@ -356,3 +356,42 @@ def test_packed_stage_graph_replays_equal_eager(cuda):
         for f in FIELDS_EXNER + (("z_b_lo",) if g.a.z_b_lo is not None else ()):
             assert torch.equal(getattr(g.a, f), getattr(e.a, f)), f
         assert torch.equal(g.dt, e.dt)
+
+
+def test_separate_entries_on_record_views_equal_packed_step(cuda):
+    """The launches of the fused step issued one entry at a time
+    (pxh_roe_exner_stage_sweep, pxh_bc_apply, pxh_roe_exner_frame_sweep) with
+    the first stage's output as a record view (record_fields) leave the state
+    and dt of pxh_roe_exner_ssprk2 with PXH_PACKED_STAGE bit for bit, 12 steps
+    of the radial dam break (float32 FAST, compensated bed)."""
+    from pyexner_amd.stepper import record_fields
+    mesh, arr, params = cases.radial_dambreak(nx=300, ny=260, dx=0.05, n_bumps=8, radius=4.0)
+    runs = []
+    for _ in range(2):
+        sg, mask, solver, sc = gpu_case(mesh, arr, params, torch.float32)
+        runs.append((ExnerStepper(sc.backend, sg, mask, 0.5), sc))
+    (ref, _), (st, sc) = runs
+    assert ref.packed and st.packed and sc.boundaries.frame_local
+    be, bm, m0 = sc.backend, sc.boundaries, st.m0
+    stream = torch.cuda.current_stream().cuda_stream
+    g = be.grid_for(st.no_nodata, st.z_uniform)
+    G_H, G_HUV, G_ZB, G_G = _hip.G_H, _hip.G_HUV, _hip.G_ZB, _hip.G_G
+    hy, bed1 = bm.program((G_H, G_HUV)), bm.program((G_ZB,) if be.sed.mpm else (G_ZB, G_G))
+    end = bm.program((G_ZB, G_G))
+    fa, fb, fc, dtp, part = st.fa, record_fields(st.b), st.fc, st.dt.data_ptr(), be.workspace.data_ptr()
+    be.dt_into(fa, m0, 0.5, dtp, stream)
+    for _ in range(12):
+        ref.ssprk2_step()
+        _hip.call("pxh_roe_exner_stage_sweep", g, fa, fb, None, m0, be.sed, dtp, _hip.SWEEP_SKIP_G, None, stream)
+        _hip.call("pxh_bc_apply", g, fb, hy, stream)
+        _hip.call("pxh_roe_exner_frame_sweep", g, fb, fa, fb, None, m0, be.sed, dtp, _hip.SWEEP_SKIP_G, 0.0, None,
+                  stream)
+        _hip.call("pxh_bc_apply", g, fb, bed1, stream)
+        _hip.call("pxh_roe_exner_stage_sweep", g, fb, fc, fa, m0, be.sed, dtp, 0, part, stream)
+        _hip.call("pxh_bc_apply", g, fc, hy, stream)
+        _hip.call("pxh_roe_exner_frame_sweep", g, fc, fb, None, fa, m0, be.sed, dtp, 0, 0.5, part, stream)
+        _hip.call("pxh_bc_apply", g, fa, end, stream)
+    torch.cuda.synchronize()
+    for f in FIELDS_EXNER + (("z_b_lo",) if st.a.z_b_lo is not None else ()):
+        assert torch.equal(getattr(ref.a, f), getattr(st.a, f)), f
+    assert torch.equal(ref.dt, st.dt)

@@ -616,3 +616,51 @@ def test_multi_domain_sweeps_check_their_arguments():
     assert lib.pxh_roe_exner_frame_sweep_multi(_hip.PXH_F64, 0, None, 1, 8, 0, sed, p, 0, None) == _hip.PXH_E_ARG
     sed.mpm, sed.n_sed = 1, 0  # MPM without a sediment class
     assert lib.pxh_roe_exner_frame_sweep_multi(_hip.PXH_F64, 0, p, 1, 8, 0, sed, p, 0, None) == _hip.PXH_E_RANGE
+
+
+def _record_fields(base, es, packed):
+    """pxh_fields of made-up addresses (never dereferenced): h, hu, hv, z_b as
+    one record view (one, two, three elements past h) or as separate arrays."""
+    f = _hip.Fields()
+    step = es if packed else 1 << 20
+    f.h, f.hu, f.hv, f.z_b = base, base + step, base + 2 * step, base + 3 * step
+    f.z, f.n, f.n_b, f.G = (base + (k << 24) for k in range(4, 8))
+    return f
+
+
+def test_record_views_only_where_the_fused_step_keeps_them():
+    """A pxh_fields whose hu, hv, z_b sit one, two, three elements past h (the
+    first stage's records of PXH_PACKED_STAGE) is accepted by the stage and
+    frame sweeps only as the first stage's output or the second stage's input
+    on the float32 hardware-math path, and refused by the other entries
+    (host-only checks: every call below fails before any launch)."""
+    lib = _hip.load()
+    sed = _hip.Sediments()
+    p = ctypes.c_void_p(16)
+    g32 = _hip.Grid(64, 64, _hip.PXH_F32, 0, 0.05)
+    gx = _hip.Grid(64, 64, _hip.PXH_F32, _hip.EXACT_F32, 0.05)
+    g64 = _hip.Grid(64, 64, _hip.PXH_F64, 0, 0.05)
+    rec, a, c = _record_fields(1 << 40, 4, True), _record_fields(2 << 40, 4, False), _record_fields(3 << 40, 4, False)
+    rec64 = _record_fields(1 << 40, 8, True)
+    E = _hip.PXH_E_ARG
+    stage = lib.pxh_roe_exner_stage_sweep
+    frame = lib.pxh_roe_exner_frame_sweep
+    # stage sweep: records in without old (first stage), records out with old
+    # (second stage), records in and out, old as records
+    assert stage(g32, rec, a, None, p, sed, p, 0, None, None) == E
+    assert stage(g32, a, rec, c, p, sed, p, 0, None, None) == E
+    assert stage(g32, rec, rec, c, p, sed, p, 0, None, None) == E
+    assert stage(g32, a, c, rec, p, sed, p, 0, None, None) == E
+    # float64 and exact float32 keep no records
+    assert stage(g64, a, rec64, None, p, sed, p, 0, None, None) == E
+    assert stage(gx, a, rec, None, p, sed, p, 0, None, None) == E
+    # frame sweep: first stage bed as records, hyd records with a plain out;
+    # second stage hyd as records, old as records
+    assert frame(g32, rec, rec, rec, None, p, sed, p, 0, 0.5, None, None) == E
+    assert frame(g32, rec, a, c, None, p, sed, p, 0, 0.5, None, None) == E
+    assert frame(g32, rec, a, None, c, p, sed, p, 0, 0.5, None, None) == E
+    assert frame(g32, c, a, None, rec, p, sed, p, 0, 0.5, None, None) == E
+    # entries that never read records
+    assert lib.pxh_roe_exner_hydro_sweep(g32, rec, a, p, p, None) == E
+    assert lib.pxh_roe_exner_hydro_sweep(g32, a, rec, p, p, None) == E
+    assert lib.pxh_roe_exner_bed_sweep(g32, rec, a, c, None, p, sed, p, None, None) == E
