@@ -65,7 +65,9 @@ extern "C" {
  * record of four per cell (b->hu, hv, z_b are not read or written: b is
  * scratch), which the second stage reads as one stream instead of four.
  * Results are those of the same call without it; other entries and paths
- * ignore it (b->h's first ny * nx elements are then b's h).  The separate
+ * ignore it (b->h's first ny * nx elements are then b's h).  In an
+ * ensemble plan's flags it asks the same of pxh_ensemble_ssprk2: b->h holds
+ * 4 elements per arena cell, the records of every member.  The separate
  * stage / frame sweep entries take the same records as a view
  * (pxh_roe_exner_stage_sweep). */
 #define PXH_PACKED_STAGE 8

@@ -49,9 +49,25 @@ __device__ __forceinline__ int find_member(const EnsMember *__restrict__ tab, in
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
+// A member's fields at element offset off; rec: f's h, hu, hv, z_b are the
+// four-element records of PXH_PACKED_STAGE over the arena, so the member's
+// records start at f.h + 4 off, with hu, hv, z_b one, two, three elements on.
+template <typename T>
+__device__ __forceinline__ F<T> shifted_rec(const F<T> &f, long off, bool rec) {
+    F<T> r = shifted(f, off);
+    if (rec && f.h) {
+        r.h = f.h + 4 * off;
+        r.hu = r.h + 1;
+        r.hv = r.h + 2;
+        r.z_b = r.h + 3;
+    }
+    return r;
+}
+
 // ST: the static-field specialisation of stage_strip_run, a promise that
 // holds for every member (each member's uniform z is its own first cell).
-template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST>
+// PK: the first stage's output (out in MODE 0, in in MODE 1) as records.
+template <typename T, bool FAST, int MODE, bool DT, bool LO, bool MPM, int ST, bool PK = false>
 __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> cb, F<T> old,
                                                         const uint8_t *__restrict__ m0, const T *__restrict__ dts,
                                                         const EnsMember *__restrict__ tab, int n, int sr,
@@ -65,8 +81,9 @@ __global__ __launch_bounds__(SW *FWPB) void k_ens_stage(F<T> in, F<T> out, F<T> 
     const int local = lb - m.sblk0;
     const int by = local / m.sgx, bx = local - by * m.sgx;
     const StagePos p = stage_pos_at(bx, by, m.ny, m.nx, sr);
-    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, ST>(p, shifted(in, m.off), shifted(out, m.off),
-                                                             shifted(cb, m.off), shifted(old, m.off), m0 + m.off,
+    T vmin = stage_strip_run<T, FAST, MODE, DT, LO, MPM, ST, PK>(
+        p, shifted_rec(in, m.off, PK && MODE == 1), shifted_rec(out, m.off, PK && MODE == 0), shifted(cb, m.off),
+        shifted(old, m.off), m0 + m.off,
                                                              dts + i, m.ny, m.nx, static_cast<T>(m.dx), seds[i],
                                                              write_G, DT ? xs + m.xs0 : nullptr);
     if (MODE == 1 && DT) {
@@ -79,14 +96,15 @@ template <typename T, bool FAST, int MODE>
 __global__ __launch_bounds__(256) void k_ens_band_bed(F<T> hyd, F<T> bed, F<T> out, F<T> old,
                                                       const uint8_t *__restrict__ m0, const T *__restrict__ dts,
                                                       const EnsMember *__restrict__ tab, int n,
-                                                      const SedK *__restrict__ seds, int write_G) {
+                                                      const SedK *__restrict__ seds, int write_G, int hs, int zs,
+                                                      int os) {
     const int i = find_member<&EnsMember::fblk0>(tab, n, blockIdx.x);
     const EnsMember m = tab[i];
     const long t = long(blockIdx.x - m.fblk0) * 256 + threadIdx.x;
     if (t < m.nband)
-        band_bed_cell<T, FAST, MODE>(t, shifted(hyd, m.off), shifted(bed, m.off), shifted(out, m.off),
-                                     shifted(old, m.off), m0 + m.off, dts + i, m.ny, m.nx, static_cast<T>(m.dx),
-                                     seds[i], write_G);
+        band_bed_cell<T, FAST, MODE>(t, shifted_rec(hyd, m.off, hs == 4), shifted_rec(bed, m.off, zs == 4),
+                                     shifted_rec(out, m.off, os == 4), shifted(old, m.off), m0 + m.off, dts + i,
+                                     m.ny, m.nx, static_cast<T>(m.dx), seds[i], write_G, hs, zs, os);
 }
 
 // The dt minima of each member's frame edges and strip-seam edges.

@@ -1064,14 +1064,25 @@ static const EnsPlanHeader *ens_header(const void *plan) {
 template <typename T, bool FAST, int MODE, bool DT>
 static void ens_stage(const EnsPlanHeader &h, const EnsMember *tab, const SedK *seds, hipStream_t st, const F<T> &in,
                       const F<T> &out, const F<T> &cb, const F<T> &old, const uint8_t *m0, const T *dts,
-                      int write_G, T *partial, T *xs) {
+                      int write_G, T *partial, T *xs, bool pk = false) {
     const bool lo = sizeof(T) == 4 && in.z_b_lo != nullptr;
+    constexpr bool has_pk = FAST && sizeof(T) == 4;
     // the plan's PXH_NO_NODATA / PXH_Z_UNIFORM promises, as launch_stage takes the grid's
     const bool nn = (h.flags & PXH_NO_NODATA) != 0, zu = nn && (h.flags & PXH_Z_UNIFORM) != 0;
     constexpr bool has_zu = FAST || sizeof(T) == 8;
-#define PXH_ENS_ST(LO_, MPM_, ST_)                                                                                  \
-    hipLaunchKernelGGL((k_ens_stage<T, FAST, MODE, DT, LO_, MPM_, ST_>), dim3(h.sblocks), dim3(SW * FWPB), 0, st,  \
-                       in, out, cb, old, m0, dts, tab, h.n, strip_rows(h.flags), seds, write_G, partial, xs)
+#define PXH_ENS_PK(LO_, MPM_, ST_, PK_)                                                                         \
+    hipLaunchKernelGGL((k_ens_stage<T, FAST, MODE, DT, LO_, MPM_, ST_, PK_>), dim3(h.sblocks), dim3(SW * FWPB), 0, \
+                       st, in, out, cb, old, m0, dts, tab, h.n, strip_rows(h.flags), seds, write_G, partial, xs)
+#define PXH_ENS_ST(LO_, MPM_, ST_)                   \
+    do {                                             \
+        if constexpr (has_pk) {                      \
+            if (pk) {                                \
+                PXH_ENS_PK(LO_, MPM_, ST_, true);    \
+                break;                               \
+            }                                        \
+        }                                            \
+        PXH_ENS_PK(LO_, MPM_, ST_, false);           \
+    } while (0)
 #define PXH_ENS(LO_, MPM_)                              \
     do {                                                \
         if constexpr (has_zu) {                         \
@@ -1094,6 +1105,7 @@ static void ens_stage(const EnsPlanHeader &h, const EnsMember *tab, const SedK *
     else PXH_ENS(false, false);
 #undef PXH_ENS
 #undef PXH_ENS_ST
+#undef PXH_ENS_PK
 }
 
 template <typename T, bool FAST>
@@ -1119,21 +1131,31 @@ static int ens_ssprk2_t(const EnsPlanHeader &h, const char *pd, const pxh_fields
         if (int e = ens_dt_t<T, FAST>(h, pd, a, m0, cfl, dts, part, st)) return e;
     const uint32_t HY = (1u << PXH_G_H) | (1u << PXH_G_HUV);
     const uint32_t BED = (1u << PXH_G_ZB) | (h.mpm ? 0u : (1u << PXH_G_G));
+    // PXH_PACKED_STAGE (float32 hardware math): b's h, hu, hv, z_b as
+    // four-float records over the arena at b.h, which holds 4 cells' worth
+    const bool pk = FAST && sizeof(T) == 4 && (h.flags & PXH_PACKED_STAGE);
+    F<T> bq = b;
+    if (pk) {
+        bq.hu = b.h + 1;
+        bq.hv = b.h + 2;
+        bq.z_b = b.h + 3;
+    }
+    const int sq = pk ? 4 : 1;
     // stage 1: b = S(a)
     T *xs = reinterpret_cast<T *>(reinterpret_cast<char *>(part) + ens_xs_offset(h));
-    ens_stage<T, FAST, 0, false>(h, tab, seds, st, a, b, none, none, m0, dts, h.mpm ? 0 : 1, nullptr, nullptr);
+    ens_stage<T, FAST, 0, false>(h, tab, seds, st, a, bq, none, none, m0, dts, h.mpm ? 0 : 1, nullptr, nullptr, pk);
     if (int e = launch_status()) return e;
-    if (int e = bc_launch<T>(&g, b, bc, HY, st)) return e;
-    hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 0>), dim3(h.fblocks), dim3(256), 0, st, b, a, b, none, m0, dts, tab,
-                       h.n, seds, h.mpm ? 0 : 1);
+    if (int e = bc_launch<T>(&g, bq, bc, HY, st, sq)) return e;
+    hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 0>), dim3(h.fblocks), dim3(256), 0, st, bq, a, bq, none, m0, dts,
+                       tab, h.n, seds, h.mpm ? 0 : 1, sq, 1, sq);
     if (int e = launch_status()) return e;
-    if (int e = bc_launch<T>(&g, b, bc, BED, st)) return e;
+    if (int e = bc_launch<T>(&g, bq, bc, BED, st, sq)) return e;
     // stage 2: c = S(b), a = 0.5 a + 0.5 c, and each member's dt of the new a
-    ens_stage<T, FAST, 1, true>(h, tab, seds, st, b, none, c, a, m0, dts, 1, part, xs);
+    ens_stage<T, FAST, 1, true>(h, tab, seds, st, bq, none, c, a, m0, dts, 1, part, xs, pk);
     if (int e = launch_status()) return e;
     if (int e = bc_launch<T>(&g, c, bc, HY, st)) return e;
-    hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 1>), dim3(h.fblocks), dim3(256), 0, st, c, b, none, a, m0, dts, tab,
-                       h.n, seds, 1);
+    hipLaunchKernelGGL((k_ens_band_bed<T, FAST, 1>), dim3(h.fblocks), dim3(256), 0, st, c, bq, none, a, m0, dts,
+                       tab, h.n, seds, 1, 1, sq, 1);
     if (int e = launch_status()) return e;
     hipLaunchKernelGGL((k_ens_band_dt<T, FAST>), dim3(h.tblocks), dim3(256), 0, st, a, m0, tab, h.n,
                        strip_rows(h.flags), xs, part);

@@ -27,10 +27,13 @@ class Ensemble:
     frame-local rings, one dtype; `cfl` is shared (each member its own dt)."""
 
     def __init__(self, members, dtype=torch.float32, device="cuda", cfl=0.5, exact_f32=False, keep_members=False,
-                 static=True):
+                 static=True, packed=True):
         """static: check once whether no member has a nodata cell and each
         member's z holds one value, and if so let the fused stage sweeps skip
-        those streams (PXH_NO_NODATA / PXH_Z_UNIFORM; results unchanged)."""
+        those streams (PXH_NO_NODATA / PXH_Z_UNIFORM; results unchanged).
+        packed (float32): the first stage's output keeps h, hu, hv, z_b as one
+        four-float record per cell in a 4-cell-wide b.h (PXH_PACKED_STAGE;
+        results unchanged)."""
         self.n = len(members)
         # with keep_members, (backend, state, mask) of every member set up alone
         # (for comparisons against single-domain stepping)
@@ -73,6 +76,10 @@ class Ensemble:
         self.a = states[0].replace(**arena)
         ev = EVOLVING + (("z_b_lo",) if "z_b_lo" in names else ())
         self.b = self.a.replace(**{n: torch.empty_like(arena[n]) for n in ev})
+        self.packed = bool(packed) and dtype == torch.float32
+        if self.packed:
+            wide = torch.empty(4 * total, dtype=dtype, device=device)
+            self.b = self.b.replace(h=wide[:total])
         self.c = self.a.replace(**{n: torch.empty_like(arena[n]) for n in ev})
         self.fa, self.fb, self.fc = (fields_struct(x) for x in (self.a, self.b, self.c))
         self.mask = torch.cat(masks).to(torch.uint8).contiguous()
@@ -107,9 +114,11 @@ class Ensemble:
         self._code = _hip.PXH_F32 if dtype == torch.float32 else _hip.PXH_F64
         self.no_nodata = not bool(self.mask.any())
         self.z_uniform = self.no_nodata and all(bool((s.z == s.z.reshape(-1)[0]).all()) for s in states)
-        self._flags = int(meta[0][3]) & ~(_hip.NO_NODATA | _hip.Z_UNIFORM)
+        self._flags = int(meta[0][3]) & ~(_hip.NO_NODATA | _hip.Z_UNIFORM | _hip.PACKED_STAGE)
         if static and self.no_nodata:
             self._flags |= _hip.NO_NODATA | (_hip.Z_UNIFORM if self.z_uniform else 0)
+        if self.packed:
+            self._flags |= _hip.PACKED_STAGE
         self.device = device
         self.workspace = None
         self._build_plan()

@@ -90,3 +90,22 @@ def test_ensemble_static_flags_change_no_result(cuda, dtype, uniform_z):
         assert torch.equal(runs[0].dt, runs[1].dt), k
     for f in names:
         assert torch.equal(getattr(runs[0].a, f), getattr(runs[1].a, f)), f
+
+
+@pytest.mark.parametrize("static", [True, False])
+def test_ensemble_packed_stage_changes_no_result(cuda, static):
+    """PXH_PACKED_STAGE in the plan (float32 FAST) only changes how the first
+    stage's output is laid out in b (four-float records over the arena): 30
+    SSPRK2 steps with and without it are equal bit for bit, dt and every
+    field, with and without the static-field promises."""
+    members = _members()
+    runs = [Ensemble(members, dtype=torch.float32, cfl=0.5, static=static, packed=p) for p in (True, False)]
+    assert runs[0].packed and runs[0]._flags & _hip.PACKED_STAGE and not runs[1]._flags & _hip.PACKED_STAGE
+    names = FIELDS_EXNER + (("z_b_lo",) if runs[0].a.z_b_lo is not None else ())
+    for k in range(30):
+        for e in runs:
+            e.ssprk2_step()
+        torch.cuda.synchronize()
+        assert torch.equal(runs[0].dt, runs[1].dt), k
+    for f in names:
+        assert torch.equal(getattr(runs[0].a, f), getattr(runs[1].a, f)), f
